@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Throughput of the gfx950 wavefront path tracer on the 737K-triangle workload.
+
+One step renders the whole frame: 6x6 displaced icospheres (737,290 triangles),
+4 quad lights, a 1024x512 sky + sun environment, 1024x1024 pixels, 64 samples
+per pixel, max depth 8 (adaptive sampling off, no clamps).  Inputs are resident
+in HBM before the timed region.  Prints one JSON line.
+
+  value     samples/s = W * H * spp * steps / wall time
+  roofline  the closest-hit (extend) kernel: algorithmic bytes = 48 B per
+            active closest ray (32 B ray read + 16 B hit written), divided by
+            the kernel's HIP-event time over the timed steps
+  cpu_baseline  the oracle (CPU restatement of the reference integrator) on a
+            64-row band of the same frame, on this host's cores
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--height", type=int, default=1024)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not offered: this build runs on one GPU "
+              "(the path shards by sample_offset ranges; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from photon_amd import abi, scenes
+    from photon_amd.backend import HipBackend, PathTracer, make_camera
+
+    t = time.time()
+    scene = scenes.sphere_grid(seed=1234)
+    gen_s = time.time() - t
+    pt = PathTracer(HipBackend(device=0))
+    t = time.time()
+    pt.set_scene(scene)
+    build_s = time.time() - t
+    cam = make_camera(scene.camera["look_from"], scene.camera["look_at"], scene.camera["vup"], scene.camera["vfov"],
+                      args.width / args.height)
+    pt.camera = cam
+    pt.params = abi.default_render_params(args.width, args.height, args.spp, args.depth)
+    print(f"[bench] scene {scene.triangle_count} tris generated in {gen_s:.1f}s, set_scene {build_s:.1f}s",
+          file=sys.stderr, flush=True)
+
+    for i in range(args.warmup):
+        pt.render(read=False)
+        print(f"[bench] warmup {i} {pt.ctx.stats().last_render_ms:.1f} ms", file=sys.stderr, flush=True)
+    pt.ctx.set_tuning(time_kernels=1)
+    closest = shadow = shade = 0
+    ext_ms = shd_ms = sdw_ms = 0.0
+    ext_launches = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        pt.render(read=False)
+        st = pt.ctx.stats()
+        closest += st.closest_rays
+        shadow += st.shadow_rays
+        shade += st.shade_events
+        ext_ms += st.extend_ms
+        shd_ms += st.shade_ms
+        sdw_ms += st.shadow_ms
+        ext_launches += st.extend_launches
+        print(f"[bench] step {i} {st.last_render_ms:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    pt.ctx.set_tuning(time_kernels=0)
+
+    npix = args.width * args.height
+    samples = npix * args.spp * args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    res = pt.read_result()
+    assert np.all(np.isfinite(res.color)), "non-finite pixels in the benchmark frame"
+
+    alg_bytes = 48.0 * closest  # over all extend launches of the timed steps
+    achieved = alg_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else None
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                "kernel": "k_extend", "launches": ext_launches,
+                "avg_launch_ms": ext_ms / max(ext_launches, 1),
+                "alg_bytes_per_launch": alg_bytes / max(ext_launches, 1)}
+    traffic_file = os.path.join(ROOT, "profiles", "extend_traffic.json")
+    if os.path.exists(traffic_file):
+        roofline["traffic"] = json.load(open(traffic_file)).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        cores = min(16, os.cpu_count() or 1)
+        o = oracle.OracleScene(scene)
+        rows = min(args.cpu_rows, args.height)
+        y0 = (args.height - rows) // 2
+        _, _, secs = o.render(cam, pt.params, threads=cores, pixel_range=(y0 * args.width, (y0 + rows) * args.width))
+        cpu_samples = rows * args.width * args.spp
+        cpu = {"value": cpu_samples / secs, "unit": "samples/s", "cores": cores, "kind": "port",
+               "sample": f"rows {y0}-{y0 + rows - 1} of the same 1024x1024x64spp frame ({cpu_samples} samples, "
+                         f"{secs:.1f} s)"}
+
+    line = {
+        "metric": "samples/s & Mrays/s at 1024^2 64spp (plain)",
+        "value": samples / wall,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"spheres6x6_l5 ({scene.triangle_count} tris, 4 quad lights, 1024x512 sky+sun env), "
+                               f"{args.width}x{args.height}, {args.spp} spp, max depth {args.depth}",
+                   "width": args.width, "height": args.height, "spp": args.spp, "max_depth": args.depth,
+                   "parallelism": "single GPU"},
+        "mrays_per_s": (closest + shadow) / wall / 1e6,
+        "rays": {"closest": closest, "shadow": shadow, "shade_events": shade},
+        "kernel_ms_per_step": {"extend": ext_ms / args.steps, "shade": shd_ms / args.steps,
+                               "shadow": sdw_ms / args.steps},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "bvh_build_s": build_s,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

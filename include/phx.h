@@ -1,0 +1,405 @@
+/*
+ * phx.h - C ABI of the MI355X (gfx950) wavefront path-tracing backend.
+ *
+ * Plain C types, plain pointers and sizes.  Every entry point returns PHX_OK (0)
+ * or a negative PHX_ERR_* code; phx_last_error() gives the message of the last
+ * failure on a context.  One context serves one host thread at a time (the
+ * reference's backends are driven from a single thread, RenderBridge.cpp:66-77);
+ * the library starts no host threads.  All kernels of a context run on the
+ * context's HIP stream, or on the stream passed to a call that takes one.
+ *
+ * Data types mirror the reference's scene model so a scene converted for the
+ * reference converts 1:1 for this backend:
+ *   phx_material   <- pt::Material       (include/photon/pt/material.h:35-82)
+ *   phx_light      <- pt::Light          (include/photon/pt/light.h:19-35)
+ *   phx_volume     <- pt::TfVolume       (include/photon/pt/tf_volume.h:14-33)
+ *   phx_texture_info <- pt::TextureInfo  (include/photon/pt/texture.h:11-17)
+ *   phx_camera     <- pt::Camera         (include/photon/pt/camera.h:11-26)
+ *   phx_render_params <- pt::RenderParams (include/photon/pt/pathtracer.h:16-55)
+ *   phx_hit        <- pt::HitResult      (include/photon/pt/backend/ray_backend.h:15-29)
+ * Per-parameter attribute/sampler sources of the ANARI device (material.h:16-19,
+ * ParamSource) are not part of this ABI; texture-atlas ids are.
+ */
+#ifndef PHX_H
+#define PHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHX_ABI_VERSION 1
+
+enum {
+  PHX_OK = 0,
+  PHX_ERR_INVALID = -1,   /* bad argument or size */
+  PHX_ERR_HIP = -2,       /* a HIP runtime call failed */
+  PHX_ERR_NO_SCENE = -3,  /* render/trace before a scene (or accel) was set */
+  PHX_ERR_OVERFLOW = -4,  /* a fixed-size device resource is too small */
+  PHX_ERR_NO_FRAME = -5,  /* read before any frame was rendered */
+  PHX_ERR_ALLOC = -6,     /* device or host allocation failed */
+  PHX_ERR_UNSUPPORTED = -7
+};
+
+/* Material::flags bits (material.h:29-33). */
+#define PHX_MATERIAL_FLAG_EMITTER_ONLY 1
+#define PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA 2
+#define PHX_MATERIAL_FLAG_EMIT_BOTH_SIDES 4
+#define PHX_MATERIAL_FLAG_REFRACTIVE 8
+#define PHX_MATERIAL_FLAG_EMIT_BACK_ONLY 16
+
+/* AlphaMode (material.h:22-26). */
+#define PHX_ALPHA_BLEND 0
+#define PHX_ALPHA_OPAQUE 1
+#define PHX_ALPHA_MASK 2
+
+/* LightType (light.h:11-17). */
+#define PHX_LIGHT_POINT 0
+#define PHX_LIGHT_DIRECTIONAL 1
+#define PHX_LIGHT_SPOT 2
+#define PHX_LIGHT_AREA 3
+
+/* pt::Material (material.h:35-82) without the ParamSource members. 36 words. */
+typedef struct phx_material {
+  float base_color[3];
+  float metallic;
+  float roughness;      /* GGX alpha (the pt-level value, not perceptual) */
+  float ior;
+  float transmission;
+  float specular;       /* 0.5 = Fresnel reflectance of ior */
+  float clearcoat;
+  float clearcoat_roughness;
+  float sheen;
+  float sheen_tint;
+  float specular_tint;
+  float anisotropic;
+  float emission_strength;
+  float subsurface;
+  float emission[3];
+  float subsurface_radius;
+  float subsurface_color[3];
+  float alpha;
+  int32_t thin;
+  int32_t flags;
+  int32_t alpha_mode;
+  float alpha_cutoff;
+  int32_t base_color_tex;
+  int32_t normal_tex;
+  int32_t roughness_tex;
+  int32_t metallic_tex;
+  int32_t emission_tex;
+  int32_t alpha_tex;
+  int32_t pad_[2];
+} phx_material;
+
+/* pt::Light (light.h:19-35). 24 words. */
+typedef struct phx_light {
+  uint32_t type;
+  float position[3];
+  float direction[3];
+  float color[3];
+  float intensity;
+  float spot_angle;
+  float spot_falloff;
+  uint32_t mesh_prim_begin;
+  uint32_t mesh_prim_count;
+  float area;
+  float edge1[3];
+  float edge2[3];
+  float radius;
+  uint32_t pad_;
+} phx_light;
+
+/* pt::TextureInfo (texture.h:11-17): texels are RGB floats in one atlas. */
+typedef struct phx_texture_info {
+  uint32_t offset;
+  uint32_t width;
+  uint32_t height;
+  uint32_t pad_;
+} phx_texture_info;
+
+/* pt::TfVolume (tf_volume.h:14-33). 36 words. */
+typedef struct phx_volume {
+  float world_to_local[12]; /* row-major 3x4 */
+  float origin[3];
+  float spacing[3];
+  uint32_t dims[3];
+  uint32_t data_offset;
+  uint32_t tf_offset;
+  uint32_t tf_size;
+  float value_lo;
+  float value_hi;
+  float density_scale;
+  float majorant;
+  uint32_t linear;
+  uint32_t macro_offset;
+  uint32_t macro_dims[3];
+  uint32_t pad_[3];
+} phx_volume;
+
+/* Everything pt::Scene (scene.h:41-72) holds, as host arrays. */
+typedef struct phx_scene_desc {
+  const float *positions;       /* vertex_count * 3 */
+  uint32_t vertex_count;
+  const uint32_t *indices;      /* triangle_count * 3 */
+  uint32_t triangle_count;
+  const float *normals;         /* NULL or vertex_count * 3 */
+  const float *texcoords;       /* NULL or vertex_count * 2 */
+  const float *vertex_colors;   /* NULL or vertex_count * 3 (x < 0: none) */
+  const uint32_t *material_ids; /* NULL or triangle_count */
+
+  const phx_material *materials;
+  uint32_t material_count;
+  const phx_light *lights;
+  uint32_t light_count;
+
+  /* Emissive primitives for NEE (Scene::emissive_prim_ids/areas). NULL: the
+   * library derives them from the materials as the render bridge does
+   * (RenderBridge.cpp:780-813). */
+  const uint32_t *emissive_prim_ids;
+  const float *emissive_prim_areas;
+  uint32_t emissive_count;
+
+  const float *texture_pixels; /* RGB */
+  uint32_t texture_pixel_count;
+  const phx_texture_info *texture_infos;
+  uint32_t texture_count;
+
+  /* Equirect environment (environment_map.h:13-25): row 0 is +Y of the map. */
+  uint32_t env_width;
+  uint32_t env_height;
+  const float *env_pixels; /* env_height * env_width * 3 */
+  float env_world_to_tex[9];
+  int32_t env_visible;
+
+  const phx_volume *volumes;
+  uint32_t volume_count;
+  const float *volume_values;
+  uint64_t volume_value_count;
+  const float *volume_tf; /* RGBA entries */
+  uint32_t volume_tf_count;
+  const float *volume_majorants;
+  uint32_t volume_majorant_count;
+
+  float ray_epsilon; /* 0: derived from the bounds (pathtracer.cpp:489-502) */
+} phx_scene_desc;
+
+/* pt::Camera (camera.h:11-26) in its derived form. */
+typedef struct phx_camera {
+  float origin[3];
+  float lower_left[3];
+  float horizontal[3];
+  float vertical[3];
+  float u[3], v[3], w[3];
+  float lens_radius;
+  float focus_dist;
+  int32_t is_ortho;
+  float clip_near;
+  float clip_far;
+} phx_camera;
+
+/* pt::RenderParams (pathtracer.h:16-55) without the background image. */
+typedef struct phx_render_params {
+  uint32_t width;
+  uint32_t height;
+  uint32_t samples_per_pixel;
+  uint32_t max_depth;
+  uint32_t sample_offset;
+  float background_color[3];
+  float background_alpha;
+  float ambient_color[3];
+  float ambient_radiance;
+  int32_t adaptive_sampling; /* must be 0: the variance-based early exit is not offered */
+  float throughput_clamp;
+  int32_t max_diffuse_bounces;
+  int32_t max_glossy_bounces;
+  int32_t max_transmission_bounces;
+  int32_t max_volume_bounces;
+  float clamp_direct_sum;
+  float clamp_indirect_sum;
+  int32_t bottom_up;
+  float image_region[4];
+} phx_render_params;
+
+/* pt::HitResult (ray_backend.h:15-29). 24 words. */
+typedef struct phx_hit {
+  float t;
+  float position[3];
+  float normal[3];
+  float shading_normal[3];
+  float uv[2];
+  float bary[2];
+  float interpolated_color[3];
+  uint32_t prim_id;
+  uint32_t geom_id;
+  uint32_t inst_id;
+  uint32_t material_id;
+  uint32_t hit;
+  uint32_t has_interpolated_color;
+  uint32_t pad_;
+} phx_hit;
+
+/* Compact closest-hit record used by the integrator: t, primitive, (b1, b2). */
+typedef struct phx_hit_compact {
+  float t;
+  uint32_t prim_id; /* 0xffffffff: no hit */
+  float b1;
+  float b2;
+} phx_hit_compact;
+
+/* Launch shape and queue sizing, chosen by the autotuner or by hand. */
+typedef struct phx_tuning {
+  uint32_t trace_block;      /* work-group size of the traversal kernels (64..1024) */
+  uint32_t shade_block;      /* work-group size of the shading kernels */
+  uint32_t paths_in_flight;  /* path slots per wavefront pass (0: auto) */
+  uint32_t sort_by_material; /* 1: partition the shade queue by material key */
+  uint32_t use_graph;        /* 1: replay the bounce loop as a captured hipGraph */
+  uint32_t time_kernels;     /* 1: time each kernel launch with HIP events (phx_stats) */
+  uint32_t pad_[2];
+} phx_tuning;
+
+/* Counters kept by the kernels (active rays only). */
+typedef struct phx_stats {
+  uint64_t closest_rays;
+  uint64_t shadow_rays;
+  uint64_t shade_events;
+  uint64_t samples;
+  uint64_t pixels;
+  uint64_t scene_bytes;
+  uint64_t bvh_nodes;
+  uint64_t bvh_bytes;
+  double last_render_ms;
+  /* With phx_tuning.time_kernels = 1: HIP-event time of every launch of the
+   * closest-hit (extend), shade and shadow kernels of the last render. */
+  uint64_t extend_launches;
+  double extend_ms;
+  uint64_t shade_launches;
+  double shade_ms;
+  uint64_t shadow_launches;
+  double shadow_ms;
+} phx_stats;
+
+typedef struct phx_context phx_context;
+
+/* ---- context ------------------------------------------------------------
+ * Replaces create_best_backend() + KokkosBackend construction
+ * (backend_factory.cpp:28-80, kokkos_backend.h:11-20) and Renderer::initialize
+ * (RenderBridge.h:185). */
+int phx_create(phx_context **out, int device);
+void phx_destroy(phx_context *ctx);
+const char *phx_last_error(const phx_context *ctx);
+/* RayBackend::name() (ray_backend.h:53): "hip". */
+const char *phx_backend_name(void);
+int phx_abi_version(void);
+int phx_set_tuning(phx_context *ctx, const phx_tuning *tuning);
+int phx_get_tuning(const phx_context *ctx, phx_tuning *tuning);
+int phx_get_stats(const phx_context *ctx, phx_stats *stats);
+
+/* ---- seam A: RayBackend (ray_backend.h:44-54) ---------------------------
+ * phx_build_accel replaces RayBackend::build_accel(const Scene&)
+ * (kokkos_backend.cpp:11-19): only the geometry members of the descriptor
+ * are read.  The trace calls replace trace_closest / trace_occluded
+ * (kokkos_backend.cpp:21-86): device pointers, rays as in RayBatch (origins
+ * and directions 3 floats each, tmin, tmax); ray i is inactive iff
+ * !(tmax > tmin).  They are asynchronous on `stream` (NULL: the context's). */
+int phx_build_accel(phx_context *ctx, const phx_scene_desc *scene);
+int phx_trace_closest(phx_context *ctx, const float *d_origins, const float *d_directions,
+                      const float *d_tmin, const float *d_tmax, uint32_t count, phx_hit *d_hits,
+                      void *stream);
+int phx_trace_closest_compact(phx_context *ctx, const float *d_origins, const float *d_directions,
+                              const float *d_tmin, const float *d_tmax, uint32_t count,
+                              phx_hit_compact *d_hits, void *stream);
+int phx_trace_occluded(phx_context *ctx, const float *d_origins, const float *d_directions,
+                       const float *d_tmin, const float *d_tmax, uint32_t count,
+                       uint32_t *d_occluded, void *stream);
+int phx_synchronize(phx_context *ctx);
+
+/* ---- integrator: PathTracer (pathtracer.h:65-73) ------------------------
+ * phx_set_scene replaces PathTracer::set_scene + build_accel
+ * (pathtracer.cpp:393-503, RenderBridge.cpp:822-825).  phx_render replaces
+ * PathTracer::render (pathtracer.cpp:564-1159); its result is read with
+ * phx_read_result (RenderResult, pathtracer.h:57-63; rows as rendered). */
+int phx_set_scene(phx_context *ctx, const phx_scene_desc *scene);
+int phx_render(phx_context *ctx, const phx_camera *camera, const phx_render_params *params);
+int phx_read_result(phx_context *ctx, float *color_rgb, float *alpha, float *depth, float *normal_xyz,
+                    float *albedo_rgb);
+/* Device pointer of the resolved color (h*w*3 floats), valid until the next render. */
+int phx_result_device_color(phx_context *ctx, const float **d_color);
+
+/* Disney BSDF of the shading kernels on n tuples in device memory: materials,
+ * wo, wi, geometric normal, shading normal (3 floats each).  phx_bsdf_eval
+ * replaces disney_bsdf_eval + disney_bsdf_pdf (disney_bsdf.h:288-355): f*cos
+ * (3 floats) and pdf per tuple.  phx_bsdf_sample replaces disney_bsdf_sample
+ * (disney_bsdf.h:381-521) with an xorshift32 stream seeded per tuple (rng.h:13):
+ * wi, f, pdf and flags (bit 0 specular, bit 1 diffuse).  Asynchronous. */
+int phx_bsdf_eval(phx_context *ctx, const phx_material *d_materials, const float *d_wo, const float *d_wi,
+                  const float *d_ng, const float *d_sn, uint32_t count, float *d_f, float *d_pdf, void *stream);
+int phx_bsdf_sample(phx_context *ctx, const phx_material *d_materials, const float *d_wo, const float *d_ng,
+                    const float *d_sn, const uint32_t *d_seeds, uint32_t count, float *d_wi, float *d_f,
+                    float *d_pdf, uint32_t *d_flags, void *stream);
+
+/* Camera::make_perspective / make_orthographic (camera.h:32-77). */
+int phx_camera_perspective(phx_camera *out, const float look_from[3], const float look_at[3],
+                           const float vup[3], float vfov_degrees, float aspect, float aperture,
+                           float focus_dist);
+int phx_camera_orthographic(phx_camera *out, const float look_from[3], const float look_at[3],
+                            const float vup[3], float height, float aspect);
+
+/* Macro-cell majorants of a volume (RenderBridge.cpp:656-778): fills
+ * vol->macro_dims and vol->majorant and writes the cells to `out`
+ * (capacity in floats; the needed count is returned in *out_count). */
+int phx_volume_majorants(phx_volume *vol, const float *values, const float *tf_rgba,
+                         uint32_t tf_size, float *out, uint32_t capacity, uint32_t *out_count);
+
+/* ---- seam B: photon::bridge::Renderer (RenderBridge.h:175-214) ----------
+ * Progressive accumulation over frames (RenderBridge.cpp:828-928): `reset`
+ * drops the accumulated samples; sample_offset advances by the frame's spp.
+ * read_* give the accumulated frame, rows starting at the bottom. */
+typedef struct phx_camera_data {
+  float position[3];
+  float direction[3];
+  float up[3];
+  int32_t orthographic;
+  float fovy; /* radians */
+  float height;
+  float aspect;
+  float aperture_radius;
+  float focus_distance;
+  float image_region[4];
+  float clip_near;
+  float clip_far;
+} phx_camera_data;
+
+typedef struct phx_render_settings {
+  uint32_t width;
+  uint32_t height;
+  uint32_t pixel_samples;
+  uint32_t max_depth;
+  float background[4];
+  float ambient_color[3];
+  float ambient_radiance;
+  int32_t adaptive_sampling;
+  float throughput_clamp;
+  int32_t max_diffuse_bounces;
+  int32_t max_glossy_bounces;
+  int32_t max_transmission_bounces;
+  int32_t max_volume_bounces;
+  float clamp_direct;
+  float clamp_indirect;
+} phx_render_settings;
+
+int phx_bridge_render(phx_context *ctx, const phx_camera_data *camera, const phx_render_settings *settings,
+                      int reset);
+uint32_t phx_bridge_accumulated_samples(const phx_context *ctx);
+int phx_bridge_read_color(phx_context *ctx, float *rgba);
+int phx_bridge_read_depth(phx_context *ctx, float *depth);
+int phx_bridge_read_normal(phx_context *ctx, float *xyz);
+int phx_bridge_read_albedo(phx_context *ctx, float *rgb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHX_H */

@@ -1,0 +1,1090 @@
+// ORACLE - test infrastructure only (see orc_core.h).  The reference's BVH
+// build, traversal, light tables and the PathTracer::render integrator
+// restated on the CPU, plus the extern "C" entry points the tests call.
+#include <omp.h>
+
+#include <cstdio>
+#include <memory>
+#include <numeric>
+
+#include "orc_scene.h"
+
+namespace orc {
+
+// ---- BVH build: bvh.cpp:13-125 ----------------------------------------------
+namespace {
+struct PrimRef {
+  uint32_t id;
+  Box b;
+  V3<float> c;
+};
+uint32_t build_node(std::vector<Node> &nodes, std::vector<PrimRef> &prims, std::vector<uint32_t> &ids, uint32_t begin,
+                    uint32_t end) {
+  const uint32_t node_id = uint32_t(nodes.size());
+  nodes.push_back({});
+  Box b, cb;
+  for (uint32_t i = begin; i < end; ++i) {
+    extend(b, prims[i].b.lo);
+    extend(b, prims[i].b.hi);
+    extend(cb, prims[i].c);
+  }
+  Node &n = nodes[node_id];
+  n.b = b;
+  const uint32_t pb = uint32_t(ids.size());
+  for (uint32_t i = begin; i < end; ++i) ids.push_back(prims[i].id);
+  n.begin = pb;
+  n.count = end - begin;
+  if (n.count <= 4) {  // bvh.cpp:42-48
+    n.leaf = 1;
+    return node_id;
+  }
+  ids.resize(pb);
+  n.begin = begin;
+  const V3<float> dg = cb.hi - cb.lo;
+  int axis = 0;  // bvh.cpp:53-58
+  if (dg.y > dg.x) axis = 1;
+  if ((axis == 0 ? dg.z : dg.z) > (axis == 0 ? dg.x : dg.y)) axis = 2;
+  auto key = [axis](const PrimRef &p) { return axis == 0 ? p.c.x : (axis == 1 ? p.c.y : p.c.z); };
+  const uint32_t mid = begin + (end - begin) / 2;
+  std::nth_element(prims.begin() + begin, prims.begin() + mid, prims.begin() + end,
+                   [&](const PrimRef &a, const PrimRef &c) { return key(a) < key(c); });
+  const uint32_t l = build_node(nodes, prims, ids, begin, mid);
+  const uint32_t r = build_node(nodes, prims, ids, mid, end);
+  nodes[node_id].left = l;
+  nodes[node_id].right = r;
+  nodes[node_id].leaf = 0;
+  return node_id;
+}
+}  // namespace
+
+void build_bvh(const Mesh &m, Bvh &out) {
+  const uint32_t nt = m.ntri();
+  std::vector<PrimRef> prims(nt);
+  for (uint32_t i = 0; i < nt; ++i) {
+    const V3<float> v0 = m.P(m.idx[i * 3]), v1 = m.P(m.idx[i * 3 + 1]), v2 = m.P(m.idx[i * 3 + 2]);
+    prims[i].id = i;
+    Box b;
+    extend(b, v0);
+    extend(b, v1);
+    extend(b, v2);
+    prims[i].b = b;
+    prims[i].c = (v0 + v1 + v2) * (1.f / 3.f);
+  }
+  out.nodes.clear();
+  out.prim_ids.clear();
+  out.nodes.reserve(size_t(nt) * 2);
+  out.prim_ids.reserve(nt);
+  out.root = build_node(out.nodes, prims, out.prim_ids, 0, nt);
+}
+
+// mesh_intersector.h:57-124
+ClosestHit trace_bvh(const Mesh &m, const Bvh &bvh, const V3<float> &o, const V3<float> &d, float tmin, float tmax,
+                     bool any_hit) {
+  ClosestHit best;
+  best.t = tmax;
+  if (!(tmax > tmin) || bvh.nodes.empty()) return best;
+  const V3<float> inv = safe_inv(d);
+  uint32_t stack[64];
+  int sp = 0;
+  stack[sp++] = bvh.root;
+  while (sp > 0) {
+    const Node &n = bvh.nodes[stack[--sp]];
+    if (aabb_entry(n.b, o, inv, tmin, best.t) < 0.f) continue;
+    if (n.leaf) {
+      for (uint32_t i = 0; i < n.count; ++i) {
+        const uint32_t prim = bvh.prim_ids[n.begin + i];
+        const TriHit h = intersect_triangle(o, d, m.P(m.idx[prim * 3]), m.P(m.idx[prim * 3 + 1]), m.P(m.idx[prim * 3 + 2]),
+                                            tmin, best.t);
+        if (!h.hit) continue;
+        best.t = h.t;
+        best.prim = prim;
+        best.hit = true;
+        best.u = h.u;
+        best.v = h.v;
+        if (any_hit) return best;
+      }
+    } else if (sp + 2 <= 64) {
+      const float dl = aabb_entry(bvh.nodes[n.left].b, o, inv, tmin, best.t);
+      const float dr = aabb_entry(bvh.nodes[n.right].b, o, inv, tmin, best.t);
+      if (dl >= 0.f && dr >= 0.f) {
+        if (dl < dr) {
+          stack[sp++] = n.right;
+          stack[sp++] = n.left;
+        } else {
+          stack[sp++] = n.left;
+          stack[sp++] = n.right;
+        }
+      } else if (dl >= 0.f) {
+        stack[sp++] = n.left;
+      } else if (dr >= 0.f) {
+        stack[sp++] = n.right;
+      }
+    }
+  }
+  return best;
+}
+
+ClosestHit trace_brute(const Mesh &m, const V3<float> &o, const V3<float> &d, float tmin, float tmax, bool any_hit) {
+  ClosestHit best;
+  best.t = tmax;
+  if (!(tmax > tmin)) return best;
+  const uint32_t nt = m.ntri();
+  for (uint32_t p = 0; p < nt; ++p) {
+    const TriHit h = intersect_triangle(o, d, m.P(m.idx[p * 3]), m.P(m.idx[p * 3 + 1]), m.P(m.idx[p * 3 + 2]), tmin, best.t);
+    if (!h.hit) continue;
+    best.t = h.t;
+    best.prim = p;
+    best.u = h.u;
+    best.v = h.v;
+    best.hit = true;
+    if (any_hit) return best;
+  }
+  return best;
+}
+
+// ---- majorants: RenderBridge.cpp:656-778 ------------------------------------
+uint32_t build_majorants(phx_volume &v, const float *values, const float *tf_rgba, uint32_t tf_size,
+                         std::vector<float> &out) {
+  out.clear();
+  float max_op = 0.f;
+  for (uint32_t t = 0; t < tf_size; ++t) max_op = std::max(max_op, tf_rgba[4 * t + 3]);
+  for (int d = 0; d < 3; ++d) v.dims[d] = std::max<uint32_t>(v.dims[d], 1);
+  v.majorant = max_op * v.density_scale;
+  const bool complete = values != nullptr;
+  if (!complete) v.majorant = 0.f;
+  uint32_t sub[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    const uint32_t cells = v.dims[d] > 1 ? v.dims[d] - 1 : 1;
+    if (cells <= 32) {
+      sub[d] = std::min<uint32_t>(std::max<uint32_t>((32 + cells - 1) / cells, 1), 16);
+      v.macro_dims[d] = cells * sub[d];
+    } else {
+      v.macro_dims[d] = std::min<uint32_t>(std::max<uint32_t>((v.dims[d] + 2) / 4, 1), 128);
+    }
+  }
+  const float range = v.value_hi - v.value_lo;
+  const bool refined = sub[0] > 0 && sub[1] > 0 && sub[2] > 0;
+  if (!refined)
+    for (int d = 0; d < 3; ++d)
+      if (sub[d] > 0) v.macro_dims[d] = std::min<uint32_t>(std::max<uint32_t>((v.dims[d] + 2) / 4, 1), 128);
+  auto node = [&](uint32_t x, uint32_t y, uint32_t z) {
+    x = std::min(x, v.dims[0] - 1);
+    y = std::min(y, v.dims[1] - 1);
+    z = std::min(z, v.dims[2] - 1);
+    return values[(size_t(z) * v.dims[1] + y) * v.dims[0] + x];
+  };
+  auto interp = [&](float gx, float gy, float gz) {
+    const uint32_t x = uint32_t(gx), y = uint32_t(gy), z = uint32_t(gz);
+    const float fx = gx - float(x), fy = gy - float(y), fz = gz - float(z);
+    const float c00 = node(x, y, z) * (1.f - fx) + node(x + 1, y, z) * fx;
+    const float c10 = node(x, y + 1, z) * (1.f - fx) + node(x + 1, y + 1, z) * fx;
+    const float c01 = node(x, y, z + 1) * (1.f - fx) + node(x + 1, y, z + 1) * fx;
+    const float c11 = node(x, y + 1, z + 1) * (1.f - fx) + node(x + 1, y + 1, z + 1) * fx;
+    return (c00 * (1.f - fy) + c10 * fy) * (1.f - fz) + (c01 * (1.f - fy) + c11 * fy) * fz;
+  };
+  for (uint32_t mz = 0; mz < v.macro_dims[2]; ++mz)
+    for (uint32_t my = 0; my < v.macro_dims[1]; ++my)
+      for (uint32_t mx = 0; mx < v.macro_dims[0]; ++mx) {
+        float bound = 0.f;
+        if (complete && tf_size > 0) {
+          const uint32_t mm[3] = {mx, my, mz};
+          float vmin = std::numeric_limits<float>::max(), vmax = -std::numeric_limits<float>::max();
+          if (refined) {
+            for (int corner = 0; corner < 8; ++corner) {
+              float g[3];
+              for (int d = 0; d < 3; ++d) g[d] = float(mm[d] + ((corner >> d) & 1)) / float(sub[d]);
+              const float val = interp(g[0], g[1], g[2]);
+              vmin = std::min(vmin, val);
+              vmax = std::max(vmax, val);
+            }
+          } else {
+            uint32_t first[3], last[3];
+            for (int d = 0; d < 3; ++d) {
+              const float cells = float(v.dims[d] > 1 ? v.dims[d] - 1 : 1);
+              const float lo = float(mm[d]) / float(v.macro_dims[d]) * cells;
+              const float hi = float(mm[d] + 1) / float(v.macro_dims[d]) * cells;
+              first[d] = uint32_t(std::max(std::floor(lo) - 1.f, 0.f));
+              last[d] = std::min<uint32_t>(uint32_t(std::ceil(hi)) + 1, v.dims[d] - 1);
+            }
+            for (uint32_t z = first[2]; z <= last[2]; ++z)
+              for (uint32_t y = first[1]; y <= last[1]; ++y)
+                for (uint32_t x = first[0]; x <= last[0]; ++x) {
+                  const float val = values[(size_t(z) * v.dims[1] + y) * v.dims[0] + x];
+                  vmin = std::min(vmin, val);
+                  vmax = std::max(vmax, val);
+                }
+          }
+          auto tpos = [&](float val) {
+            float x = range != 0.f ? (val - v.value_lo) / range : 0.f;
+            x = std::min(std::max(x, 0.f), 1.f);
+            return x * float(tf_size - 1);
+          };
+          const size_t i0 = size_t(std::floor(tpos(vmin)));
+          const size_t i1 = std::min(size_t(std::ceil(tpos(vmax))), size_t(tf_size) - 1);
+          for (size_t t = i0; t <= i1; ++t) bound = std::max(bound, tf_rgba[4 * t + 3]);
+          if (!(vmin <= vmax)) bound = max_op;
+        }
+        out.push_back(bound * v.density_scale * 1.01f);
+      }
+  return uint32_t(out.size());
+}
+
+// ---- scene -----------------------------------------------------------------
+struct Scene {
+  Mesh mesh;
+  Bvh bvh;
+  std::vector<phx_material> mats;
+  std::vector<phx_light> nee_lights;  // after build_light_tables
+  std::vector<uint32_t> emissive_ids;
+  std::vector<float> emissive_areas;
+  std::vector<float> emissive_cdf;
+  std::vector<float> prim_light_pdf;
+  uint32_t emissive_count{0};
+  EnvMap env;
+  bool has_env{false}, env_visible{true};
+  Atlas atlas;
+  Volumes vol;
+  float ray_eps{1e-3f};
+  bool brute{false};  // trace by brute force instead of the BVH
+};
+
+static void scene_load(Scene &s, const phx_scene_desc &d) {
+  const uint32_t nv = d.vertex_count, nt = d.triangle_count;
+  s.mesh.pos.assign(d.positions, d.positions + size_t(nv) * 3);
+  s.mesh.idx.assign(d.indices, d.indices + size_t(nt) * 3);
+  if (d.normals) s.mesh.nrm.assign(d.normals, d.normals + size_t(nv) * 3);
+  if (d.texcoords) s.mesh.uv.assign(d.texcoords, d.texcoords + size_t(nv) * 2);
+  if (d.vertex_colors) s.mesh.col.assign(d.vertex_colors, d.vertex_colors + size_t(nv) * 3);
+  if (d.material_ids) s.mesh.mat.assign(d.material_ids, d.material_ids + nt);
+  s.mats.assign(d.materials, d.materials + d.material_count);
+  build_bvh(s.mesh, s.bvh);
+
+  // Emissive list: given, or derived as RenderBridge.cpp:780-813 does.
+  s.emissive_ids.clear();
+  s.emissive_areas.clear();
+  if (d.emissive_prim_ids) {
+    s.emissive_ids.assign(d.emissive_prim_ids, d.emissive_prim_ids + d.emissive_count);
+    s.emissive_areas.assign(d.emissive_prim_areas, d.emissive_prim_areas + d.emissive_count);
+  } else {
+    const uint32_t mc = std::max<uint32_t>(d.material_count, 1);
+    for (uint32_t t = 0; t < nt; ++t) {
+      const uint32_t mid = (!s.mesh.mat.empty() && s.mesh.mat[t] < mc) ? s.mesh.mat[t] : 0u;
+      if (mid >= s.mats.size() || !material_may_emit(mat_from<float>(s.mats[mid]))) continue;
+      const V3<float> p0 = s.mesh.P(s.mesh.idx[3 * t]), p1 = s.mesh.P(s.mesh.idx[3 * t + 1]), p2 = s.mesh.P(s.mesh.idx[3 * t + 2]);
+      const float area = 0.5f * length(cross(p1 - p0, p2 - p0));
+      if (!(area > 0.f)) continue;
+      s.emissive_ids.push_back(t);
+      s.emissive_areas.push_back(area);
+    }
+  }
+  const bool has_em = !s.emissive_ids.empty();
+
+  // build_light_tables: pathtracer.cpp:411-503
+  s.nee_lights.clear();
+  for (uint32_t i = 0; i < d.light_count; ++i) {
+    const phx_light &l = d.lights[i];
+    if (has_em && l.type == PHX_LIGHT_AREA && l.mesh_prim_count > 0) continue;
+    s.nee_lights.push_back(l);
+  }
+  s.emissive_count = 0;
+  s.emissive_cdf.clear();
+  s.prim_light_pdf.clear();
+  if (has_em) {
+    const uint32_t n = uint32_t(s.emissive_ids.size());
+    std::vector<double> w(n, 0.0);
+    double total = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t prim = s.emissive_ids[i];
+      const uint32_t mid = !s.mesh.mat.empty() ? s.mesh.mat[prim] : 0u;
+      float power = 1.f;
+      if (mid < s.mats.size()) power = luminance(material_emission(mat_from<float>(s.mats[mid])));
+      w[i] = double(s.emissive_areas[i]) * double(power > 1e-6f ? power : 1e-6f);
+      if (!(s.emissive_areas[i] > 0.f)) w[i] = 0.0;
+      total += w[i];
+    }
+    if (total > 0.0) {
+      s.emissive_count = n;
+      s.emissive_cdf.assign(n, 0.f);
+      s.prim_light_pdf.assign(nt, 0.f);
+      double run = 0.0;
+      for (uint32_t i = 0; i < n; ++i) {
+        run += w[i];
+        s.emissive_cdf[i] = float(run / total);
+        if (w[i] > 0.0 && s.emissive_ids[i] < nt)
+          s.prim_light_pdf[s.emissive_ids[i]] = float(w[i] / total / double(s.emissive_areas[i]));
+      }
+      s.emissive_cdf[n - 1] = 1.f;
+    }
+  }
+
+  s.has_env = d.env_width > 0 && d.env_height > 0 && d.env_pixels;
+  if (s.has_env) {
+    s.env.width = d.env_width;
+    s.env.height = d.env_height;
+    s.env.pixels.assign(d.env_pixels, d.env_pixels + size_t(d.env_width) * d.env_height * 3);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        s.env.w2t[r * 3 + c] = d.env_world_to_tex[r * 3 + c];
+        s.env.t2w[c * 3 + r] = d.env_world_to_tex[r * 3 + c];
+      }
+    s.env.build_cdf();
+  }
+  s.env_visible = d.env_visible != 0;
+  if (d.texture_count) {
+    s.atlas.pixels.assign(d.texture_pixels, d.texture_pixels + size_t(d.texture_pixel_count) * 3);
+    s.atlas.infos.assign(d.texture_infos, d.texture_infos + d.texture_count);
+  }
+  if (d.volume_count) {
+    s.vol.vols.assign(d.volumes, d.volumes + d.volume_count);
+    s.vol.values.assign(d.volume_values, d.volume_values + d.volume_value_count);
+    s.vol.tf.assign(d.volume_tf, d.volume_tf + size_t(d.volume_tf_count) * 4);
+    s.vol.majorants.assign(d.volume_majorants, d.volume_majorants + d.volume_majorant_count);
+  }
+  // pathtracer.cpp:489-502
+  s.ray_eps = 1e-3f;
+  if (d.ray_epsilon > 0.f) {
+    s.ray_eps = d.ray_epsilon;
+  } else if (!s.bvh.nodes.empty() && nt > 0) {
+    const Box &b = s.bvh.nodes[s.bvh.root].b;
+    if (b.hi.x >= b.lo.x) {
+      const float eps = 1e-4f * length(b.hi - b.lo);
+      s.ray_eps = eps < 1e-6f ? 1e-6f : (eps > 1e-2f ? 1e-2f : eps);
+    }
+  }
+}
+
+// Full hit record: mesh_intersector.h:126-178.
+template <class T> struct Hit {
+  bool hit{false};
+  T t{0};
+  V3<T> p, n, sn, color;
+  V2<T> uv, bary;
+  uint32_t prim{0}, mat{0};
+  bool has_color{false};
+};
+template <class T> static Hit<T> closest(const Scene &s, const V3<T> &o, const V3<T> &d, T tmin, T tmax) {
+  const V3<float> of(o), df(d);
+  const ClosestHit ch = s.brute ? trace_brute(s.mesh, of, df, float(tmin), float(tmax), false)
+                                : trace_bvh(s.mesh, s.bvh, of, df, float(tmin), float(tmax), false);
+  Hit<T> h;
+  if (!ch.hit) return h;
+  h.hit = true;
+  h.t = T(ch.t);
+  const Mesh &m = s.mesh;
+  const uint32_t i0 = m.idx[ch.prim * 3], i1 = m.idx[ch.prim * 3 + 1], i2 = m.idx[ch.prim * 3 + 2];
+  const V3<T> v0 = v3<T>(&m.pos[size_t(i0) * 3]), v1 = v3<T>(&m.pos[size_t(i1) * 3]), v2 = v3<T>(&m.pos[size_t(i2) * 3]);
+  h.p = o + d * h.t;
+  const V3<T> nn = normalize(cross(v1 - v0, v2 - v0));
+  h.n = nn;
+  h.sn = nn;
+  h.bary = {T(ch.u), T(ch.v)};
+  const T w0 = T(1) - T(ch.u) - T(ch.v), w1 = T(ch.u), w2 = T(ch.v);
+  if (!m.nrm.empty()) {
+    h.sn = normalize(v3<T>(&m.nrm[size_t(i0) * 3]) * w0 + v3<T>(&m.nrm[size_t(i1) * 3]) * w1 + v3<T>(&m.nrm[size_t(i2) * 3]) * w2);
+    if (dot(h.sn, nn) < T(0)) h.sn = h.sn * T(-1);
+  }
+  if (!m.uv.empty()) {
+    const float *a = &m.uv[size_t(i0) * 2], *b = &m.uv[size_t(i1) * 2], *c = &m.uv[size_t(i2) * 2];
+    h.uv = {T(a[0]) * w0 + T(b[0]) * w1 + T(c[0]) * w2, T(a[1]) * w0 + T(b[1]) * w1 + T(c[1]) * w2};
+  }
+  if (!m.col.empty() && m.col[size_t(i0) * 3] >= 0.f) {
+    h.color = v3<T>(&m.col[size_t(i0) * 3]) * w0 + v3<T>(&m.col[size_t(i1) * 3]) * w1 + v3<T>(&m.col[size_t(i2) * 3]) * w2;
+    h.has_color = true;
+  }
+  h.prim = ch.prim;
+  h.mat = m.mat.empty() ? 0u : m.mat[ch.prim];
+  return h;
+}
+static bool occluded(const Scene &s, const V3<float> &o, const V3<float> &d, float tmin, float tmax) {
+  const ClosestHit ch = s.brute ? trace_brute(s.mesh, o, d, tmin, tmax, true) : trace_bvh(s.mesh, s.bvh, o, d, tmin, tmax, true);
+  return ch.hit;
+}
+
+// pathtracer.cpp:186-247 (texture members only).
+template <class T> static Mat<T> resolve_material(const Scene &s, uint32_t mid, const V2<T> &uv, bool has_color,
+                                                  const V3<T> &color, T &alpha, const Mat<T> *override_mats) {
+  Mat<T> m = override_mats ? override_mats[mid] : (s.mats.empty() ? Mat<T>{} : mat_from<T>(s.mats[mid]));
+  const bool tex = !s.atlas.infos.empty();
+  alpha = m.alpha;
+  if (m.alpha_tex >= 0 && tex) alpha = s.atlas.sample<T>(m.alpha_tex, uv.x, uv.y).x;
+  if (m.base_color_tex >= 0 && tex) m.base_color = s.atlas.sample<T>(m.base_color_tex, uv.x, uv.y);
+  else if (has_color) m.base_color = color;
+  if (m.roughness_tex >= 0 && tex) m.roughness = s.atlas.sample<T>(m.roughness_tex, uv.x, uv.y).x;
+  if (m.metallic_tex >= 0 && tex) m.metallic = s.atlas.sample<T>(m.metallic_tex, uv.x, uv.y).x;
+  auto c01 = [](T x) { return x < T(0) ? T(0) : (x > T(1) ? T(1) : x); };
+  m.metallic = c01(m.metallic);
+  m.roughness = c01(m.roughness);
+  m.transmission = c01(m.transmission);
+  if (m.alpha_mode == PHX_ALPHA_OPAQUE) alpha = T(1);
+  else if (m.alpha_mode == PHX_ALPHA_MASK) alpha = alpha >= m.alpha_cutoff ? T(1) : T(0);
+  else alpha = c01(alpha);
+  return m;
+}
+template <class T> static bool emits_on_side(const Mat<T> &m, bool front) {  // pathtracer.cpp:250-256
+  if (m.flags & PHX_MATERIAL_FLAG_EMIT_BOTH_SIDES) return true;
+  const bool back_only = (m.flags & PHX_MATERIAL_FLAG_EMIT_BACK_ONLY) != 0;
+  return front ? !back_only : back_only;
+}
+// pathtracer.cpp:40-47
+template <class T> static V3<T> clamp_contribution(const V3<T> &c, int lb, T ds, T is) {
+  const T limit = lb > 0 ? is : ds;
+  if (!(limit > T(0))) return c;
+  const T sum = std::fabs(c.x) + std::fabs(c.y) + std::fabs(c.z);
+  return sum > limit ? c * (limit / sum) : c;
+}
+// pathtracer.cpp:51-58
+static bool count_bounce(uint32_t &kinds, uint32_t shift, int limit) {
+  uint32_t count = (kinds >> shift) & 0x7fu;
+  if (count < 0x7fu) ++count;
+  kinds = (kinds & ~(0x7fu << shift)) | (count << shift);
+  return limit >= 0 && int(count) > limit;
+}
+static const uint32_t PATH_EMISSION_ONLY = 0x80000000u;
+
+template <class T> struct Direct { V3<T> wi, L; T pdf{0}, dist{0}; bool mis{false}, valid{false}; };
+// pathtracer.cpp:270-367
+template <class T> static Direct<T> sample_direct(const Scene &s, const V3<T> &p, Rng &rng, uint32_t strategies,
+                                                  const Mat<T> *ov) {
+  Direct<T> ds;
+  if (strategies == 0) return ds;
+  const uint32_t nl = uint32_t(s.nee_lights.size());
+  const T pick_pdf = T(1) / T(strategies);
+  uint32_t pick = uint32_t(rng.next_f32() * float(strategies));
+  if (pick >= strategies) pick = strategies - 1;
+  if (pick < nl) {
+    const LightSample<T> ls = sample_light<T>(s.nee_lights[pick], p, rng);
+    if (!(ls.pdf > T(0))) return ds;
+    ds.wi = ls.wi;
+    ds.L = ls.Li;
+    ds.pdf = ls.pdf * pick_pdf;
+    ds.dist = ls.dist;
+    ds.mis = false;
+    ds.valid = true;
+    return ds;
+  }
+  if (s.emissive_count > 0 && pick == nl) {
+    const float xi = rng.next_f32();
+    uint32_t lo = 0, hi = s.emissive_count - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (s.emissive_cdf[mid] <= xi) lo = mid + 1;
+      else hi = mid;
+    }
+    const uint32_t prim = s.emissive_ids[lo];
+    const T pdf_area = T(s.prim_light_pdf[prim]);
+    if (!(pdf_area > T(0))) return ds;
+    const Mesh &m = s.mesh;
+    const V3<T> v0 = v3<T>(&m.pos[size_t(m.idx[prim * 3]) * 3]), v1 = v3<T>(&m.pos[size_t(m.idx[prim * 3 + 1]) * 3]),
+                v2 = v3<T>(&m.pos[size_t(m.idx[prim * 3 + 2]) * 3]);
+    const T su = std::sqrt(draw<T>(rng));
+    const T r2 = draw<T>(rng);
+    const T b0 = T(1) - su, b1 = r2 * su, b2 = T(1) - b0 - b1;
+    const V3<T> lp = v0 * b0 + v1 * b1 + v2 * b2;
+    const V3<T> to = lp - p;
+    const T d2 = dot(to, to);
+    if (!(d2 > T(0))) return ds;
+    const T dist = std::sqrt(d2);
+    const V3<T> wi = to * (T(1) / dist);
+    const V3<T> ln = normalize(cross(v1 - v0, v2 - v0));
+    const T cl = -dot(wi, ln);
+    if (std::fabs(cl) < T(1e-6)) return ds;
+    // resolve_material at the light point: its texcoords are not interpolated
+    // there (SurfaceQuery::uv stays 0), pathtracer.cpp:333-341.
+    T alpha = T(1);
+    const uint32_t mid = m.mat.empty() ? 0u : m.mat[prim];
+    const Mat<T> em = resolve_material<T>(s, mid, V2<T>{}, false, V3<T>{}, alpha, ov);
+    if (!emits_on_side(em, cl > T(0))) return ds;
+    ds.wi = wi;
+    ds.L = material_emission(em) * alpha;
+    ds.pdf = pdf_area * d2 / std::fabs(cl) * pick_pdf;
+    ds.dist = dist;
+    ds.mis = true;
+    ds.valid = true;
+    return ds;
+  }
+  if (s.has_env) {
+    T ep = T(0);
+    const V3<T> wi = s.env.sample_direction<T>(rng, ep);
+    if (!(ep > T(0))) return ds;
+    ds.wi = wi;
+    ds.L = s.env.evaluate<T>(wi);
+    ds.pdf = ep * pick_pdf;
+    ds.dist = T(1e30);
+    ds.mis = true;
+    ds.valid = true;
+  }
+  return ds;
+}
+
+struct Params {
+  phx_render_params p;
+  phx_camera cam;
+};
+
+// Per-pixel accumulators of PathTracer::render (pathtracer.cpp:505-562).
+template <class T> struct PixelOut {
+  V3<T> accum{};
+  T alpha{0};
+  uint32_t ns{0};
+  V3<T> albedo{}, normal{};
+  T depth{T(1e30)};
+};
+
+// One pixel, all its samples: pathtracer.cpp:688-1127.  The reference runs the
+// bounce loop over all pixels in lockstep; each pixel's state only depends on
+// its own streams, so running the pixels one after another gives the same
+// values.  `ov` replaces the scene materials (finite-difference engine).
+template <class T>
+static void render_pixel(const Scene &s, const Params &P, uint32_t idx, PixelOut<T> &out, const Mat<T> *ov) {
+  const phx_render_params &rp = P.p;
+  const uint32_t w = rp.width, h = rp.height;
+  const uint32_t max_depth = rp.max_depth > 0 ? rp.max_depth : 1u;
+  const uint32_t strategies = uint32_t(s.nee_lights.size()) + (s.emissive_count > 0 ? 1u : 0u) + (s.has_env ? 1u : 0u);
+  const uint32_t max_it = max_depth + 1u + 8u;
+  const bool has_vol = !s.vol.empty();
+  const T eps = T(s.ray_eps);
+  const T cd = T(rp.clamp_direct_sum), ci = T(rp.clamp_indirect_sum);
+  const V3<T> ambient = v3<T>(rp.ambient_color) * T(rp.ambient_radiance);
+  const V3<T> bg = v3<T>(rp.background_color);
+  const uint32_t x = idx % w, y = idx / w;
+  out = PixelOut<T>{};
+  for (uint32_t sidx = 0; sidx < rp.samples_per_pixel; ++sidx) {
+    const uint32_t sample = sidx + rp.sample_offset;
+    // pt_init_sample: pathtracer.cpp:727-759
+    Rng crng = make_rng(idx, sample, 0u);
+    T u = (T(x) + draw<T>(crng)) / T(w);
+    T v = (T(y) + draw<T>(crng)) / T(h);
+    if (!rp.bottom_up) v = T(1) - v;
+    u = T(rp.image_region[0]) + (T(rp.image_region[2]) - T(rp.image_region[0])) * u;
+    v = T(rp.image_region[1]) + (T(rp.image_region[3]) - T(rp.image_region[1])) * v;
+    V3<T> ro, rd;
+    camera_ray<T>(P.cam, u, v, crng, ro, rd);
+    T rtmin, rtmax;
+    clip_range<T>(P.cam, rd, rtmin, rtmax);
+    V3<T> tp{T(1), T(1), T(1)};
+    bool active = true, primary = true;
+    T prev_pdf = T(-1), path_dist = T(0);
+    uint32_t depth = 0, kinds = 0;
+    out.ns += 1;
+    for (uint32_t it = 0; it < max_it && active; ++it) {
+      const Hit<T> hr = closest<T>(s, ro, rd, rtmin, rtmax);
+      Rng rng = make_rng(idx, sample, 1u + it);
+      const bool emission_only = (kinds & PATH_EMISSION_ONLY) != 0u;
+      bool nee = false;
+      V3<T> nee_c{}, so, sd;
+      T stmin = T(1), stmax = T(0);
+      // pt_shade: pathtracer.cpp:787-1089
+      if (has_vol) {
+        V3<T> alb{};
+        const T t_end = hr.hit ? hr.t : rtmax;
+        const T ts = s.vol.sample_scatter<T>(ro, rd, rtmin, t_end, rng, alb);
+        if (ts >= T(0)) {
+          const V3<T> p = ro + rd * ts;
+          if (primary) {  // :800-809 (the AOVs come from pixel_aov)
+            primary = false;
+            out.alpha += T(1);
+          }
+          if (depth >= max_depth || emission_only) {
+            active = false;
+          } else {
+            const V3<T> tpv = tp * alb;
+            const T phase = T(0.25) * K<T>::INV_PI;
+            const Direct<T> ds = sample_direct<T>(s, p, rng, strategies, ov);
+            if (ds.valid) {
+              const T wm = ds.mis ? power_heuristic(1, ds.pdf, 1, phase) : T(1);
+              nee_c = clamp_contribution(tpv * ds.L * (phase * wm / ds.pdf), int(depth), cd, ci);
+              nee = true;
+              so = p;
+              sd = ds.wi;
+              stmin = T(0);
+              stmax = ds.dist < T(1e29) ? ds.dist * (T(1) - T(1e-4)) : T(1e30);
+            }
+            V3<T> tn = tpv;
+            bool killed = false;
+            if (depth >= 2) {
+              const T mt = fmaxT(tpv.x, fmaxT(tpv.y, tpv.z));
+              const T surv = fminT(fmaxT(mt, T(0.05)), T(0.95));
+              if (draw<T>(rng) >= surv) killed = true;
+              else tn = tpv * (T(1) / surv);
+            }
+            if (killed) {
+              active = false;
+            } else {
+              tp = tn;
+              prev_pdf = phase;
+              depth += 1;
+              if (count_bounce(kinds, 24u, rp.max_volume_bounces)) kinds |= PATH_EMISSION_ONLY;
+              ro = p;
+              rd = sample_uniform_sphere<T>(rng);
+              rtmin = T(0);
+              rtmax = T(1e30);
+            }
+          }
+          goto pack;
+        }
+      }
+      if (!hr.hit) {
+        V3<T> L{};
+        if (primary) {
+          if (s.has_env && s.env_visible) {
+            L = s.env.evaluate<T>(rd);
+            out.alpha += T(1);
+          } else {
+            L = bg * T(rp.background_alpha);
+            out.alpha += T(rp.background_alpha);
+          }
+        } else {
+          L = ambient;
+          if (s.has_env) {
+            T wm = T(1);
+            if (prev_pdf > T(0)) wm = power_heuristic(1, prev_pdf, 1, s.env.pdf<T>(rd) / T(strategies));
+            L = L + s.env.evaluate<T>(rd) * wm;
+          }
+        }
+        V3<T> fm = tp * L;
+        if (!primary || (s.has_env && s.env_visible)) fm = clamp_contribution(fm, int(depth) - 1, cd, ci);
+        out.accum = out.accum + fm;
+        active = false;
+        goto pack;
+      }
+      {
+        const V3<T> p = hr.p;
+        const V3<T> wo = -rd;
+        V3<T> hsn = hr.sn;
+        if (!(dot(hsn, hsn) > T(0.25))) hsn = hr.n;
+        else if (dot(hsn, hr.n) < T(0)) hsn = hsn * T(-1);
+        T alpha = T(1);
+        const Mat<T> mat = resolve_material<T>(s, hr.mat, hr.uv, hr.has_color, hr.color, alpha, ov);
+        const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0 && primary;
+        if (hidden || (alpha < T(1) && draw<T>(rng) >= alpha)) {
+          ro = p;
+          rtmin = eps;
+          rtmax = T(1e30);
+          path_dist += hr.t;
+          if (primary) {
+            T tn, tf;
+            clip_range<T>(P.cam, rd, tn, tf);
+            if (tf < T(1e30)) {
+              rtmax = tf - path_dist;
+              if (rtmax < eps) {
+                rtmin = T(1);
+                rtmax = T(0);
+              }
+            }
+          }
+          prev_pdf = T(-1);
+          goto pack;
+        }
+        V3<T> n = hr.n, sn = hsn;
+        const bool front = dot(n, wo) >= T(0);
+        if (!(mat.transmission > T(0)) && !front) {
+          n = n * T(-1);
+          sn = sn * T(-1);
+        }
+        if (mat.normal_tex >= 0 && !s.atlas.infos.empty()) {
+          V3<T> nm = s.atlas.sample<T>(mat.normal_tex, hr.uv.x, hr.uv.y);
+          nm = nm * T(2) - V3<T>{T(1), T(1), T(1)};
+          V3<T> ta, bi;
+          onb_from_normal(sn, ta, bi);
+          sn = normalize(ta * nm.x + bi * nm.y + sn * nm.z);
+        }
+        if (primary) {  // :974-983 (the AOVs come from pixel_aov)
+          primary = false;
+          out.alpha += T(1);
+        }
+        if (material_may_emit(mat) && emits_on_side(mat, front)) {
+          T wm = T(1);
+          if (prev_pdf > T(0) && s.emissive_count > 0) {
+            const T pa = T(s.prim_light_pdf[hr.prim]);
+            const T cl = std::fabs(dot(hr.n, wo));
+            if (pa > T(0) && cl > T(0)) wm = power_heuristic(1, prev_pdf, 1, pa * hr.t * hr.t / cl / T(strategies));
+          }
+          out.accum = out.accum + clamp_contribution(tp * material_emission(mat) * wm, int(depth) - 1, cd, ci);
+        }
+        if ((mat.flags & PHX_MATERIAL_FLAG_EMITTER_ONLY) != 0 || depth >= max_depth || emission_only) {
+          active = false;
+          goto pack;
+        }
+        {
+          const Direct<T> ds = sample_direct<T>(s, p, rng, strategies, ov);
+          if (ds.valid) {
+            const V3<T> f = disney_bsdf_eval(mat, wo, ds.wi, n, sn);
+            if (f.x > T(0) || f.y > T(0) || f.z > T(0)) {
+              T wm = T(1);
+              if (ds.mis) wm = power_heuristic(1, ds.pdf, 1, disney_bsdf_pdf(mat, wo, ds.wi, n, sn));
+              nee_c = clamp_contribution(tp * f * ds.L * (wm / ds.pdf), int(depth), cd, ci);
+              nee = true;
+              const V3<T> sorg = p + n * eps;
+              so = sorg;
+              stmin = T(0);
+              if (ds.dist < T(1e29)) {
+                const V3<T> tl = p + ds.wi * ds.dist - sorg;
+                const T sdist = length(tl);
+                sd = sdist > T(0) ? tl * (T(1) / sdist) : ds.wi;
+                stmax = sdist * (T(1) - T(1e-3));
+              } else {
+                sd = ds.wi;
+                stmax = T(1e30);
+              }
+            }
+          }
+        }
+        const BsdfSample<T> bs = disney_bsdf_sample(mat, wo, n, sn, rng);
+        if (bs.pdf <= T(0) || (bs.f.x <= T(0) && bs.f.y <= T(0) && bs.f.z <= T(0))) {
+          active = false;
+          goto pack;
+        }
+        V3<T> tpn = tp * (bs.f * (T(1) / bs.pdf));
+        if (depth >= 2) {
+          const T mt = fmaxT(tpn.x, fmaxT(tpn.y, tpn.z));
+          const T surv = fminT(fmaxT(mt, T(0.05)), T(0.95));
+          if (draw<T>(rng) >= surv) {
+            active = false;
+            goto pack;
+          }
+          tpn = tpn * (T(1) / surv);
+        }
+        if (rp.throughput_clamp > 0.f) {
+          const T mm = fmaxT(tpn.x, fmaxT(tpn.y, tpn.z));
+          if (mm > T(rp.throughput_clamp)) tpn = tpn * (T(rp.throughput_clamp) / mm);
+        }
+        if (!(tpn.x == tpn.x && tpn.y == tpn.y && tpn.z == tpn.z)) {
+          active = false;
+          goto pack;
+        }
+        tp = tpn;
+        prev_pdf = bs.is_specular ? T(-1) : bs.pdf;
+        depth += 1;
+        const V3<T> wi = normalize(bs.wi);
+        const T side = dot(wi, hr.n) < T(0) ? T(-1) : T(1);
+        const bool through = (dot(wi, hr.n) < T(0)) != (dot(wo, hr.n) < T(0));
+        const bool exceeded = through ? count_bounce(kinds, 16u, rp.max_transmission_bounces)
+                                      : (bs.is_diffuse ? count_bounce(kinds, 0u, rp.max_diffuse_bounces)
+                                                       : count_bounce(kinds, 8u, rp.max_glossy_bounces));
+        if (exceeded) kinds |= PATH_EMISSION_ONLY;
+        ro = p + hr.n * (side * eps);
+        rd = wi;
+        rtmin = T(0);
+        rtmax = T(1e30);
+      }
+    pack:
+      // trace_occluded + pt_nee_accum_pack: pathtracer.cpp:1092-1121
+      if (nee) {
+        const V3<float> sof(so), sdf(sd);
+        if (!occluded(s, sof, sdf, float(stmin), float(stmax))) {
+          V3<T> c = nee_c;
+          if (has_vol) {
+            Rng trng = make_rng(idx, sample, 0x10000u + it);
+            c = c * s.vol.transmittance<T>(so, sd, stmin, stmax, trng);
+          }
+          if (c.x == c.x && c.y == c.y && c.z == c.z) out.accum = out.accum + c;
+        }
+      }
+    }
+  }
+}
+
+// AOVs are written by a separate, simpler pass: the first camera-ray event of
+// the first sample (pathtracer.cpp:800-809, 974-983).  They do not feed the
+// color, so the color pass above leaves them out.
+template <class T> static void pixel_aov(const Scene &s, const Params &P, uint32_t idx, PixelOut<T> &out) {
+  const phx_render_params &rp = P.p;
+  const uint32_t w = rp.width, h = rp.height, x = idx % w, y = idx / w;
+  const uint32_t max_depth = rp.max_depth > 0 ? rp.max_depth : 1u;
+  const uint32_t max_it = max_depth + 1u + 8u;
+  const T eps = T(s.ray_eps);
+  for (uint32_t sidx = 0; sidx < rp.samples_per_pixel; ++sidx) {
+    const uint32_t sample = sidx + rp.sample_offset;
+    Rng crng = make_rng(idx, sample, 0u);
+    T u = (T(x) + draw<T>(crng)) / T(w);
+    T v = (T(y) + draw<T>(crng)) / T(h);
+    if (!rp.bottom_up) v = T(1) - v;
+    u = T(rp.image_region[0]) + (T(rp.image_region[2]) - T(rp.image_region[0])) * u;
+    v = T(rp.image_region[1]) + (T(rp.image_region[3]) - T(rp.image_region[1])) * v;
+    V3<T> ro, rd;
+    camera_ray<T>(P.cam, u, v, crng, ro, rd);
+    T rtmin, rtmax;
+    clip_range<T>(P.cam, rd, rtmin, rtmax);
+    T path_dist = T(0);
+    for (uint32_t it = 0; it < max_it; ++it) {
+      const Hit<T> hr = closest<T>(s, ro, rd, rtmin, rtmax);
+      Rng rng = make_rng(idx, sample, 1u + it);
+      if (!s.vol.empty()) {
+        V3<T> alb{};
+        const T ts = s.vol.sample_scatter<T>(ro, rd, rtmin, hr.hit ? hr.t : rtmax, rng, alb);
+        if (ts >= T(0)) {
+          out.depth = path_dist + ts;
+          out.normal = -rd;
+          out.albedo = alb;
+          return;
+        }
+      }
+      if (!hr.hit) break;
+      V3<T> hsn = hr.sn;
+      if (!(dot(hsn, hsn) > T(0.25))) hsn = hr.n;
+      else if (dot(hsn, hr.n) < T(0)) hsn = hsn * T(-1);
+      T alpha = T(1);
+      const Mat<T> mat = resolve_material<T>(s, hr.mat, hr.uv, hr.has_color, hr.color, alpha, nullptr);
+      const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0;
+      if (hidden || (alpha < T(1) && draw<T>(rng) >= alpha)) {
+        ro = hr.p;
+        rtmin = eps;
+        rtmax = T(1e30);
+        path_dist += hr.t;
+        T tn, tf;
+        clip_range<T>(P.cam, rd, tn, tf);
+        if (tf < T(1e30)) {
+          rtmax = tf - path_dist;
+          if (rtmax < eps) break;
+        }
+        continue;
+      }
+      V3<T> sn = hsn;
+      const V3<T> wo = -rd;
+      if (!(mat.transmission > T(0)) && !(dot(hr.n, wo) >= T(0))) sn = sn * T(-1);
+      if (mat.normal_tex >= 0 && !s.atlas.infos.empty()) {
+        V3<T> nm = s.atlas.sample<T>(mat.normal_tex, hr.uv.x, hr.uv.y);
+        nm = nm * T(2) - V3<T>{T(1), T(1), T(1)};
+        V3<T> ta, bi;
+        onb_from_normal(sn, ta, bi);
+        sn = normalize(ta * nm.x + bi * nm.y + sn * nm.z);
+      }
+      out.depth = path_dist + hr.t;
+      out.normal = normalize(sn);
+      out.albedo = mat.base_color;
+      return;
+    }
+  }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ---- extern "C" API used by the tests -------------------------------------
+extern "C" {
+
+void *orc_scene_create(const phx_scene_desc *d, int brute) {
+  auto *s = new Scene();
+  scene_load(*s, *d);
+  s->brute = brute != 0;
+  return s;
+}
+void orc_scene_destroy(void *s) { delete static_cast<Scene *>(s); }
+float orc_scene_ray_epsilon(void *s) { return static_cast<Scene *>(s)->ray_eps; }
+uint32_t orc_scene_bvh_nodes(void *s) { return uint32_t(static_cast<Scene *>(s)->bvh.nodes.size()); }
+uint32_t orc_scene_emissive(void *sp, float *cdf, float *prim_pdf, uint32_t *ids) {
+  Scene *s = static_cast<Scene *>(sp);
+  if (cdf) std::copy(s->emissive_cdf.begin(), s->emissive_cdf.end(), cdf);
+  if (prim_pdf) std::copy(s->prim_light_pdf.begin(), s->prim_light_pdf.end(), prim_pdf);
+  if (ids) std::copy(s->emissive_ids.begin(), s->emissive_ids.begin() + s->emissive_count, ids);
+  return s->emissive_count;
+}
+
+// Closest hit of n rays: t, prim (0xffffffff: none), b1, b2.  mode 0: the
+// reference BVH, 1: brute force.
+void orc_trace_closest(void *sp, const float *org, const float *dir, const float *tmin, const float *tmax, uint32_t n,
+                       float *t_out, uint32_t *prim_out, float *u_out, float *v_out, int mode) {
+  Scene *s = static_cast<Scene *>(sp);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < int64_t(n); ++i) {
+    const V3<float> o = v3<float>(&org[i * 3]), d = v3<float>(&dir[i * 3]);
+    const ClosestHit h = mode ? trace_brute(s->mesh, o, d, tmin[i], tmax[i], false)
+                              : trace_bvh(s->mesh, s->bvh, o, d, tmin[i], tmax[i], false);
+    t_out[i] = h.hit ? h.t : 0.f;
+    prim_out[i] = h.hit ? h.prim : 0xffffffffu;
+    u_out[i] = h.hit ? h.u : 0.f;
+    v_out[i] = h.hit ? h.v : 0.f;
+  }
+}
+// Full HitResult of n rays (mesh_intersector.h:126-178) in phx_hit layout.
+void orc_trace_closest_full(void *sp, const float *org, const float *dir, const float *tmin, const float *tmax, uint32_t n,
+                            phx_hit *out) {
+  Scene *s = static_cast<Scene *>(sp);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < int64_t(n); ++i) {
+    const Hit<float> h = closest<float>(*s, v3<float>(&org[i * 3]), v3<float>(&dir[i * 3]), tmin[i], tmax[i]);
+    phx_hit r{};
+    r.hit = h.hit;
+    if (h.hit) {
+      r.t = h.t;
+      r.position[0] = h.p.x; r.position[1] = h.p.y; r.position[2] = h.p.z;
+      r.normal[0] = h.n.x; r.normal[1] = h.n.y; r.normal[2] = h.n.z;
+      r.shading_normal[0] = h.sn.x; r.shading_normal[1] = h.sn.y; r.shading_normal[2] = h.sn.z;
+      r.uv[0] = h.uv.x; r.uv[1] = h.uv.y;
+      r.bary[0] = h.bary.x; r.bary[1] = h.bary.y;
+      r.interpolated_color[0] = h.color.x; r.interpolated_color[1] = h.color.y; r.interpolated_color[2] = h.color.z;
+      r.prim_id = h.prim;
+      r.material_id = h.mat;
+      r.has_interpolated_color = h.has_color;
+    }
+    out[i] = r;
+  }
+}
+void orc_trace_occluded(void *sp, const float *org, const float *dir, const float *tmin, const float *tmax, uint32_t n,
+                        uint32_t *occ, int mode) {
+  Scene *s = static_cast<Scene *>(sp);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < int64_t(n); ++i) {
+    const V3<float> o = v3<float>(&org[i * 3]), d = v3<float>(&dir[i * 3]);
+    const ClosestHit h = mode ? trace_brute(s->mesh, o, d, tmin[i], tmax[i], true)
+                              : trace_bvh(s->mesh, s->bvh, o, d, tmin[i], tmax[i], true);
+    occ[i] = h.hit ? 1u : 0u;
+  }
+}
+
+// Disney BSDF on n (material, wo, wi, n, sn) tuples: f*cos (3) and pdf.
+void orc_bsdf_eval(const phx_material *mats, const float *wo, const float *wi, const float *ng, const float *sn, uint32_t n,
+                   float *f_out, float *pdf_out, int use_double) {
+  for (uint32_t i = 0; i < n; ++i) {
+    if (use_double) {
+      const Mat<double> m = mat_from<double>(mats[i]);
+      const V3<double> a = v3<double>(&wo[i * 3]), b = v3<double>(&wi[i * 3]), c = v3<double>(&ng[i * 3]), d = v3<double>(&sn[i * 3]);
+      const V3<double> f = disney_bsdf_eval(m, a, b, c, d);
+      f_out[i * 3] = float(f.x); f_out[i * 3 + 1] = float(f.y); f_out[i * 3 + 2] = float(f.z);
+      pdf_out[i] = float(disney_bsdf_pdf(m, a, b, c, d));
+    } else {
+      const Mat<float> m = mat_from<float>(mats[i]);
+      const V3<float> a = v3<float>(&wo[i * 3]), b = v3<float>(&wi[i * 3]), c = v3<float>(&ng[i * 3]), d = v3<float>(&sn[i * 3]);
+      const V3<float> f = disney_bsdf_eval(m, a, b, c, d);
+      f_out[i * 3] = f.x; f_out[i * 3 + 1] = f.y; f_out[i * 3 + 2] = f.z;
+      pdf_out[i] = disney_bsdf_pdf(m, a, b, c, d);
+    }
+  }
+}
+// disney_bsdf_sample with a per-tuple seed: wi (3), f (3), pdf, flags
+// (bit 0 specular, bit 1 diffuse).
+void orc_bsdf_sample(const phx_material *mats, const float *wo, const float *ng, const float *sn, const uint32_t *seeds,
+                     uint32_t n, float *wi_out, float *f_out, float *pdf_out, uint32_t *flags_out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const Mat<float> m = mat_from<float>(mats[i]);
+    Rng rng(seeds[i]);
+    const BsdfSample<float> s =
+        disney_bsdf_sample(m, v3<float>(&wo[i * 3]), v3<float>(&ng[i * 3]), v3<float>(&sn[i * 3]), rng);
+    wi_out[i * 3] = s.wi.x; wi_out[i * 3 + 1] = s.wi.y; wi_out[i * 3 + 2] = s.wi.z;
+    f_out[i * 3] = s.f.x; f_out[i * 3 + 1] = s.f.y; f_out[i * 3 + 2] = s.f.z;
+    pdf_out[i] = s.pdf;
+    flags_out[i] = (s.is_specular ? 1u : 0u) | (s.is_diffuse ? 2u : 0u);
+  }
+}
+
+void orc_rng(uint32_t pixel, uint32_t sample, uint32_t dim, uint32_t n, uint32_t *u32_out, float *f32_out) {
+  Rng r = make_rng(pixel, sample, dim);
+  Rng r2 = r;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (u32_out) u32_out[i] = r.next_u32();
+    if (f32_out) f32_out[i] = r2.next_f32();
+  }
+}
+uint32_t orc_hash(uint32_t x) { return hash_u32(x); }
+float orc_power_heuristic(float a, float b) { return power_heuristic(1, a, 1, b); }
+void orc_onb(const float *n, float *t, float *b) {
+  V3<float> tt, bb;
+  onb_from_normal(v3<float>(n), tt, bb);
+  t[0] = tt.x; t[1] = tt.y; t[2] = tt.z;
+  b[0] = bb.x; b[1] = bb.y; b[2] = bb.z;
+}
+// Environment map: sample (dir, pdf) with seeds, pdf(dir), evaluate(dir).
+void orc_env_sample(void *sp, const uint32_t *seeds, uint32_t n, float *dir, float *pdf) {
+  Scene *s = static_cast<Scene *>(sp);
+  for (uint32_t i = 0; i < n; ++i) {
+    Rng r(seeds[i]);
+    float p = 0.f;
+    const V3<float> d = s->env.sample_direction<float>(r, p);
+    dir[i * 3] = d.x; dir[i * 3 + 1] = d.y; dir[i * 3 + 2] = d.z;
+    pdf[i] = p;
+  }
+}
+void orc_env_eval(void *sp, const float *dir, uint32_t n, float *pdf, float *rgb) {
+  Scene *s = static_cast<Scene *>(sp);
+  for (uint32_t i = 0; i < n; ++i) {
+    const V3<float> d = v3<float>(&dir[i * 3]);
+    if (pdf) pdf[i] = s->env.pdf<float>(d);
+    if (rgb) {
+      const V3<float> c = s->env.evaluate<float>(d);
+      rgb[i * 3] = c.x; rgb[i * 3 + 1] = c.y; rgb[i * 3 + 2] = c.z;
+    }
+  }
+}
+void orc_env_cdf(void *sp, float *marginal, float *conditional) {
+  Scene *s = static_cast<Scene *>(sp);
+  std::copy(s->env.marginal.begin(), s->env.marginal.end(), marginal);
+  std::copy(s->env.conditional.begin(), s->env.conditional.end(), conditional);
+}
+void orc_camera_perspective(phx_camera *out, const float *from, const float *at, const float *up, float vfov, float aspect,
+                            float aperture, float focus) {
+  *out = make_perspective(v3<float>(from), v3<float>(at), v3<float>(up), vfov, aspect, aperture, focus);
+}
+uint32_t orc_volume_majorants(phx_volume *v, const float *values, const float *tf, uint32_t tf_size, float *out,
+                              uint32_t cap) {
+  std::vector<float> m;
+  const uint32_t n = build_majorants(*v, values, tf, tf_size, m);
+  if (out) std::copy(m.begin(), m.begin() + std::min(n, cap), out);
+  return n;
+}
+void orc_volume_transmittance(void *sp, const float *org, const float *dir, float tmin, float tmax, const uint32_t *seeds,
+                              uint32_t n, float *tr) {
+  Scene *s = static_cast<Scene *>(sp);
+  for (uint32_t i = 0; i < n; ++i) {
+    Rng r(seeds[i]);
+    tr[i] = s->vol.transmittance<float>(v3<float>(org), v3<float>(dir), tmin, tmax, r);
+  }
+}
+void orc_volume_scatter(void *sp, const float *org, const float *dir, float tmin, float tmax, const uint32_t *seeds,
+                        uint32_t n, float *t_out) {
+  Scene *s = static_cast<Scene *>(sp);
+  for (uint32_t i = 0; i < n; ++i) {
+    Rng r(seeds[i]);
+    V3<float> alb{};
+    t_out[i] = s->vol.sample_scatter<float>(v3<float>(org), v3<float>(dir), tmin, tmax, r, alb);
+  }
+}
+
+// PathTracer::render (pathtracer.cpp:564-1159) on the CPU with `threads`
+// OpenMP threads (0: all).  Outputs are W*H rows as rendered (row y = idx / W):
+// color rgb, alpha, depth, normal, albedo (any may be NULL).  `pixel_begin`,
+// `pixel_end` restrict the work to a pixel range (bounded CPU baselines).
+// Returns the wall time in seconds.
+double orc_render(void *sp, const phx_camera *cam, const phx_render_params *rp, float *color, float *alpha, float *depth,
+                  float *normal, float *albedo, int threads, int use_double, uint32_t pixel_begin, uint32_t pixel_end) {
+  Scene *s = static_cast<Scene *>(sp);
+  Params P{*rp, *cam};
+  const uint32_t npix = rp->width * rp->height;
+  if (pixel_end == 0 || pixel_end > npix) pixel_end = npix;
+  if (threads > 0) omp_set_num_threads(threads);
+  const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = int64_t(pixel_begin); i < int64_t(pixel_end); ++i) {
+    const uint32_t idx = uint32_t(i);
+    float c[3], a, d, nn[3], al[3];
+    if (use_double) {
+      PixelOut<double> o;
+      render_pixel<double>(*s, P, idx, o, nullptr);
+      pixel_aov<double>(*s, P, idx, o);
+      const double inv = o.ns ? 1.0 / double(o.ns) : 0.0;
+      c[0] = float(o.accum.x * inv); c[1] = float(o.accum.y * inv); c[2] = float(o.accum.z * inv);
+      a = float(o.alpha * inv);
+      d = float(o.depth);
+      nn[0] = float(o.normal.x); nn[1] = float(o.normal.y); nn[2] = float(o.normal.z);
+      al[0] = float(o.albedo.x); al[1] = float(o.albedo.y); al[2] = float(o.albedo.z);
+    } else {
+      PixelOut<float> o;
+      render_pixel<float>(*s, P, idx, o, nullptr);
+      pixel_aov<float>(*s, P, idx, o);
+      const float inv = o.ns ? 1.f / float(o.ns) : 0.f;
+      c[0] = o.accum.x * inv; c[1] = o.accum.y * inv; c[2] = o.accum.z * inv;
+      a = o.alpha * inv;
+      d = o.depth;
+      nn[0] = o.normal.x; nn[1] = o.normal.y; nn[2] = o.normal.z;
+      al[0] = o.albedo.x; al[1] = o.albedo.y; al[2] = o.albedo.z;
+    }
+    if (color) { color[idx * 3] = c[0]; color[idx * 3 + 1] = c[1]; color[idx * 3 + 2] = c[2]; }
+    if (alpha) alpha[idx] = a;
+    if (depth) depth[idx] = d;
+    if (normal) { normal[idx * 3] = nn[0]; normal[idx * 3 + 1] = nn[1]; normal[idx * 3 + 2] = nn[2]; }
+    if (albedo) { albedo[idx * 3] = al[0]; albedo[idx * 3 + 1] = al[1]; albedo[idx * 3 + 2] = al[2]; }
+  }
+  return omp_get_wtime() - t0;
+}
+
+}  // extern "C"

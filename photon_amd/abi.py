@@ -1,0 +1,226 @@
+"""ctypes mirrors of the C ABI structs declared in include/phx.h.
+
+The field order and sizes are those of the header; ``check_layout`` asserts the
+sizes against the values the header fixes, so a drift fails loudly.
+"""
+import ctypes as C
+
+F3 = C.c_float * 3
+
+
+class Material(C.Structure):
+    """pt::Material (reference include/photon/pt/material.h:35-82), no ParamSources."""
+    _fields_ = [
+        ("base_color", F3), ("metallic", C.c_float),
+        ("roughness", C.c_float), ("ior", C.c_float), ("transmission", C.c_float), ("specular", C.c_float),
+        ("clearcoat", C.c_float), ("clearcoat_roughness", C.c_float), ("sheen", C.c_float), ("sheen_tint", C.c_float),
+        ("specular_tint", C.c_float), ("anisotropic", C.c_float), ("emission_strength", C.c_float),
+        ("subsurface", C.c_float),
+        ("emission", F3), ("subsurface_radius", C.c_float),
+        ("subsurface_color", F3), ("alpha", C.c_float),
+        ("thin", C.c_int32), ("flags", C.c_int32), ("alpha_mode", C.c_int32), ("alpha_cutoff", C.c_float),
+        ("base_color_tex", C.c_int32), ("normal_tex", C.c_int32), ("roughness_tex", C.c_int32),
+        ("metallic_tex", C.c_int32), ("emission_tex", C.c_int32), ("alpha_tex", C.c_int32),
+        ("pad_", C.c_int32 * 2),
+    ]
+
+
+class Light(C.Structure):
+    """pt::Light (light.h:19-35)."""
+    _fields_ = [
+        ("type", C.c_uint32), ("position", F3), ("direction", F3), ("color", F3), ("intensity", C.c_float),
+        ("spot_angle", C.c_float), ("spot_falloff", C.c_float), ("mesh_prim_begin", C.c_uint32),
+        ("mesh_prim_count", C.c_uint32), ("area", C.c_float), ("edge1", F3), ("edge2", F3), ("radius", C.c_float),
+        ("pad_", C.c_uint32),
+    ]
+
+
+class TextureInfo(C.Structure):
+    _fields_ = [("offset", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class Volume(C.Structure):
+    """pt::TfVolume (tf_volume.h:14-33)."""
+    _fields_ = [
+        ("world_to_local", C.c_float * 12), ("origin", F3), ("spacing", F3), ("dims", C.c_uint32 * 3),
+        ("data_offset", C.c_uint32), ("tf_offset", C.c_uint32), ("tf_size", C.c_uint32),
+        ("value_lo", C.c_float), ("value_hi", C.c_float), ("density_scale", C.c_float), ("majorant", C.c_float),
+        ("linear", C.c_uint32), ("macro_offset", C.c_uint32), ("macro_dims", C.c_uint32 * 3),
+        ("pad_", C.c_uint32 * 3),
+    ]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("positions", C.POINTER(C.c_float)), ("vertex_count", C.c_uint32),
+        ("indices", C.POINTER(C.c_uint32)), ("triangle_count", C.c_uint32),
+        ("normals", C.POINTER(C.c_float)), ("texcoords", C.POINTER(C.c_float)),
+        ("vertex_colors", C.POINTER(C.c_float)), ("material_ids", C.POINTER(C.c_uint32)),
+        ("materials", C.POINTER(Material)), ("material_count", C.c_uint32),
+        ("lights", C.POINTER(Light)), ("light_count", C.c_uint32),
+        ("emissive_prim_ids", C.POINTER(C.c_uint32)), ("emissive_prim_areas", C.POINTER(C.c_float)),
+        ("emissive_count", C.c_uint32),
+        ("texture_pixels", C.POINTER(C.c_float)), ("texture_pixel_count", C.c_uint32),
+        ("texture_infos", C.POINTER(TextureInfo)), ("texture_count", C.c_uint32),
+        ("env_width", C.c_uint32), ("env_height", C.c_uint32), ("env_pixels", C.POINTER(C.c_float)),
+        ("env_world_to_tex", C.c_float * 9), ("env_visible", C.c_int32),
+        ("volumes", C.POINTER(Volume)), ("volume_count", C.c_uint32),
+        ("volume_values", C.POINTER(C.c_float)), ("volume_value_count", C.c_uint64),
+        ("volume_tf", C.POINTER(C.c_float)), ("volume_tf_count", C.c_uint32),
+        ("volume_majorants", C.POINTER(C.c_float)), ("volume_majorant_count", C.c_uint32),
+        ("ray_epsilon", C.c_float),
+    ]
+
+
+class Camera(C.Structure):
+    """pt::Camera (camera.h:11-26), derived form."""
+    _fields_ = [
+        ("origin", F3), ("lower_left", F3), ("horizontal", F3), ("vertical", F3),
+        ("u", F3), ("v", F3), ("w", F3),
+        ("lens_radius", C.c_float), ("focus_dist", C.c_float), ("is_ortho", C.c_int32),
+        ("clip_near", C.c_float), ("clip_far", C.c_float),
+    ]
+
+
+class RenderParams(C.Structure):
+    """pt::RenderParams (pathtracer.h:16-55) without the background image."""
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("samples_per_pixel", C.c_uint32),
+        ("max_depth", C.c_uint32), ("sample_offset", C.c_uint32),
+        ("background_color", F3), ("background_alpha", C.c_float),
+        ("ambient_color", F3), ("ambient_radiance", C.c_float),
+        ("adaptive_sampling", C.c_int32), ("throughput_clamp", C.c_float),
+        ("max_diffuse_bounces", C.c_int32), ("max_glossy_bounces", C.c_int32),
+        ("max_transmission_bounces", C.c_int32), ("max_volume_bounces", C.c_int32),
+        ("clamp_direct_sum", C.c_float), ("clamp_indirect_sum", C.c_float),
+        ("bottom_up", C.c_int32), ("image_region", C.c_float * 4),
+    ]
+
+
+class Hit(C.Structure):
+    """pt::HitResult (ray_backend.h:15-29)."""
+    _fields_ = [
+        ("t", C.c_float), ("position", F3), ("normal", F3), ("shading_normal", F3),
+        ("uv", C.c_float * 2), ("bary", C.c_float * 2), ("interpolated_color", F3),
+        ("prim_id", C.c_uint32), ("geom_id", C.c_uint32), ("inst_id", C.c_uint32), ("material_id", C.c_uint32),
+        ("hit", C.c_uint32), ("has_interpolated_color", C.c_uint32), ("pad_", C.c_uint32),
+    ]
+
+
+class HitCompact(C.Structure):
+    _fields_ = [("t", C.c_float), ("prim_id", C.c_uint32), ("b1", C.c_float), ("b2", C.c_float)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("trace_block", C.c_uint32), ("shade_block", C.c_uint32), ("paths_in_flight", C.c_uint32),
+        ("sort_by_material", C.c_uint32), ("use_graph", C.c_uint32), ("time_kernels", C.c_uint32),
+        ("pad_", C.c_uint32 * 2),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("shade_events", C.c_uint64),
+        ("samples", C.c_uint64), ("pixels", C.c_uint64), ("scene_bytes", C.c_uint64),
+        ("bvh_nodes", C.c_uint64), ("bvh_bytes", C.c_uint64), ("last_render_ms", C.c_double),
+        ("extend_launches", C.c_uint64), ("extend_ms", C.c_double), ("shade_launches", C.c_uint64),
+        ("shade_ms", C.c_double), ("shadow_launches", C.c_uint64), ("shadow_ms", C.c_double),
+    ]
+
+
+class CameraData(C.Structure):
+    """bridge::CameraData (RenderBridge.h:128-144)."""
+    _fields_ = [
+        ("position", F3), ("direction", F3), ("up", F3), ("orthographic", C.c_int32),
+        ("fovy", C.c_float), ("height", C.c_float), ("aspect", C.c_float),
+        ("aperture_radius", C.c_float), ("focus_distance", C.c_float), ("image_region", C.c_float * 4),
+        ("clip_near", C.c_float), ("clip_far", C.c_float),
+    ]
+
+
+class RenderSettings(C.Structure):
+    """bridge::RenderSettings (RenderBridge.h:146-170) without the background image."""
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("pixel_samples", C.c_uint32), ("max_depth", C.c_uint32),
+        ("background", C.c_float * 4), ("ambient_color", F3), ("ambient_radiance", C.c_float),
+        ("adaptive_sampling", C.c_int32), ("throughput_clamp", C.c_float),
+        ("max_diffuse_bounces", C.c_int32), ("max_glossy_bounces", C.c_int32),
+        ("max_transmission_bounces", C.c_int32), ("max_volume_bounces", C.c_int32),
+        ("clamp_direct", C.c_float), ("clamp_indirect", C.c_float),
+    ]
+
+
+class View(C.Structure):
+    _fields_ = [
+        ("camera", Camera), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+        ("sample_offset", C.c_uint32), ("scene_slot", C.c_uint32), ("pad_", C.c_uint32 * 3),
+    ]
+
+
+EXPECTED_SIZES = {Material: 144, Light: 96, TextureInfo: 16, Volume: 144, Hit: 96, HitCompact: 16, Camera: 104}
+
+
+def check_layout():
+    for cls, size in EXPECTED_SIZES.items():
+        got = C.sizeof(cls)
+        if got != size:
+            raise AssertionError(f"{cls.__name__}: ctypes size {got} != {size} fixed by include/phx.h")
+
+
+PHX_MATERIAL_FLAG_EMITTER_ONLY = 1
+PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA = 2
+PHX_MATERIAL_FLAG_EMIT_BOTH_SIDES = 4
+PHX_MATERIAL_FLAG_REFRACTIVE = 8
+PHX_MATERIAL_FLAG_EMIT_BACK_ONLY = 16
+PHX_ALPHA_BLEND, PHX_ALPHA_OPAQUE, PHX_ALPHA_MASK = 0, 1, 2
+PHX_LIGHT_POINT, PHX_LIGHT_DIRECTIONAL, PHX_LIGHT_SPOT, PHX_LIGHT_AREA = 0, 1, 2, 3
+
+
+def default_material():
+    """pt::Material{} defaults (material.h:35-82)."""
+    m = Material()
+    m.base_color[:] = (0.8, 0.8, 0.8)
+    m.metallic = 0.0
+    m.roughness = 0.5
+    m.ior = 1.5
+    m.transmission = 0.0
+    m.specular = 0.5
+    m.clearcoat = 0.0
+    m.clearcoat_roughness = 0.03
+    m.sheen = 0.0
+    m.sheen_tint = 0.5
+    m.specular_tint = 0.0
+    m.anisotropic = 0.0
+    m.emission[:] = (0.0, 0.0, 0.0)
+    m.emission_strength = 0.0
+    m.subsurface = 0.0
+    m.subsurface_color[:] = (1.0, 1.0, 1.0)
+    m.subsurface_radius = 1.0
+    m.alpha = 1.0
+    m.thin = 0
+    m.flags = 0
+    m.alpha_mode = PHX_ALPHA_BLEND
+    m.alpha_cutoff = 0.5
+    for k in ("base_color_tex", "normal_tex", "roughness_tex", "metallic_tex", "emission_tex", "alpha_tex"):
+        setattr(m, k, -1)
+    return m
+
+
+def default_render_params(width, height, spp, max_depth):
+    """RenderParams with the settings every parity run fixes: adaptive sampling
+    off, no throughput clamp, no light clamps, unlimited bounce kinds."""
+    p = RenderParams()
+    p.width, p.height, p.samples_per_pixel, p.max_depth = width, height, spp, max_depth
+    p.sample_offset = 0
+    p.background_color[:] = (0.0, 0.0, 0.0)
+    p.background_alpha = 1.0
+    p.ambient_color[:] = (1.0, 1.0, 1.0)
+    p.ambient_radiance = 0.0
+    p.adaptive_sampling = 0
+    p.throughput_clamp = 0.0
+    p.max_diffuse_bounces = p.max_glossy_bounces = p.max_transmission_bounces = p.max_volume_bounces = -1
+    p.clamp_direct_sum = p.clamp_indirect_sum = 0.0
+    p.bottom_up = 0
+    p.image_region[:] = (0.0, 0.0, 1.0, 1.0)
+    return p

@@ -1,0 +1,670 @@
+// C ABI of the gfx950 path-tracing backend (include/phx.h): context, scene
+// upload, light tables, the wavefront render loop and the render-bridge
+// accumulation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/phx.h"
+#include "bvh_build.h"
+#include "context.h"
+#include "dev_trace.h"
+#include "integrator.h"
+
+namespace {
+
+using namespace phx;
+using namespace phx_detail;
+
+float length3(const float *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+// Emissive-triangle test of the render bridge (material.h:97-103).
+bool may_emit(const phx_material &m) {
+  if (m.emission_strength <= 0.f) return false;
+  return m.emission_tex >= 0 || m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f;
+}
+
+// Marginal/conditional CDFs of an equirect map (environment_map.cpp:8-65),
+// single precision like the reference.
+void env_cdf(const float *px, uint32_t w, uint32_t h, std::vector<float> &marg, std::vector<float> &cond) {
+  const float PI = 3.14159265358979323846f;
+  marg.assign(h + 1, 0.f);
+  cond.assign(size_t(h) * (w + 1), 0.f);
+  std::vector<float> roww(h, 0.f);
+  for (uint32_t y = 0; y < h; ++y) {
+    float *c = &cond[size_t(y) * (w + 1)];
+    const float theta = (float(y) + 0.5f) * (PI / float(h));
+    const float st = std::sin(theta);
+    float sum = 0.f;
+    for (uint32_t x = 0; x < w; ++x) {
+      const float *p = px + (size_t(y) * w + x) * 3;
+      sum += 0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2];
+      c[x + 1] = sum;
+    }
+    if (sum > 0.f) {
+      const float inv = 1.f / sum;
+      for (uint32_t x = 0; x <= w; ++x) c[x] *= inv;
+      c[w] = 1.f;
+    } else {
+      for (uint32_t x = 0; x <= w; ++x) c[x] = float(x) / float(w);
+    }
+    roww[y] = sum * st;
+  }
+  float total = 0.f;
+  for (uint32_t y = 0; y < h; ++y) {
+    total += roww[y];
+    marg[y + 1] = total;
+  }
+  if (total > 0.f) {
+    const float inv = 1.f / total;
+    for (uint32_t y = 0; y <= h; ++y) marg[y] *= inv;
+    marg[h] = 1.f;
+  } else {
+    for (uint32_t y = 0; y <= h; ++y) marg[y] = float(y) / float(h);
+  }
+}
+
+int ensure_trace_scratch(phx_context *c) {
+  if (c->trace_grid == 0) c->trace_grid = uint32_t(c->cu_count) * 8u;
+  HIP_TRY(c, c->spill.reserve(size_t(c->trace_grid) * 256 * (kMaxStack - kShortStack) * sizeof(int)));
+  HIP_TRY(c, c->fetch.reserve(64));
+  return PHX_OK;
+}
+
+bool ray_args_ok(const float *a, const float *b, const float *c, const float *d) { return a && b && c && d; }
+
+}  // namespace
+
+extern "C" {
+
+const char *phx_backend_name(void) { return "hip"; }
+int phx_abi_version(void) { return PHX_ABI_VERSION; }
+
+int phx_create(phx_context **out, int device) {
+  if (!out) return PHX_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PHX_ERR_HIP;
+  if (device < 0 || device >= n) return PHX_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return PHX_ERR_HIP;
+  auto *c = new phx_context();
+  c->device = device;
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return PHX_ERR_HIP;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->cu_count = cus;
+  c->tuning.trace_block = 256;
+  c->tuning.shade_block = 256;
+  c->tuning.paths_in_flight = 0;
+  c->tuning.sort_by_material = 0;
+  c->tuning.use_graph = 0;
+  *out = c;
+  return PHX_OK;
+}
+
+void phx_destroy(phx_context *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  // DBuf members free their memory in the destructor.
+  hipStream_t s = c->stream;
+  for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+const char *phx_last_error(const phx_context *c) { return c ? c->error.c_str() : "no context"; }
+
+int phx_set_tuning(phx_context *c, const phx_tuning *t) {
+  if (!c || !t) return PHX_ERR_INVALID;
+  if (t->trace_block != 0 && t->trace_block != 256) return fail(c, PHX_ERR_UNSUPPORTED, "trace_block must be 256");
+  if (t->shade_block != 0 && t->shade_block != 256) return fail(c, PHX_ERR_UNSUPPORTED, "shade_block must be 256");
+  c->tuning = *t;
+  if (!c->tuning.trace_block) c->tuning.trace_block = 256;
+  if (!c->tuning.shade_block) c->tuning.shade_block = 256;
+  return PHX_OK;
+}
+int phx_get_tuning(const phx_context *c, phx_tuning *t) {
+  if (!c || !t) return PHX_ERR_INVALID;
+  *t = c->tuning;
+  return PHX_OK;
+}
+int phx_get_stats(const phx_context *c, phx_stats *s) {
+  if (!c || !s) return PHX_ERR_INVALID;
+  *s = c->stats;
+  return PHX_OK;
+}
+int phx_synchronize(phx_context *c) {
+  if (!c) return PHX_ERR_INVALID;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PHX_OK;
+}
+
+int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
+  if (!c || !d) return PHX_ERR_INVALID;
+  if (!d->positions || !d->indices || d->vertex_count == 0 || d->triangle_count == 0)
+    return fail(c, PHX_ERR_INVALID, "build_accel: empty mesh");
+  if (d->triangle_count >= (1u << 28)) return fail(c, PHX_ERR_INVALID, "build_accel: more than 2^28 triangles");
+  HIP_TRY(c, hipSetDevice(c->device));
+  c->have_accel = false;
+  c->have_scene = false;
+  c->scene_bytes = 0;
+  Bvh4 bvh;
+  std::string err;
+  Bvh4Options opt;
+  if (build_bvh4(d->positions, d->vertex_count, d->indices, d->triangle_count, opt, bvh, err) != 0)
+    return fail(c, PHX_ERR_INVALID, err);
+  // Stack bound: a visit pushes at most 3 entries per level below the root.
+  if (3 * bvh.depth + 1 > uint32_t(kMaxStack))
+    return fail(c, PHX_ERR_OVERFLOW, "build_accel: BVH too deep for the 64-entry traversal stack");
+  c->bvh_depth = bvh.depth;
+  int r;
+  if ((r = upload(c, c->nodes, bvh.nodes.data(), bvh.nodes.size()))) return r;
+  if ((r = upload(c, c->tris, bvh.tris.data(), bvh.tris.size()))) return r;
+  if ((r = upload(c, c->indices, d->indices, size_t(d->triangle_count) * 3))) return r;
+  if ((r = upload(c, c->positions, d->positions, size_t(d->vertex_count) * 3))) return r;
+  if ((r = upload(c, c->normals, d->normals, d->normals ? size_t(d->vertex_count) * 3 : 0))) return r;
+  if ((r = upload(c, c->texcoords, d->texcoords, d->texcoords ? size_t(d->vertex_count) * 2 : 0))) return r;
+  if ((r = upload(c, c->colors, d->vertex_colors, d->vertex_colors ? size_t(d->vertex_count) * 3 : 0))) return r;
+  if (d->material_ids) {
+    std::vector<uint32_t> mids(d->material_ids, d->material_ids + d->triangle_count);
+    const uint32_t mc = std::max<uint32_t>(d->material_count, 1);
+    for (auto &m : mids)
+      if (m >= mc) m = 0;  // out-of-range ids fall back to material 0 (RenderBridge.cpp:479-484)
+    if ((r = upload(c, c->material_ids, mids.data(), mids.size()))) return r;
+  } else {
+    c->material_ids.release();
+  }
+  c->triangle_count = d->triangle_count;
+  c->vertex_count = d->vertex_count;
+  // Bounds of the triangles' vertices (the BVH root box).
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = 0; i < d->triangle_count * 3; ++i) {
+    const float *p = d->positions + size_t(d->indices[i]) * 3;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], p[a]);
+      hi[a] = std::max(hi[a], p[a]);
+    }
+  }
+  std::memcpy(c->bounds_lo, lo, sizeof lo);
+  std::memcpy(c->bounds_hi, hi, sizeof hi);
+  DevScene &s = c->dev;
+  s = DevScene{};
+  s.nodes = c->nodes.as<Node4>();
+  s.tris = c->tris.as<TriRec>();
+  s.indices = c->indices.as<uint32_t>();
+  s.positions = c->positions.as<float>();
+  s.normals = c->normals.as<float>();
+  s.texcoords = c->texcoords.as<float>();
+  s.colors = c->colors.as<float>();
+  s.material_ids = c->material_ids.as<uint32_t>();
+  s.triangle_count = d->triangle_count;
+  c->stats.bvh_nodes = bvh.nodes.size();
+  c->stats.bvh_bytes = bvh.nodes.size() * sizeof(Node4) + bvh.tris.size() * sizeof(TriRec);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->have_accel = true;
+  return PHX_OK;
+}
+
+int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
+  if (!c || !d) return PHX_ERR_INVALID;
+  int r = phx_build_accel(c, d);
+  if (r) return r;
+  if (d->material_count == 0 || !d->materials) return fail(c, PHX_ERR_INVALID, "set_scene: no materials");
+  const uint32_t nt = d->triangle_count;
+  std::vector<phx_material> mats(d->materials, d->materials + d->material_count);
+  auto mat_of = [&](uint32_t prim) -> uint32_t {
+    const uint32_t m = d->material_ids ? d->material_ids[prim] : 0u;
+    return m < mats.size() ? m : 0u;
+  };
+
+  // Emissive triangles (RenderBridge.cpp:780-813) unless given.
+  std::vector<uint32_t> em_ids;
+  std::vector<float> em_area;
+  if (d->emissive_prim_ids && d->emissive_prim_areas) {
+    em_ids.assign(d->emissive_prim_ids, d->emissive_prim_ids + d->emissive_count);
+    em_area.assign(d->emissive_prim_areas, d->emissive_prim_areas + d->emissive_count);
+    for (uint32_t id : em_ids)
+      if (id >= nt) return fail(c, PHX_ERR_INVALID, "set_scene: emissive primitive id out of range");
+  } else {
+    for (uint32_t t = 0; t < nt; ++t) {
+      if (!may_emit(mats[mat_of(t)])) continue;
+      const float *p0 = d->positions + size_t(d->indices[3 * t]) * 3;
+      const float *p1 = d->positions + size_t(d->indices[3 * t + 1]) * 3;
+      const float *p2 = d->positions + size_t(d->indices[3 * t + 2]) * 3;
+      const float e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+      const float e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+      const float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      const float area = 0.5f * length3(cr);
+      if (!(area > 0.f)) continue;
+      em_ids.push_back(t);
+      em_area.push_back(area);
+    }
+  }
+  const bool has_em = !em_ids.empty();
+
+  // Light tables (pathtracer.cpp:411-503).
+  std::vector<phx_light> nee;
+  for (uint32_t i = 0; i < d->light_count; ++i) {
+    const phx_light &l = d->lights[i];
+    if (has_em && l.type == PHX_LIGHT_AREA && l.mesh_prim_count > 0) continue;
+    nee.push_back(l);
+  }
+  std::vector<float> cdf, prim_pdf;
+  uint32_t em_count = 0;
+  if (has_em) {
+    const uint32_t n = uint32_t(em_ids.size());
+    std::vector<double> wts(n, 0.0);
+    double total = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+      const phx_material &m = mats[mat_of(em_ids[i])];
+      const float e[3] = {m.emission[0] * m.emission_strength, m.emission[1] * m.emission_strength,
+                          m.emission[2] * m.emission_strength};
+      const float power = 0.2126f * e[0] + 0.7152f * e[1] + 0.0722f * e[2];
+      wts[i] = double(em_area[i]) * double(power > 1e-6f ? power : 1e-6f);
+      if (!(em_area[i] > 0.f)) wts[i] = 0.0;
+      total += wts[i];
+    }
+    if (total > 0.0) {
+      em_count = n;
+      cdf.assign(n, 0.f);
+      prim_pdf.assign(nt, 0.f);
+      double run = 0.0;
+      for (uint32_t i = 0; i < n; ++i) {
+        run += wts[i];
+        cdf[i] = float(run / total);
+        if (wts[i] > 0.0) prim_pdf[em_ids[i]] = float(wts[i] / total / double(em_area[i]));
+      }
+      cdf[n - 1] = 1.f;
+    }
+  }
+
+  if ((r = upload(c, c->materials, mats.data(), mats.size()))) return r;
+  if ((r = upload(c, c->lights, nee.data(), nee.size()))) return r;
+  if ((r = upload(c, c->em_ids, em_ids.data(), em_count ? em_ids.size() : 0))) return r;
+  if ((r = upload(c, c->em_cdf, cdf.data(), cdf.size()))) return r;
+  if ((r = upload(c, c->prim_light_pdf, prim_pdf.data(), prim_pdf.size()))) return r;
+
+  DevScene &s = c->dev;
+  s.materials = c->materials.as<phx_material>();
+  s.material_count = uint32_t(mats.size());
+  s.lights = c->lights.as<phx_light>();
+  s.light_count = uint32_t(nee.size());
+  s.em_ids = c->em_ids.as<uint32_t>();
+  s.em_cdf = c->em_cdf.as<float>();
+  s.prim_light_pdf = c->prim_light_pdf.as<float>();
+  s.em_count = em_count;
+
+  // Environment map (RenderBridge.cpp:594-620): non-finite and negative
+  // texels become 0, then the CDFs.
+  s.has_env = 0;
+  s.env_visible = d->env_visible ? 1 : 0;
+  std::vector<float> envpx, marg, cond;
+  if (d->env_width > 0 && d->env_height > 0 && d->env_pixels) {
+    const size_t n = size_t(d->env_width) * d->env_height * 3;
+    envpx.assign(d->env_pixels, d->env_pixels + n);
+    for (float &v : envpx) v = (std::isfinite(v) && v > 0.f) ? v : 0.f;
+    env_cdf(envpx.data(), d->env_width, d->env_height, marg, cond);
+    if ((r = upload(c, c->env_pixels, envpx.data(), envpx.size()))) return r;
+    if ((r = upload(c, c->env_marginal, marg.data(), marg.size()))) return r;
+    if ((r = upload(c, c->env_cond, cond.data(), cond.size()))) return r;
+    s.has_env = 1;
+    s.env_w = d->env_width;
+    s.env_h = d->env_height;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        s.w2t[i * 3 + j] = d->env_world_to_tex[i * 3 + j];
+        s.t2w[j * 3 + i] = d->env_world_to_tex[i * 3 + j];
+      }
+  }
+  s.env_pixels = c->env_pixels.as<float>();
+  s.env_marginal = c->env_marginal.as<float>();
+  s.env_cond = c->env_cond.as<float>();
+
+  // Texture atlas.
+  s.tex_count = 0;
+  if (d->texture_count && d->texture_infos && d->texture_pixels) {
+    for (uint32_t i = 0; i < d->texture_count; ++i) {
+      const phx_texture_info &ti = d->texture_infos[i];
+      if (size_t(ti.offset) + size_t(ti.width) * ti.height > d->texture_pixel_count)
+        return fail(c, PHX_ERR_INVALID, "set_scene: texture outside the atlas");
+    }
+    if ((r = upload(c, c->tex_pixels, d->texture_pixels, size_t(d->texture_pixel_count) * 3))) return r;
+    if ((r = upload(c, c->tex_infos, d->texture_infos, d->texture_count))) return r;
+    s.tex_count = d->texture_count;
+  }
+  s.tex_pixels = c->tex_pixels.as<float>();
+  s.tex_infos = c->tex_infos.as<phx_texture_info>();
+
+  // Volumes.
+  s.vol_count = 0;
+  if (d->volume_count && d->volumes) {
+    for (uint32_t i = 0; i < d->volume_count; ++i) {
+      const phx_volume &v = d->volumes[i];
+      const size_t cells = size_t(v.dims[0]) * v.dims[1] * v.dims[2];
+      if (v.data_offset + cells > d->volume_value_count || v.tf_offset + v.tf_size > d->volume_tf_count)
+        return fail(c, PHX_ERR_INVALID, "set_scene: volume data out of range");
+      const size_t mcells = size_t(v.macro_dims[0]) * v.macro_dims[1] * v.macro_dims[2];
+      if (v.macro_offset + mcells > d->volume_majorant_count)
+        return fail(c, PHX_ERR_INVALID, "set_scene: volume majorants out of range");
+    }
+    if ((r = upload(c, c->volumes, d->volumes, d->volume_count))) return r;
+    if ((r = upload(c, c->vol_values, d->volume_values, d->volume_value_count))) return r;
+    if ((r = upload(c, c->vol_tf, reinterpret_cast<const float4 *>(d->volume_tf), d->volume_tf_count))) return r;
+    if ((r = upload(c, c->majorants, d->volume_majorants, d->volume_majorant_count))) return r;
+    s.vol_count = d->volume_count;
+  }
+  s.volumes = c->volumes.as<phx_volume>();
+  s.vol_values = c->vol_values.as<float>();
+  s.vol_tf = c->vol_tf.as<float4>();
+  s.majorants = c->majorants.as<float>();
+
+  s.strategy_count = s.light_count + (em_count > 0 ? 1u : 0u) + (s.has_env ? 1u : 0u);
+  // Ray offset (pathtracer.cpp:489-502).
+  float eps = 1e-3f;
+  if (d->ray_epsilon > 0.f) {
+    eps = d->ray_epsilon;
+  } else if (c->bounds_hi[0] >= c->bounds_lo[0]) {
+    const float dg[3] = {c->bounds_hi[0] - c->bounds_lo[0], c->bounds_hi[1] - c->bounds_lo[1],
+                         c->bounds_hi[2] - c->bounds_lo[2]};
+    const float e = 1e-4f * length3(dg);
+    eps = e < 1e-6f ? 1e-6f : (e > 1e-2f ? 1e-2f : e);
+  }
+  s.ray_eps = eps;
+  c->stats.scene_bytes = c->scene_bytes;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->have_scene = true;
+  return PHX_OK;
+}
+
+int phx_trace_closest_compact(phx_context *c, const float *o, const float *d, const float *tmin, const float *tmax,
+                              uint32_t n, phx_hit_compact *out, void *stream) {
+  if (!c || !ray_args_ok(o, d, tmin, tmax) || (n && !out)) return PHX_ERR_INVALID;
+  if (!c->have_accel) return fail(c, PHX_ERR_NO_SCENE, "trace before build_accel");
+  if (n == 0) return PHX_OK;
+  int r = ensure_trace_scratch(c);
+  if (r) return r;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
+  HIP_TRY(c, launch_trace_compact(c->dev, o, d, tmin, tmax, n, out, c->fetch.as<uint32_t>(), c->spill.as<int>(),
+                                  c->trace_grid, st));
+  return PHX_OK;
+}
+int phx_trace_closest(phx_context *c, const float *o, const float *d, const float *tmin, const float *tmax,
+                      uint32_t n, phx_hit *out, void *stream) {
+  if (!c || !ray_args_ok(o, d, tmin, tmax) || (n && !out)) return PHX_ERR_INVALID;
+  if (!c->have_accel) return fail(c, PHX_ERR_NO_SCENE, "trace before build_accel");
+  if (n == 0) return PHX_OK;
+  int r = ensure_trace_scratch(c);
+  if (r) return r;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
+  HIP_TRY(c, launch_trace_full(c->dev, o, d, tmin, tmax, n, out, c->fetch.as<uint32_t>(), c->spill.as<int>(),
+                               c->trace_grid, st));
+  return PHX_OK;
+}
+int phx_trace_occluded(phx_context *c, const float *o, const float *d, const float *tmin, const float *tmax,
+                       uint32_t n, uint32_t *out, void *stream) {
+  if (!c || !ray_args_ok(o, d, tmin, tmax) || (n && !out)) return PHX_ERR_INVALID;
+  if (!c->have_accel) return fail(c, PHX_ERR_NO_SCENE, "trace before build_accel");
+  if (n == 0) return PHX_OK;
+  int r = ensure_trace_scratch(c);
+  if (r) return r;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
+  HIP_TRY(c, launch_trace_occluded(c->dev, o, d, tmin, tmax, n, out, c->fetch.as<uint32_t>(), c->spill.as<int>(),
+                                   c->trace_grid, st));
+  return PHX_OK;
+}
+
+int phx_camera_perspective(phx_camera *out, const float f[3], const float at[3], const float up[3], float vfov,
+                           float aspect, float aperture, float focus) {
+  if (!out || !f || !at || !up) return PHX_ERR_INVALID;
+  // Camera::make_perspective (camera.h:32-55).
+  const float theta = vfov * 3.1415926535f / 180.f;
+  const float h = std::tan(theta * 0.5f);
+  const float vh = 2.f * h, vw = aspect * vh;
+  float w[3] = {f[0] - at[0], f[1] - at[1], f[2] - at[2]};
+  auto norm = [](float *v) {
+    const float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const float inv = l > 0.f ? 1.f / l : 0.f;
+    if (l > 0.f) {
+      v[0] *= inv;
+      v[1] *= inv;
+      v[2] *= inv;
+    } else {
+      v[0] = v[1] = v[2] = 0.f;
+    }
+  };
+  auto crs = [](const float *a, const float *b, float *o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  norm(w);
+  float u[3], v[3];
+  crs(up, w, u);
+  norm(u);
+  crs(w, u, v);
+  phx_camera c{};
+  for (int i = 0; i < 3; ++i) {
+    c.origin[i] = f[i];
+    c.u[i] = u[i];
+    c.v[i] = v[i];
+    c.w[i] = w[i];
+    c.horizontal[i] = u[i] * (vw * focus);
+    c.vertical[i] = v[i] * (vh * focus);
+  }
+  for (int i = 0; i < 3; ++i)
+    c.lower_left[i] = ((f[i] - c.horizontal[i] * 0.5f) - c.vertical[i] * 0.5f) - w[i] * focus;
+  c.lens_radius = aperture * 0.5f;
+  c.focus_dist = focus;
+  c.is_ortho = 0;
+  c.clip_near = 0.f;
+  c.clip_far = 1e30f;
+  *out = c;
+  return PHX_OK;
+}
+
+int phx_camera_orthographic(phx_camera *out, const float f[3], const float at[3], const float up[3], float height,
+                            float aspect) {
+  if (!out || !f || !at || !up) return PHX_ERR_INVALID;
+  // Camera::make_orthographic (camera.h:57-77).
+  phx_camera c{};
+  int r = phx_camera_perspective(&c, f, at, up, 90.f, aspect, 0.f, 1.f);
+  if (r) return r;
+  const float vh = height, vw = height * aspect;
+  for (int i = 0; i < 3; ++i) {
+    c.horizontal[i] = c.u[i] * vw;
+    c.vertical[i] = c.v[i] * vh;
+    c.lower_left[i] = (f[i] - c.horizontal[i] * 0.5f) - c.vertical[i] * 0.5f;
+  }
+  c.focus_dist = 1.f;
+  c.lens_radius = 0.f;
+  c.is_ortho = 1;
+  *out = c;
+  return PHX_OK;
+}
+
+int phx_render(phx_context *c, const phx_camera *cam, const phx_render_params *p) {
+  if (!c || !cam || !p) return PHX_ERR_INVALID;
+  if (!c->have_scene) return fail(c, PHX_ERR_NO_SCENE, "render before set_scene");
+  if (p->width == 0 || p->height == 0) return fail(c, PHX_ERR_INVALID, "render: empty frame");
+  if (p->adaptive_sampling) return fail(c, PHX_ERR_UNSUPPORTED, "render: adaptive sampling is not offered");
+  const uint64_t npix64 = uint64_t(p->width) * p->height;
+  if (npix64 > (1u << 26)) return fail(c, PHX_ERR_INVALID, "render: frame larger than 2^26 pixels");
+  const uint32_t npix = uint32_t(npix64);
+  const uint32_t spp = std::max<uint32_t>(p->samples_per_pixel, 1);
+  const uint32_t max_depth = p->max_depth > 0 ? p->max_depth : 1u;
+  const uint32_t max_it = max_depth + 1u + 8u;  // pathtracer.cpp:682
+  if (max_it > kMaxIterations) return fail(c, PHX_ERR_INVALID, "render: max_depth too large");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+
+  // Samples per pass: enough path slots to fill the GPU several times over.
+  uint32_t target = c->tuning.paths_in_flight ? c->tuning.paths_in_flight : (1u << 22);
+  uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(spp, target / npix));
+  const uint32_t cap = npix * k;
+  if (cap > c->q_capacity) {
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(c, c->q_o[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, c->q_d[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, c->q_tp[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, c->q_st[b].reserve(size_t(cap) * 16));
+    }
+    HIP_TRY(c, c->q_hit.reserve(size_t(cap) * 16));
+    HIP_TRY(c, c->sh_o.reserve(size_t(cap) * 16));
+    HIP_TRY(c, c->sh_d.reserve(size_t(cap) * 16));
+    HIP_TRY(c, c->sh_c.reserve(size_t(cap) * 16));
+    HIP_TRY(c, c->slot_L.reserve(size_t(cap) * 16));
+    HIP_TRY(c, c->slot_aov0.reserve(size_t(cap) * 16));
+    HIP_TRY(c, c->slot_aov1.reserve(size_t(cap) * 16));
+    c->q_capacity = cap;
+  }
+  HIP_TRY(c, c->counters.reserve(kCounterWords * 4));
+  HIP_TRY(c, c->stats_buf.reserve(kStatWords * 8));
+  int r = ensure_trace_scratch(c);
+  if (r) return r;
+  if (c->f_w != p->width || c->f_h != p->height || !c->f_accum.p) {
+    HIP_TRY(c, c->f_accum.reserve(size_t(npix) * 16));
+    HIP_TRY(c, c->f_aov0.reserve(size_t(npix) * 16));
+    HIP_TRY(c, c->f_aov1.reserve(size_t(npix) * 16));
+    HIP_TRY(c, c->f_color.reserve(size_t(npix) * 12));
+    HIP_TRY(c, c->f_alpha.reserve(size_t(npix) * 4));
+    HIP_TRY(c, c->f_depth.reserve(size_t(npix) * 4));
+    HIP_TRY(c, c->f_normal.reserve(size_t(npix) * 12));
+    HIP_TRY(c, c->f_albedo.reserve(size_t(npix) * 12));
+    c->f_w = p->width;
+    c->f_h = p->height;
+  }
+  WaveQueues q{};
+  for (int b = 0; b < 2; ++b) {
+    q.o[b] = c->q_o[b].as<float4>();
+    q.d[b] = c->q_d[b].as<float4>();
+    q.tp[b] = c->q_tp[b].as<float4>();
+    q.st[b] = c->q_st[b].as<uint4>();
+  }
+  q.hit = c->q_hit.as<float4>();
+  q.sh_o = c->sh_o.as<float4>();
+  q.sh_d = c->sh_d.as<float4>();
+  q.sh_c = c->sh_c.as<float4>();
+  q.slot_L = c->slot_L.as<float4>();
+  q.slot_aov0 = c->slot_aov0.as<float4>();
+  q.slot_aov1 = c->slot_aov1.as<float4>();
+  q.counters = c->counters.as<uint32_t>();
+  q.stats = c->stats_buf.as<unsigned long long>();
+  q.spill = c->spill.as<int>();
+  FrameBuffers fb{c->f_accum.as<float4>(), c->f_aov0.as<float4>(), c->f_aov1.as<float4>(), c->f_color.as<float>(),
+                  c->f_alpha.as<float>(),  c->f_depth.as<float>(), c->f_normal.as<float>(), c->f_albedo.as<float>()};
+
+  WaveParams w{};
+  w.cam = *cam;
+  w.width = p->width;
+  w.height = p->height;
+  w.npix = npix;
+  w.max_depth = max_depth;
+  w.bottom_up = p->bottom_up;
+  for (int i = 0; i < 4; ++i) w.region[i] = p->image_region[i];
+  w.bg = vec3{p->background_color[0], p->background_color[1], p->background_color[2]};
+  w.bg_alpha = p->background_alpha;
+  w.ambient = vec3{p->ambient_color[0] * p->ambient_radiance, p->ambient_color[1] * p->ambient_radiance,
+                   p->ambient_color[2] * p->ambient_radiance};
+  w.throughput_clamp = p->throughput_clamp;
+  w.clamp_direct = p->clamp_direct_sum;
+  w.clamp_indirect = p->clamp_indirect_sum;
+  w.max_diffuse = p->max_diffuse_bounces;
+  w.max_glossy = p->max_glossy_bounces;
+  w.max_transmission = p->max_transmission_bounces;
+  w.max_volume = p->max_volume_bounces;
+
+  hipStream_t st = c->stream;
+  const bool timed = c->tuning.time_kernels != 0;
+  const uint32_t passes = (spp + k - 1) / k;
+  const size_t n_ev = timed ? size_t(passes) * max_it * 3 * 2 : 0;
+  while (c->events.size() < n_ev) {
+    hipEvent_t e;
+    HIP_TRY(c, hipEventCreate(&e));
+    c->events.push_back(e);
+  }
+  size_t ev = 0;
+  auto mark = [&]() -> hipError_t { return timed ? hipEventRecord(c->events[ev++], st) : hipSuccess; };
+  HIP_TRY(c, launch_clear_u32(reinterpret_cast<uint32_t *>(q.stats), kStatWords * 2, st));
+  for (uint32_t s0 = 0; s0 < spp; s0 += k) {
+    w.k = std::min(k, spp - s0);
+    w.nslots = npix * w.k;
+    w.sample_base = s0 + p->sample_offset;
+    HIP_TRY(c, launch_reset(q, w.nslots, st));
+    HIP_TRY(c, launch_generate(w, q, st));
+    for (uint32_t it = 0; it < max_it; ++it) {
+      HIP_TRY(c, mark());
+      HIP_TRY(c, launch_extend(c->dev, q, it, c->trace_grid, st));
+      HIP_TRY(c, mark());
+      HIP_TRY(c, mark());
+      HIP_TRY(c, launch_shade(c->dev, w, q, it, st));
+      HIP_TRY(c, mark());
+      HIP_TRY(c, mark());
+      HIP_TRY(c, launch_shadow(c->dev, w, q, it, c->trace_grid, st));
+      HIP_TRY(c, mark());
+    }
+    HIP_TRY(c, launch_accumulate(w, q, fb, s0 == 0, st));
+  }
+  HIP_TRY(c, launch_resolve(npix, spp, fb, st));
+  unsigned long long hs[kStatWords];
+  HIP_TRY(c, hipMemcpyAsync(hs, q.stats, sizeof hs, hipMemcpyDeviceToHost, st));
+  HIP_TRY(c, hipStreamSynchronize(st));
+  const auto t1 = std::chrono::steady_clock::now();
+  c->stats.extend_launches = c->stats.shade_launches = c->stats.shadow_launches = 0;
+  c->stats.extend_ms = c->stats.shade_ms = c->stats.shadow_ms = 0.0;
+  for (size_t e = 0; e + 1 < ev; e += 6) {
+    float a = 0.f, b = 0.f, d = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&a, c->events[e], c->events[e + 1]));
+    HIP_TRY(c, hipEventElapsedTime(&b, c->events[e + 2], c->events[e + 3]));
+    HIP_TRY(c, hipEventElapsedTime(&d, c->events[e + 4], c->events[e + 5]));
+    c->stats.extend_ms += a;
+    c->stats.shade_ms += b;
+    c->stats.shadow_ms += d;
+    c->stats.extend_launches++;
+    c->stats.shade_launches++;
+    c->stats.shadow_launches++;
+  }
+  c->stats.closest_rays = hs[kStatClosest];
+  c->stats.shadow_rays = hs[kStatShadow];
+  c->stats.shade_events = hs[kStatShade];
+  c->stats.samples = uint64_t(npix) * spp;
+  c->stats.pixels = npix;
+  c->stats.last_render_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  c->have_result = true;
+  return PHX_OK;
+}
+
+int phx_result_device_color(phx_context *c, const float **d_color) {
+  if (!c || !d_color) return PHX_ERR_INVALID;
+  if (!c->have_result) return fail(c, PHX_ERR_NO_FRAME, "no frame has been rendered");
+  *d_color = c->f_color.as<float>();
+  return PHX_OK;
+}
+
+int phx_read_result(phx_context *c, float *color, float *alpha, float *depth, float *normal, float *albedo) {
+  if (!c) return PHX_ERR_INVALID;
+  if (!c->have_result) return fail(c, PHX_ERR_NO_FRAME, "no frame has been rendered");
+  const size_t n = size_t(c->f_w) * c->f_h;
+  hipStream_t st = c->stream;
+  if (color) HIP_TRY(c, hipMemcpyAsync(color, c->f_color.p, n * 12, hipMemcpyDeviceToHost, st));
+  if (alpha) HIP_TRY(c, hipMemcpyAsync(alpha, c->f_alpha.p, n * 4, hipMemcpyDeviceToHost, st));
+  if (depth) HIP_TRY(c, hipMemcpyAsync(depth, c->f_depth.p, n * 4, hipMemcpyDeviceToHost, st));
+  if (normal) HIP_TRY(c, hipMemcpyAsync(normal, c->f_normal.p, n * 12, hipMemcpyDeviceToHost, st));
+  if (albedo) HIP_TRY(c, hipMemcpyAsync(albedo, c->f_albedo.p, n * 12, hipMemcpyDeviceToHost, st));
+  HIP_TRY(c, hipStreamSynchronize(st));
+  return PHX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// Internal: the context object behind the C ABI and its error helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/phx.h"
+#include "integrator.h"
+
+namespace phx_detail {
+
+using namespace phx;
+
+// Device allocation owned by the context.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t reserve(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    bytes = n;
+    return hipSuccess;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace phx_detail
+
+struct phx_context {
+  using DBuf = phx_detail::DBuf;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 256;
+  std::string error;
+  phx_tuning tuning{};
+  phx_stats stats{};
+
+  // geometry (seam A)
+  bool have_accel = false;
+  DBuf nodes, tris, indices, positions, normals, texcoords, colors, material_ids;
+  uint32_t triangle_count = 0, vertex_count = 0;
+  float bounds_lo[3]{}, bounds_hi[3]{};
+  uint32_t bvh_depth = 0;
+  size_t scene_bytes = 0;
+
+  // shading data
+  bool have_scene = false;
+  DBuf materials, lights, em_ids, em_cdf, prim_light_pdf, env_pixels, env_marginal, env_cond, tex_pixels, tex_infos,
+      volumes, vol_values, vol_tf, majorants;
+  phx::DevScene dev{};
+
+  // wavefront storage
+  DBuf q_o[2], q_d[2], q_tp[2], q_st[2], q_hit, sh_o, sh_d, sh_c, slot_L, slot_aov0, slot_aov1, counters, stats_buf,
+      spill, fetch;
+  uint32_t q_capacity = 0;
+  uint32_t trace_grid = 0;
+  std::vector<hipEvent_t> events;  // kernel timing (phx_tuning.time_kernels)
+
+  // resolved frame (RenderResult) and its accumulators
+  DBuf f_accum, f_aov0, f_aov1, f_color, f_alpha, f_depth, f_normal, f_albedo;
+  uint32_t f_w = 0, f_h = 0;
+  bool have_result = false;
+
+  // render bridge (seam B)
+  DBuf b_color, b_alpha, b_depth, b_normal, b_albedo;
+  uint32_t b_w = 0, b_h = 0, b_samples = 0, b_counter = 0;
+};
+
+namespace phx_detail {
+
+inline int fail(phx_context *c, int code, const std::string &m) {
+  if (c) c->error = m;
+  return code;
+}
+inline int hip_fail(phx_context *c, hipError_t e, const char *what) {
+  return fail(c, PHX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIP_TRY(c, expr)                                  \
+  do {                                                    \
+    hipError_t e_ = (expr);                               \
+    if (e_ != hipSuccess) return hip_fail((c), e_, #expr); \
+  } while (0)
+
+template <class T>
+inline int upload(phx_context *c, phx_detail::DBuf &b, const T *src, size_t count) {
+  if (!src || count == 0) {
+    b.release();
+    return PHX_OK;
+  }
+  HIP_TRY(c, b.reserve(sizeof(T) * count));
+  HIP_TRY(c, hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+  c->scene_bytes += sizeof(T) * count;
+  return PHX_OK;
+}
+
+}  // namespace phx_detail
