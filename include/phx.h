@@ -349,7 +349,10 @@ int phx_camera_orthographic(phx_camera *out, const float look_from[3], const flo
 
 /* Macro-cell majorants of a volume (RenderBridge.cpp:656-778): fills
  * vol->macro_dims and vol->majorant and writes the cells to `out`
- * (capacity in floats; the needed count is returned in *out_count). */
+ * (capacity in floats; the needed count is returned in *out_count).
+ * vol->macro_dims all 0: the reference's choice of cell counts; all > 0: that
+ * coarse grid (e.g. 32^3 over a 256^3 volume), bounded like the reference's
+ * coarse cells.  out = NULL only computes the count. */
 int phx_volume_majorants(phx_volume *vol, const float *values, const float *tf_rgba,
                          uint32_t tf_size, float *out, uint32_t capacity, uint32_t *out_count);
 

@@ -84,3 +84,20 @@ def check(ctx, code):
 
 def fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def volume_majorants(vol, values, tf):
+    """phx_volume_majorants: fills vol.macro_dims / vol.majorant, returns the cells."""
+    import numpy as np
+    values = np.ascontiguousarray(values, np.float32)
+    tf = np.ascontiguousarray(tf, np.float32)
+    n = C.c_uint32()
+    L = lib()
+    code = L.phx_volume_majorants(C.byref(vol), fptr(values), fptr(tf), tf.size // 4, None, 0, C.byref(n))
+    if code != 0:
+        raise PhxError(code, "phx_volume_majorants")
+    out = np.zeros(n.value, np.float32)
+    code = L.phx_volume_majorants(C.byref(vol), fptr(values), fptr(tf), tf.size // 4, fptr(out), n.value, C.byref(n))
+    if code != 0:
+        raise PhxError(code, "phx_volume_majorants")
+    return out

@@ -42,9 +42,12 @@ float len3(const float *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] *
 extern "C" {
 
 // RenderBridge.cpp:656-778.
-int phx_volume_majorants(phx_volume *v, const float *values, const float *tf, uint32_t tf_size, float *out,
+int phx_volume_majorants(phx_volume *vol, const float *values, const float *tf, uint32_t tf_size, float *out,
                          uint32_t capacity, uint32_t *out_count) {
-  if (!v || !out_count) return PHX_ERR_INVALID;
+  if (!vol || !out_count) return PHX_ERR_INVALID;
+  // A count-only call leaves the caller's volume untouched.
+  phx_volume local = *vol;
+  phx_volume *v = out ? vol : &local;
   float max_op = 0.f;
   for (uint32_t t = 0; t < tf_size; ++t) max_op = std::max(max_op, tf[4 * t + 3]);
   for (int d = 0; d < 3; ++d) v->dims[d] = std::max<uint32_t>(v->dims[d], 1);
@@ -53,17 +56,24 @@ int phx_volume_majorants(phx_volume *v, const float *values, const float *tf, ui
   const bool complete = values != nullptr;
   if (!complete) v->majorant = 0.f;
   uint32_t sub[3] = {0, 0, 0};
-  for (int d = 0; d < 3; ++d) {
-    const uint32_t cells = v->dims[d] > 1 ? v->dims[d] - 1 : 1;
-    if (cells <= 32) {
-      sub[d] = std::min<uint32_t>(std::max<uint32_t>((32 + cells - 1) / cells, 1), 16);
-      v->macro_dims[d] = cells * sub[d];
-    } else {
-      v->macro_dims[d] = std::min<uint32_t>(std::max<uint32_t>((v->dims[d] + 2) / 4, 1), 128);
+  const bool preset = v->macro_dims[0] > 0 && v->macro_dims[1] > 0 && v->macro_dims[2] > 0;
+  if (preset) {
+    // Caller-chosen coarse cells (each covering several grid cells); the
+    // bounds use the coarse-cell rule below.
+    for (int d = 0; d < 3; ++d) v->macro_dims[d] = std::min<uint32_t>(v->macro_dims[d], 1024);
+  } else {
+    for (int d = 0; d < 3; ++d) {
+      const uint32_t cells = v->dims[d] > 1 ? v->dims[d] - 1 : 1;
+      if (cells <= 32) {
+        sub[d] = std::min<uint32_t>(std::max<uint32_t>((32 + cells - 1) / cells, 1), 16);
+        v->macro_dims[d] = cells * sub[d];
+      } else {
+        v->macro_dims[d] = std::min<uint32_t>(std::max<uint32_t>((v->dims[d] + 2) / 4, 1), 128);
+      }
     }
   }
   const bool refined = sub[0] > 0 && sub[1] > 0 && sub[2] > 0;
-  if (!refined)
+  if (!refined && !preset)
     for (int d = 0; d < 3; ++d)
       if (sub[d] > 0) v->macro_dims[d] = std::min<uint32_t>(std::max<uint32_t>((v->dims[d] + 2) / 4, 1), 128);
   const size_t n = size_t(v->macro_dims[0]) * v->macro_dims[1] * v->macro_dims[2];

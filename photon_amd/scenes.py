@@ -405,3 +405,142 @@ def random_rays(scene, n, seed=0, inside_fraction=0.5):
     tmin = np.zeros(n, np.float32)
     tmax = np.full(n, 1e30, np.float32)
     return org.astype(np.float32), d.astype(np.float32), tmin, tmax
+
+
+# --------------------------------------------------------------------------- fast lattice fBm
+def _lattice_noise_grid(n, freq, seed):
+    """Value noise sampled on an n^3 grid over [0, 1)^3 at `freq` lattice cells
+    per unit, with smoothstep-faded trilinear weights (separable)."""
+    rng = np.random.default_rng(seed)
+    m = int(math.ceil(freq)) + 2
+    lat = rng.uniform(-1.0, 1.0, (m, m, m))
+    x = (np.arange(n) + 0.5) / n * freq
+    i = np.floor(x).astype(np.int64)
+    f = x - i
+    w = f * f * (3.0 - 2.0 * f)
+    # interpolate axis by axis: lat[z, y, x]
+    a = lat[:, :, i] * (1 - w) + lat[:, :, i + 1] * w            # (m, m, n)
+    a = a[:, i, :] * (1 - w)[None, :, None] + a[:, i + 1, :] * w[None, :, None]  # (m, n, n)
+    a = a[i, :, :] * (1 - w)[:, None, None] + a[i + 1, :, :] * w[:, None, None]  # (n, n, n)
+    return a
+
+
+def fbm_grid(n, octaves, seed, base_freq=4.0):
+    total = np.zeros((n, n, n))
+    amp, norm = 1.0, 0.0
+    for o in range(octaves):
+        total += amp * _lattice_noise_grid(n, base_freq * 2 ** o, seed + 101 * o)
+        norm += amp
+        amp *= 0.5
+    return (total / norm).astype(np.float32)  # in about [-1, 1], z-major like the volume values
+
+
+# --------------------------------------------------------------------------- gradient workload
+def textured_cornell(n_materials=32, tex_size=512, texture=None, seed=0):
+    """Cornell box whose floor is textured (512^2 RGB albedo) and whose other
+    32 triangles cycle through the remaining materials; material 0 is the
+    textured floor, materials 1..n-1 Disney diffuse/metal, the light last."""
+    base = cornell_box()
+    rng = np.random.default_rng(seed)
+    mats = []
+    floor = disney((0.5, 0.5, 0.5), 0.8)
+    floor.base_color_tex = 0
+    mats.append(floor)
+    for k in range(1, n_materials):
+        mats.append(disney(tuple(rng.uniform(0.2, 0.8, 3)), float(rng.uniform(0.3, 1.0)), float(rng.uniform(0, 0.3))))
+    light = emitter((17.0, 12.0, 4.0))
+    mats.append(light)
+    mid = np.zeros(base.triangle_count, np.uint32)
+    light_tris = np.nonzero(base.material_ids == 3)[0]
+    floor_tris = np.array([0, 1])
+    others = [t for t in range(base.triangle_count) if t not in set(light_tris) | set(floor_tris)]
+    for k, t in enumerate(others):
+        mid[t] = 1 + (k % (n_materials - 1))
+    mid[light_tris] = n_materials
+    uv = np.zeros((base.positions.shape[0], 2), np.float32)
+    # floor quad vertices 0..3 span x, z in [-1, 1]
+    fv = base.positions[:4]
+    uv[:4, 0] = (fv[:, 0] + 1) / 2
+    uv[:4, 1] = (fv[:, 2] + 1) / 2
+    s = Scene(base.positions, base.indices, mats, material_ids=mid, texcoords=uv)
+    tex = np.full((tex_size, tex_size, 3), 0.5, np.float32) if texture is None else texture
+    s.add_texture(tex)
+    s.camera = base.camera
+    s.name = "cornell_textured"
+    return s
+
+
+def _lattice_noise_2d(n, freq, seed):
+    rng = np.random.default_rng(seed)
+    m = int(math.ceil(freq)) + 2
+    lat = rng.uniform(-1.0, 1.0, (m, m))
+    x = (np.arange(n) + 0.5) / n * freq
+    i = np.floor(x).astype(np.int64)
+    f = x - i
+    w = f * f * (3.0 - 2.0 * f)
+    a = lat[:, i] * (1 - w) + lat[:, i + 1] * w
+    return a[i, :] * (1 - w)[:, None] + a[i + 1, :] * w[:, None]
+
+
+def colored_noise_texture(size, seed, octaves=3):
+    """Seeded 3-octave colored value noise in (0, 1), (size, size, 3)."""
+    out = np.zeros((size, size, 3), np.float32)
+    for c in range(3):
+        g, amp, norm = 0.0, 1.0, 0.0
+        for o in range(octaves):
+            g = g + amp * _lattice_noise_2d(size, 6.0 * 2 ** o, seed + 17 * c + 101 * o)
+            norm += amp
+            amp *= 0.5
+        out[..., c] = 0.5 + 0.45 * g / norm
+    return np.clip(out, 0.02, 0.98)
+
+
+# --------------------------------------------------------------------------- volume workload
+def fog_box(n=256, seed=7, threshold=0.4, scale=20.0, macro=32, albedo=0.9, env=True, env_size=(1024, 512),
+            majorants_fn=None):
+    """A 256^3 fBm density grid in an open box: floor, back and left walls,
+    one quad area light and the sky environment.  The transfer function maps
+    the normalized field to extinction (0 below `threshold`, ramping to 1),
+    density_scale = `scale`; the scattering albedo is `albedo`.
+    `majorants_fn(volume, values, tf)` builds the macro-cell grid (the
+    library's phx_volume_majorants is used when it is None)."""
+    b = _MeshBuilder()
+    mats = [disney((0.6, 0.6, 0.6), 1.0), emitter((30.0, 28.0, 25.0))]
+    b.add(*_quad((-2, -1, 2), (4, 0, 0), (0, 0, -4)), 0)      # floor
+    b.add(*_quad((-2, -1, -2), (4, 0, 0), (0, 3, 0)), 0)      # back wall
+    b.add(*_quad((-2, -1, 2), (0, 0, -4), (0, 3, 0)), 0)      # left wall
+    b.add(*_quad((0.2, 1.8, -0.6), (1.0, 0, 0), (0, 0, 1.0)), 1)  # light facing down
+    s = b.build(mats)
+    g = fbm_grid(n, 6, seed)
+    # normalized field: the densest 35% of the grid lies above `threshold`
+    q = np.quantile(g[::4, ::4, ::4], [0.65, 0.999])
+    v = np.clip(threshold + (g - q[0]) * (1.0 - threshold) / (q[1] - q[0]), 0.0, 1.0).astype(np.float32).reshape(-1)
+    vol = abi.Volume()
+    vol.world_to_local[:] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+    vol.origin[:] = (-1.0, -1.0, -1.0)
+    vol.spacing[:] = (2.0 / (n - 1),) * 3
+    vol.dims[:] = (n, n, n)
+    vol.data_offset = 0
+    vol.tf_offset = 0
+    vol.value_lo, vol.value_hi = 0.0, 1.0
+    vol.density_scale = scale
+    vol.linear = 1
+    vol.macro_dims[:] = (macro, macro, macro)
+    x = np.linspace(0, 1, 256)
+    tf = np.zeros((256, 4), np.float32)
+    tf[:, :3] = albedo
+    tf[:, 3] = np.clip((x - threshold) / (1 - threshold), 0, 1)
+    vol.tf_size = 256
+    if majorants_fn is None:
+        from ._lib import volume_majorants
+        majorants_fn = volume_majorants
+    maj = majorants_fn(vol, v, tf)
+    s.volumes = [vol]
+    s.volume_values = v
+    s.volume_tf = tf.reshape(-1)
+    s.volume_majorants = maj
+    if env:
+        s.env_pixels = sky_envmap(*env_size)
+    s.camera = dict(look_from=(0.0, 0.6, 4.2), look_at=(0.0, -0.1, 0.0), vup=(0.0, 1.0, 0.0), vfov=45.0, aspect=1.0)
+    s.name = f"fog{n}"
+    return s
