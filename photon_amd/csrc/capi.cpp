@@ -388,7 +388,7 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
 
 int phx_trace_closest_compact(phx_context *c, const float *o, const float *d, const float *tmin, const float *tmax,
                               uint32_t n, phx_hit_compact *out, void *stream) {
-  if (!c || !ray_args_ok(o, d, tmin, tmax) || (n && !out)) return PHX_ERR_INVALID;
+  if (!c || (n && (!ray_args_ok(o, d, tmin, tmax) || !out))) return PHX_ERR_INVALID;
   if (!c->have_accel) return fail(c, PHX_ERR_NO_SCENE, "trace before build_accel");
   if (n == 0) return PHX_OK;
   int r = ensure_trace_scratch(c);
@@ -401,7 +401,7 @@ int phx_trace_closest_compact(phx_context *c, const float *o, const float *d, co
 }
 int phx_trace_closest(phx_context *c, const float *o, const float *d, const float *tmin, const float *tmax,
                       uint32_t n, phx_hit *out, void *stream) {
-  if (!c || !ray_args_ok(o, d, tmin, tmax) || (n && !out)) return PHX_ERR_INVALID;
+  if (!c || (n && (!ray_args_ok(o, d, tmin, tmax) || !out))) return PHX_ERR_INVALID;
   if (!c->have_accel) return fail(c, PHX_ERR_NO_SCENE, "trace before build_accel");
   if (n == 0) return PHX_OK;
   int r = ensure_trace_scratch(c);
@@ -414,7 +414,7 @@ int phx_trace_closest(phx_context *c, const float *o, const float *d, const floa
 }
 int phx_trace_occluded(phx_context *c, const float *o, const float *d, const float *tmin, const float *tmax,
                        uint32_t n, uint32_t *out, void *stream) {
-  if (!c || !ray_args_ok(o, d, tmin, tmax) || (n && !out)) return PHX_ERR_INVALID;
+  if (!c || (n && (!ray_args_ok(o, d, tmin, tmax) || !out))) return PHX_ERR_INVALID;
   if (!c->have_accel) return fail(c, PHX_ERR_NO_SCENE, "trace before build_accel");
   if (n == 0) return PHX_OK;
   int r = ensure_trace_scratch(c);
