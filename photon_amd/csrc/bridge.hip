@@ -218,8 +218,8 @@ int phx_bridge_render(phx_context *c, const phx_camera_data *cd, const phx_rende
   const uint32_t spp = p.samples_per_pixel;
   c->b_counter += spp;
   const bool first = reset || c->b_samples == 0;
-  FrameBuffers fb{c->f_accum.as<float4>(), c->f_aov0.as<float4>(), c->f_aov1.as<float4>(), c->f_color.as<float>(),
-                  c->f_alpha.as<float>(),  c->f_depth.as<float>(), c->f_normal.as<float>(), c->f_albedo.as<float>()};
+  FrameBuffers fb{c->f_accum.as<float4>(), c->f_color.as<float>(),  c->f_alpha.as<float>(),
+                  c->f_depth.as<float>(),  c->f_normal.as<float>(), c->f_albedo.as<float>()};
   const uint32_t npix = w * h;
   hipLaunchKernelGGL(k_frame_accumulate, dim3((npix + 255) / 256), dim3(256), 0, c->stream, fb, c->b_color.as<float>(),
                      c->b_alpha.as<float>(), c->b_depth.as<float>(), c->b_normal.as<float>(), c->b_albedo.as<float>(),

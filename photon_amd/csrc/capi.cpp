@@ -70,13 +70,6 @@ void env_cdf(const float *px, uint32_t w, uint32_t h, std::vector<float> &marg, 
   }
 }
 
-int ensure_trace_scratch(phx_context *c) {
-  if (c->trace_grid == 0) c->trace_grid = uint32_t(c->cu_count) * 8u;
-  HIP_TRY(c, c->spill.reserve(size_t(c->trace_grid) * 256 * (kMaxStack - kShortStack) * sizeof(int)));
-  HIP_TRY(c, c->fetch.reserve(64));
-  return PHX_OK;
-}
-
 bool ray_args_ok(const float *a, const float *b, const float *c, const float *d) { return a && b && c && d; }
 
 }  // namespace
@@ -118,6 +111,7 @@ void phx_destroy(phx_context *c) {
   // DBuf members free their memory in the destructor.
   hipStream_t s = c->stream;
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  if (c->pinned) (void)hipHostFree(c->pinned);
   delete c;
   if (s) (void)hipStreamDestroy(s);
 }
@@ -492,157 +486,6 @@ int phx_camera_orthographic(phx_camera *out, const float f[3], const float at[3]
   c.lens_radius = 0.f;
   c.is_ortho = 1;
   *out = c;
-  return PHX_OK;
-}
-
-int phx_render(phx_context *c, const phx_camera *cam, const phx_render_params *p) {
-  if (!c || !cam || !p) return PHX_ERR_INVALID;
-  if (!c->have_scene) return fail(c, PHX_ERR_NO_SCENE, "render before set_scene");
-  if (p->width == 0 || p->height == 0) return fail(c, PHX_ERR_INVALID, "render: empty frame");
-  if (p->adaptive_sampling) return fail(c, PHX_ERR_UNSUPPORTED, "render: adaptive sampling is not offered");
-  const uint64_t npix64 = uint64_t(p->width) * p->height;
-  if (npix64 > (1u << 26)) return fail(c, PHX_ERR_INVALID, "render: frame larger than 2^26 pixels");
-  const uint32_t npix = uint32_t(npix64);
-  const uint32_t spp = std::max<uint32_t>(p->samples_per_pixel, 1);
-  const uint32_t max_depth = p->max_depth > 0 ? p->max_depth : 1u;
-  const uint32_t max_it = max_depth + 1u + 8u;  // pathtracer.cpp:682
-  if (max_it > kMaxIterations) return fail(c, PHX_ERR_INVALID, "render: max_depth too large");
-  HIP_TRY(c, hipSetDevice(c->device));
-  const auto t0 = std::chrono::steady_clock::now();
-
-  // Samples per pass: enough path slots to fill the GPU several times over.
-  uint32_t target = c->tuning.paths_in_flight ? c->tuning.paths_in_flight : (1u << 22);
-  uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(spp, target / npix));
-  const uint32_t cap = npix * k;
-  if (cap > c->q_capacity) {
-    for (int b = 0; b < 2; ++b) {
-      HIP_TRY(c, c->q_o[b].reserve(size_t(cap) * 16));
-      HIP_TRY(c, c->q_d[b].reserve(size_t(cap) * 16));
-      HIP_TRY(c, c->q_tp[b].reserve(size_t(cap) * 16));
-      HIP_TRY(c, c->q_st[b].reserve(size_t(cap) * 16));
-    }
-    HIP_TRY(c, c->q_hit.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->sh_o.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->sh_d.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->sh_c.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->slot_L.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->slot_aov0.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->slot_aov1.reserve(size_t(cap) * 16));
-    c->q_capacity = cap;
-  }
-  HIP_TRY(c, c->counters.reserve(kCounterWords * 4));
-  HIP_TRY(c, c->stats_buf.reserve(kStatWords * 8));
-  int r = ensure_trace_scratch(c);
-  if (r) return r;
-  if (c->f_w != p->width || c->f_h != p->height || !c->f_accum.p) {
-    HIP_TRY(c, c->f_accum.reserve(size_t(npix) * 16));
-    HIP_TRY(c, c->f_aov0.reserve(size_t(npix) * 16));
-    HIP_TRY(c, c->f_aov1.reserve(size_t(npix) * 16));
-    HIP_TRY(c, c->f_color.reserve(size_t(npix) * 12));
-    HIP_TRY(c, c->f_alpha.reserve(size_t(npix) * 4));
-    HIP_TRY(c, c->f_depth.reserve(size_t(npix) * 4));
-    HIP_TRY(c, c->f_normal.reserve(size_t(npix) * 12));
-    HIP_TRY(c, c->f_albedo.reserve(size_t(npix) * 12));
-    c->f_w = p->width;
-    c->f_h = p->height;
-  }
-  WaveQueues q{};
-  for (int b = 0; b < 2; ++b) {
-    q.o[b] = c->q_o[b].as<float4>();
-    q.d[b] = c->q_d[b].as<float4>();
-    q.tp[b] = c->q_tp[b].as<float4>();
-    q.st[b] = c->q_st[b].as<uint4>();
-  }
-  q.hit = c->q_hit.as<float4>();
-  q.sh_o = c->sh_o.as<float4>();
-  q.sh_d = c->sh_d.as<float4>();
-  q.sh_c = c->sh_c.as<float4>();
-  q.slot_L = c->slot_L.as<float4>();
-  q.slot_aov0 = c->slot_aov0.as<float4>();
-  q.slot_aov1 = c->slot_aov1.as<float4>();
-  q.counters = c->counters.as<uint32_t>();
-  q.stats = c->stats_buf.as<unsigned long long>();
-  q.spill = c->spill.as<int>();
-  FrameBuffers fb{c->f_accum.as<float4>(), c->f_aov0.as<float4>(), c->f_aov1.as<float4>(), c->f_color.as<float>(),
-                  c->f_alpha.as<float>(),  c->f_depth.as<float>(), c->f_normal.as<float>(), c->f_albedo.as<float>()};
-
-  WaveParams w{};
-  w.cam = *cam;
-  w.width = p->width;
-  w.height = p->height;
-  w.npix = npix;
-  w.max_depth = max_depth;
-  w.bottom_up = p->bottom_up;
-  for (int i = 0; i < 4; ++i) w.region[i] = p->image_region[i];
-  w.bg = vec3{p->background_color[0], p->background_color[1], p->background_color[2]};
-  w.bg_alpha = p->background_alpha;
-  w.ambient = vec3{p->ambient_color[0] * p->ambient_radiance, p->ambient_color[1] * p->ambient_radiance,
-                   p->ambient_color[2] * p->ambient_radiance};
-  w.throughput_clamp = p->throughput_clamp;
-  w.clamp_direct = p->clamp_direct_sum;
-  w.clamp_indirect = p->clamp_indirect_sum;
-  w.max_diffuse = p->max_diffuse_bounces;
-  w.max_glossy = p->max_glossy_bounces;
-  w.max_transmission = p->max_transmission_bounces;
-  w.max_volume = p->max_volume_bounces;
-
-  hipStream_t st = c->stream;
-  const bool timed = c->tuning.time_kernels != 0;
-  const uint32_t passes = (spp + k - 1) / k;
-  const size_t n_ev = timed ? size_t(passes) * max_it * 3 * 2 : 0;
-  while (c->events.size() < n_ev) {
-    hipEvent_t e;
-    HIP_TRY(c, hipEventCreate(&e));
-    c->events.push_back(e);
-  }
-  size_t ev = 0;
-  auto mark = [&]() -> hipError_t { return timed ? hipEventRecord(c->events[ev++], st) : hipSuccess; };
-  HIP_TRY(c, launch_clear_u32(reinterpret_cast<uint32_t *>(q.stats), kStatWords * 2, st));
-  for (uint32_t s0 = 0; s0 < spp; s0 += k) {
-    w.k = std::min(k, spp - s0);
-    w.nslots = npix * w.k;
-    w.sample_base = s0 + p->sample_offset;
-    HIP_TRY(c, launch_reset(q, w.nslots, st));
-    HIP_TRY(c, launch_generate(w, q, st));
-    for (uint32_t it = 0; it < max_it; ++it) {
-      HIP_TRY(c, mark());
-      HIP_TRY(c, launch_extend(c->dev, q, it, c->trace_grid, st));
-      HIP_TRY(c, mark());
-      HIP_TRY(c, mark());
-      HIP_TRY(c, launch_shade(c->dev, w, q, it, st));
-      HIP_TRY(c, mark());
-      HIP_TRY(c, mark());
-      HIP_TRY(c, launch_shadow(c->dev, w, q, it, c->trace_grid, st));
-      HIP_TRY(c, mark());
-    }
-    HIP_TRY(c, launch_accumulate(w, q, fb, s0 == 0, st));
-  }
-  HIP_TRY(c, launch_resolve(npix, spp, fb, st));
-  unsigned long long hs[kStatWords];
-  HIP_TRY(c, hipMemcpyAsync(hs, q.stats, sizeof hs, hipMemcpyDeviceToHost, st));
-  HIP_TRY(c, hipStreamSynchronize(st));
-  const auto t1 = std::chrono::steady_clock::now();
-  c->stats.extend_launches = c->stats.shade_launches = c->stats.shadow_launches = 0;
-  c->stats.extend_ms = c->stats.shade_ms = c->stats.shadow_ms = 0.0;
-  for (size_t e = 0; e + 1 < ev; e += 6) {
-    float a = 0.f, b = 0.f, d = 0.f;
-    HIP_TRY(c, hipEventElapsedTime(&a, c->events[e], c->events[e + 1]));
-    HIP_TRY(c, hipEventElapsedTime(&b, c->events[e + 2], c->events[e + 3]));
-    HIP_TRY(c, hipEventElapsedTime(&d, c->events[e + 4], c->events[e + 5]));
-    c->stats.extend_ms += a;
-    c->stats.shade_ms += b;
-    c->stats.shadow_ms += d;
-    c->stats.extend_launches++;
-    c->stats.shade_launches++;
-    c->stats.shadow_launches++;
-  }
-  c->stats.closest_rays = hs[kStatClosest];
-  c->stats.shadow_rays = hs[kStatShadow];
-  c->stats.shade_events = hs[kStatShade];
-  c->stats.samples = uint64_t(npix) * spp;
-  c->stats.pixels = npix;
-  c->stats.last_render_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  c->have_result = true;
   return PHX_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/phx.h"
+#include "dev_trace.h"
 #include "integrator.h"
 
 namespace phx_detail {
@@ -67,14 +68,14 @@ struct phx_context {
   phx::DevScene dev{};
 
   // wavefront storage
-  DBuf q_o[2], q_d[2], q_tp[2], q_st[2], q_hit, sh_o, sh_d, sh_c, slot_L, slot_aov0, slot_aov1, counters, stats_buf,
-      spill, fetch;
+  DBuf q_o[2], q_d[2], q_tp[2], q_st[2], q_hit, sh_o, sh_d, sh_c, sample_L, aov_s, counters, stats_buf, spill, fetch;
   uint32_t q_capacity = 0;
   uint32_t trace_grid = 0;
   std::vector<hipEvent_t> events;  // kernel timing (phx_tuning.time_kernels)
+  uint32_t *pinned = nullptr;      // host-visible counter readback (hipHostMalloc)
 
-  // resolved frame (RenderResult) and its accumulators
-  DBuf f_accum, f_aov0, f_aov1, f_color, f_alpha, f_depth, f_normal, f_albedo;
+  // resolved frame (RenderResult) and its accumulator
+  DBuf f_accum, f_color, f_alpha, f_depth, f_normal, f_albedo;
   uint32_t f_w = 0, f_h = 0;
   bool have_result = false;
 
@@ -107,6 +108,15 @@ inline int upload(phx_context *c, phx_detail::DBuf &b, const T *src, size_t coun
   HIP_TRY(c, b.reserve(sizeof(T) * count));
   HIP_TRY(c, hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
   c->scene_bytes += sizeof(T) * count;
+  return PHX_OK;
+}
+
+// Persistent traversal grid (8 work-groups of 256 per CU) and its per-thread
+// stack spill area.
+inline int ensure_trace_scratch(phx_context *c) {
+  if (c->trace_grid == 0) c->trace_grid = uint32_t(c->cu_count) * 8u;
+  HIP_TRY(c, c->spill.reserve(size_t(c->trace_grid) * 256 * (phx::kMaxStack - phx::kShortStack) * sizeof(int)));
+  HIP_TRY(c, c->fetch.reserve(64));
   return PHX_OK;
 }
 

@@ -1,0 +1,312 @@
+// One shading event of a path: pt_shade (pathtracer.cpp:765-1090) as a device
+// function shared by the streaming integrator and the reference-structure
+// baseline.  Inputs are the path's ray and state and its closest hit; outputs
+// are the continuing ray (if any), the shadow ray (if any) and the emission /
+// background added to the sample's radiance.
+#pragma once
+
+#include "dev_scene.h"
+
+namespace phx {
+
+constexpr uint32_t kEmissionOnly = 0x80000000u;
+
+struct ShadeParams {
+  phx_camera cam;
+  uint32_t width, height, npix;
+  uint32_t max_depth;
+  uint32_t max_it;
+  uint32_t spp;
+  uint32_t sample_offset;
+  int32_t bottom_up;
+  float region[4];
+  vec3 bg;
+  float bg_alpha;
+  vec3 ambient;
+  float throughput_clamp, clamp_direct, clamp_indirect;
+  int32_t max_diffuse, max_glossy, max_transmission, max_volume;
+};
+
+struct PathState {
+  vec3 ro, rd;
+  float tmin, tmax;
+  vec3 tp;
+  float prev_pdf;
+  uint32_t depth;
+  uint32_t path_it;  // bounce-loop iteration of this path (the RNG dimension)
+  uint32_t kinds;
+  bool primary;
+  float path_dist;
+};
+
+struct ShadeOut {
+  bool cont;
+  PathState next;
+  bool shadow;
+  vec3 so, sd;
+  float stmax;
+  vec3 sc;
+  vec3 add;      // radiance added to the sample
+  float add_alpha;
+  bool primary_event;  // the camera ray ended here (AOV source)
+};
+
+PHX_DEV bool count_bounce(uint32_t &kinds, uint32_t shift, int limit) {
+  uint32_t c = (kinds >> shift) & 0x7fu;
+  if (c < 0x7fu) ++c;
+  kinds = (kinds & ~(0x7fu << shift)) | (c << shift);
+  return limit >= 0 && int(c) > limit;
+}
+PHX_DEV vec3 clamp_contribution(vec3 c, int light_bounce, float ds, float is) {
+  const float limit = light_bounce > 0 ? is : ds;
+  if (!(limit > 0.f)) return c;
+  const float sum = fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
+  return sum > limit ? c * (limit / sum) : c;
+}
+
+// Surface attributes at a hit (mesh_intersector.h:126-178).
+struct SurfaceHit {
+  vec3 p, ng, sn;
+  vec2 uv;
+  vec3 color;
+  bool has_color;
+  uint32_t prim, mid;
+};
+PHX_DEV SurfaceHit surface_at(const DevScene &s, uint32_t tri, float t, float b1, float b2, vec3 ro, vec3 rd) {
+  SurfaceHit h;
+  const TriRec tr = s.tris[tri];
+  h.prim = __float_as_uint(tr.v0.w);
+  h.p = ro + rd * t;
+  h.ng = normalize(cross(vec3{tr.e1.x, tr.e1.y, tr.e1.z}, vec3{tr.e2.x, tr.e2.y, tr.e2.z}));
+  const uint32_t i0 = s.indices[h.prim * 3], i1 = s.indices[h.prim * 3 + 1], i2 = s.indices[h.prim * 3 + 2];
+  const float w0 = 1.f - b1 - b2;
+  h.sn = h.ng;
+  if (s.normals) {
+    h.sn = normalize(ld3(s.normals + size_t(i0) * 3) * w0 + ld3(s.normals + size_t(i1) * 3) * b1 +
+                     ld3(s.normals + size_t(i2) * 3) * b2);
+    if (dot(h.sn, h.ng) < 0.f) h.sn = h.sn * -1.f;
+  }
+  h.uv = vec2{0.f, 0.f};
+  if (s.texcoords) {
+    const float *a = s.texcoords + size_t(i0) * 2, *b = s.texcoords + size_t(i1) * 2, *c = s.texcoords + size_t(i2) * 2;
+    h.uv = {a[0] * w0 + b[0] * b1 + c[0] * b2, a[1] * w0 + b[1] * b1 + c[1] * b2};
+  }
+  h.has_color = false;
+  h.color = vec3{0.f, 0.f, 0.f};
+  if (s.colors && s.colors[size_t(i0) * 3] >= 0.f) {
+    h.color = ld3(s.colors + size_t(i0) * 3) * w0 + ld3(s.colors + size_t(i1) * 3) * b1 +
+              ld3(s.colors + size_t(i2) * 3) * b2;
+    h.has_color = true;
+  }
+  h.mid = s.material_ids ? s.material_ids[h.prim] : 0u;
+  // pathtracer.cpp:906-910
+  if (!(dot(h.sn, h.sn) > 0.25f)) h.sn = h.ng;
+  else if (dot(h.sn, h.ng) < 0.f) h.sn = h.sn * -1.f;
+  return h;
+}
+
+PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState &ps, bool hr_hit, uint32_t htri,
+                        float ht, float b1, float b2, Rng &rng, ShadeOut &out) {
+  out.cont = false;
+  out.shadow = false;
+  out.add = vec3{0.f, 0.f, 0.f};
+  out.add_alpha = 0.f;
+  out.primary_event = false;
+  const vec3 ro = ps.ro, rd = ps.rd;
+  const uint32_t depth = ps.depth;
+  bool primary = ps.primary;
+  uint32_t kinds = ps.kinds;
+  float path_dist = ps.path_dist;
+  const vec3 tp = ps.tp;
+  const float prev_pdf = ps.prev_pdf;
+  const bool emission_only = (kinds & kEmissionOnly) != 0u;
+  // The continuing path inherits everything; fields are overwritten below.
+  out.next = ps;
+  out.next.path_it = ps.path_it + 1u;
+
+  if (s.vol_count > 0) {
+    // Scattering inside a volume in front of the hit (:789-857).
+    vec3 alb{0.f, 0.f, 0.f};
+    const float t_end = hr_hit ? ht : ps.tmax;
+    const float ts = vol_sample_scatter(s, ro, rd, ps.tmin, t_end, rng, alb);
+    if (ts >= 0.f) {
+      const vec3 p = ro + rd * ts;
+      if (primary) {
+        primary = false;
+        out.add_alpha += 1.f;
+        out.primary_event = true;
+      }
+      if (depth >= w.max_depth || emission_only) return;
+      const vec3 tpv = tp * alb;
+      const float phase = 0.25f * kInvPi;
+      const Direct ds = sample_direct(s, p, rng);
+      if (ds.valid) {
+        const float wm = ds.mis ? power_heuristic(ds.pdf, phase) : 1.f;
+        out.sc = clamp_contribution(tpv * ds.L * (phase * wm / ds.pdf), int(depth), w.clamp_direct, w.clamp_indirect);
+        out.shadow = true;
+        out.so = p;
+        out.sd = ds.wi;
+        out.stmax = ds.dist < 1e29f ? ds.dist * (1.f - 1e-4f) : 1e30f;
+      }
+      vec3 tn = tpv;
+      if (depth >= 2) {
+        const float surv = fminf(fmaxf(max3(tpv), 0.05f), 0.95f);
+        if (rng.next() >= surv) return;
+        tn = tpv * (1.f / surv);
+      }
+      if (count_bounce(kinds, 24u, w.max_volume)) kinds |= kEmissionOnly;
+      out.cont = true;
+      out.next.ro = p;
+      out.next.rd = uniform_sphere(rng);
+      out.next.tmin = 0.f;
+      out.next.tmax = 1e30f;
+      out.next.tp = tn;
+      out.next.prev_pdf = phase;
+      out.next.depth = depth + 1u;
+      out.next.kinds = kinds;
+      out.next.primary = false;
+      return;
+    }
+  }
+
+  if (!hr_hit) {
+    // Miss: background for camera rays, environment with MIS (:860-898).
+    vec3 Lm{0.f, 0.f, 0.f};
+    if (primary) {
+      if (s.has_env && s.env_visible) {
+        Lm = env_eval(s, rd);
+        out.add_alpha += 1.f;
+      } else {
+        Lm = w.bg * w.bg_alpha;
+        out.add_alpha += w.bg_alpha;
+      }
+    } else {
+      Lm = w.ambient;
+      if (s.has_env) {
+        float wm = 1.f;
+        if (prev_pdf > 0.f) wm = power_heuristic(prev_pdf, env_pdf(s, rd) / float(s.strategy_count));
+        Lm = Lm + env_eval(s, rd) * wm;
+      }
+    }
+    vec3 fm = tp * Lm;
+    if (!primary || (s.has_env && s.env_visible))
+      fm = clamp_contribution(fm, int(depth) - 1, w.clamp_direct, w.clamp_indirect);
+    out.add = fm;
+    return;
+  }
+
+  const SurfaceHit sh = surface_at(s, htri, ht, b1, b2, ro, rd);
+  const vec3 p = sh.p, ng = sh.ng, wo = -rd;
+  float alpha = 1.f;
+  const phx_material mat = resolve_material(s, sh.mid, sh.uv, sh.has_color, sh.color, alpha);
+  const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0 && primary;
+  if (hidden || (alpha < 1.f && rng.next() >= alpha)) {
+    // Stochastic transparency (:929-951): continue behind the surface.
+    float ntmin = s.ray_eps, ntmax = 1e30f;
+    path_dist += ht;
+    if (primary) {
+      float cn, cf;
+      clip_range(w.cam, rd, cn, cf);
+      if (cf < 1e30f) {
+        ntmax = cf - path_dist;
+        if (ntmax < s.ray_eps) {
+          ntmin = 1.f;
+          ntmax = 0.f;
+        }
+      }
+    }
+    out.cont = true;
+    out.next.ro = p;
+    out.next.tmin = ntmin;
+    out.next.tmax = ntmax;
+    out.next.prev_pdf = -1.f;
+    out.next.path_dist = path_dist;
+    return;
+  }
+  vec3 n = ng, sn = sh.sn;
+  const bool front = dot(n, wo) >= 0.f;
+  if (!(mat.transmission > 0.f) && !front) {
+    n = n * -1.f;
+    sn = sn * -1.f;
+  }
+  if (mat.normal_tex >= 0 && s.tex_count > 0) {
+    vec3 nm = tex_sample(s, mat.normal_tex, sh.uv.x, sh.uv.y);
+    nm = nm * 2.f - vec3{1.f, 1.f, 1.f};
+    vec3 ta, bi;
+    onb(sn, ta, bi);
+    sn = normalize(ta * nm.x + bi * nm.y + sn * nm.z);
+  }
+  if (primary) {
+    primary = false;
+    out.add_alpha += 1.f;
+    out.primary_event = true;
+  }
+  if (material_may_emit(mat) && emits_on_side(mat, front)) {
+    float wm = 1.f;
+    if (prev_pdf > 0.f && s.em_count > 0) {
+      const float pa = s.prim_light_pdf[sh.prim];
+      const float cl = fabsf(dot(ng, wo));
+      if (pa > 0.f && cl > 0.f) wm = power_heuristic(prev_pdf, pa * ht * ht / cl / float(s.strategy_count));
+    }
+    out.add = clamp_contribution(tp * material_emission(mat) * wm, int(depth) - 1, w.clamp_direct, w.clamp_indirect);
+  }
+  if ((mat.flags & PHX_MATERIAL_FLAG_EMITTER_ONLY) != 0 || depth >= w.max_depth || emission_only) return;
+  {
+    // Next event estimation (:1006-1035).
+    const Direct ds = sample_direct(s, p, rng);
+    if (ds.valid) {
+      const vec3 f = bsdf_eval(mat, wo, ds.wi, n, sn);
+      if (f.x > 0.f || f.y > 0.f || f.z > 0.f) {
+        float wm = 1.f;
+        if (ds.mis) wm = power_heuristic(ds.pdf, bsdf_pdf(mat, wo, ds.wi, n, sn));
+        out.sc = clamp_contribution(tp * f * ds.L * (wm / ds.pdf), int(depth), w.clamp_direct, w.clamp_indirect);
+        const vec3 sorg = p + n * s.ray_eps;
+        vec3 sdir = ds.wi;
+        float stmax = 1e30f;
+        if (ds.dist < 1e29f) {
+          const vec3 tl = p + ds.wi * ds.dist - sorg;
+          const float sdist = length(tl);
+          sdir = sdist > 0.f ? tl * (1.f / sdist) : ds.wi;
+          stmax = sdist * (1.f - 1e-3f);
+        }
+        out.shadow = true;
+        out.so = sorg;
+        out.sd = sdir;
+        out.stmax = stmax;
+      }
+    }
+  }
+  // BSDF sample, Russian roulette, next ray (:1037-1089).
+  const BsdfSample bs = bsdf_sample(mat, wo, n, sn, rng);
+  if (bs.pdf <= 0.f || (bs.f.x <= 0.f && bs.f.y <= 0.f && bs.f.z <= 0.f)) return;
+  vec3 tpn = tp * (bs.f * (1.f / bs.pdf));
+  if (depth >= 2) {
+    const float surv = fminf(fmaxf(max3(tpn), 0.05f), 0.95f);
+    if (rng.next() >= surv) return;
+    tpn = tpn * (1.f / surv);
+  }
+  if (w.throughput_clamp > 0.f) {
+    const float mm = max3(tpn);
+    if (mm > w.throughput_clamp) tpn = tpn * (w.throughput_clamp / mm);
+  }
+  if (!finite3(tpn)) return;
+  const vec3 wi = normalize(bs.wi);
+  const float side = dot(wi, ng) < 0.f ? -1.f : 1.f;
+  const bool through = (dot(wi, ng) < 0.f) != (dot(wo, ng) < 0.f);
+  const bool exceeded = through ? count_bounce(kinds, 16u, w.max_transmission)
+                                : (bs.diffuse ? count_bounce(kinds, 0u, w.max_diffuse)
+                                              : count_bounce(kinds, 8u, w.max_glossy));
+  if (exceeded) kinds |= kEmissionOnly;
+  out.cont = true;
+  out.next.ro = p + ng * (side * s.ray_eps);
+  out.next.rd = wi;
+  out.next.tmin = 0.f;
+  out.next.tmax = 1e30f;
+  out.next.tp = tpn;
+  out.next.prev_pdf = bs.specular ? -1.f : bs.pdf;
+  out.next.depth = depth + 1u;
+  out.next.kinds = kinds;
+  out.next.primary = false;
+}
+
+}  // namespace phx

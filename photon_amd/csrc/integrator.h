@@ -1,81 +1,81 @@
-// Wavefront state of the integrator and the launchers of its kernels.
+// Streaming wavefront integrator: state layout and kernel launchers.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
-#include "dev_scene.h"
+#include "dev_shade.h"
 
 namespace phx {
 
-// Counter words (uint32) of one pass.  Iteration `it` reads the ray queue
-// counted by kRayCount + it and the shadow queue counted by kShadowCount + it;
-// the persistent traversal kernels fetch work through kFetch* + it.  All are
-// cleared by k_reset at the start of a pass, so no host read or memset sits
-// inside the bounce loop.
-constexpr uint32_t kMaxIterations = 255;
+// Counter words of one streaming run, in rings of kRing entries indexed by
+// ring(it) for wavefront iteration `it` (a global step of the wavefront, not
+// of a path): the ray queue of `it` is counted by kRayCount + ring(it), the
+// shadow queue it fills by kShadowCount + ring(it); the persistent traversal
+// kernels fetch work through kFetch* + ring(it).  kNextSample is the next
+// sample to start.  k_reset clears everything at the start of a run and
+// k_regen_commit clears the entries of it + 1 before they are used, so no
+// host write or memset sits inside the bounce loop.
+constexpr uint32_t kMaxIterations = 255;  // per path (RenderParams bound)
+constexpr uint32_t kRing = 16;
 constexpr uint32_t kRayCount = 0;
-constexpr uint32_t kShadowCount = kRayCount + kMaxIterations + 1;
-constexpr uint32_t kFetchExtend = kShadowCount + kMaxIterations + 1;
-constexpr uint32_t kFetchShadow = kFetchExtend + kMaxIterations + 1;
-constexpr uint32_t kCounterWords = kFetchShadow + kMaxIterations + 1;
+constexpr uint32_t kShadowCount = kRayCount + kRing;
+constexpr uint32_t kFetchExtend = kShadowCount + kRing;
+constexpr uint32_t kFetchShadow = kFetchExtend + kRing;
+constexpr uint32_t kRegenTake = kFetchShadow + kRing;
+constexpr uint32_t kNextSample = kRegenTake + kRing;
+constexpr uint32_t kCounterWords = kNextSample + 4;
+__host__ __device__ inline uint32_t ring(uint32_t it) { return it & (kRing - 1); }
 
 constexpr uint32_t kStatClosest = 0;
 constexpr uint32_t kStatShadow = 1;
 constexpr uint32_t kStatShade = 2;
 constexpr uint32_t kStatWords = 8;
 
-struct WaveParams {
-  phx_camera cam;
-  uint32_t width, height, npix;
-  uint32_t nslots;       // npix * k
-  uint32_t k;            // samples per pixel in this pass
-  uint32_t sample_base;  // sample index of slot group 0 (includes sample_offset)
-  uint32_t max_depth;
-  int32_t bottom_up;
-  float region[4];
-  vec3 bg;
-  float bg_alpha;
-  vec3 ambient;
-  float throughput_clamp, clamp_direct, clamp_indirect;
-  int32_t max_diffuse, max_glossy, max_transmission, max_volume;
+// A streaming run renders samples [0, total) of the chunk: sample id
+// sid = s * npix + pixel, with sample index sample_base + s.
+struct StreamParams {
+  ShadeParams sp;
+  uint32_t total;        // samples in this run
+  uint32_t capacity;     // path slots (queue length)
+  uint32_t sample_base;  // sample index of s = 0 (sample_offset + first spp of the chunk)
+  uint32_t chunk_s0;     // first spp index of the chunk (AOV bookkeeping)
 };
 
-// Queue storage (capacity = path slots of a pass).  Ray queue entries:
-// o = (origin, tmin), d = (direction, tmax), tp = (throughput, prev_pdf),
-// st = (slot, depth | primary << 16, bounce kinds, path distance bits).
-struct WaveQueues {
+// Queue entries: o = (origin, tmin), d = (direction, tmax),
+// tp = (throughput, prev_pdf), st = (sample id, depth | path_it << 8 |
+// primary << 16, bounce kinds, camera-ray distance bits).
+struct StreamQueues {
   float4 *o[2];
   float4 *d[2];
   float4 *tp[2];
   uint4 *st[2];
-  float4 *hit;      // (t, b1, b2, leaf triangle bits)
-  float4 *sh_o;     // (origin, tmin)
-  float4 *sh_d;     // (direction, tmax)
-  float4 *sh_c;     // (NEE contribution, slot bits)
-  float4 *slot_L;   // (radiance, alpha) per slot
-  float4 *slot_aov0;  // (normal, depth)
-  float4 *slot_aov1;  // (albedo, written)
+  float4 *hit;       // (t, b1, b2, leaf triangle bits)
+  float4 *sh_o;      // (origin, path_it bits); shadow rays start at t = 0
+  float4 *sh_d;      // (direction, tmax)
+  float4 *sh_c;      // (NEE contribution, sample id bits)
+  float4 *sample_L;  // per sample of the run: (radiance, alpha)
+  uint32_t *aov_s;   // per pixel: first sample index with a camera-ray event
   uint32_t *counters;
   unsigned long long *stats;
-  int *spill;       // per-thread traversal stack spill
+  int *spill;        // per-thread traversal stack spill
 };
 
 struct FrameBuffers {
   float4 *accum;  // (radiance sum, alpha sum)
-  float4 *aov0;   // (normal, depth)
-  float4 *aov1;   // (albedo, written)
   float *color, *alpha, *depth, *normal, *albedo;
 };
 
-hipError_t launch_reset(const WaveQueues &q, uint32_t nslots, hipStream_t st);
-hipError_t launch_generate(const WaveParams &w, const WaveQueues &q, hipStream_t st);
-hipError_t launch_extend(const DevScene &s, const WaveQueues &q, uint32_t it, uint32_t grid, hipStream_t st);
-hipError_t launch_shade(const DevScene &s, const WaveParams &w, const WaveQueues &q, uint32_t it, hipStream_t st);
-hipError_t launch_shadow(const DevScene &s, const WaveParams &w, const WaveQueues &q, uint32_t it, uint32_t grid,
+hipError_t launch_reset(const StreamQueues &q, hipStream_t st);
+hipError_t launch_regen(const DevScene &s, const StreamParams &w, const StreamQueues &q, uint32_t it, hipStream_t st);
+hipError_t launch_extend(const DevScene &s, const StreamQueues &q, uint32_t it, uint32_t grid, hipStream_t st);
+hipError_t launch_shade(const DevScene &s, const StreamParams &w, const StreamQueues &q, uint32_t it, hipStream_t st);
+hipError_t launch_shadow(const DevScene &s, const StreamParams &w, const StreamQueues &q, uint32_t it, uint32_t grid,
                          hipStream_t st);
-hipError_t launch_accumulate(const WaveParams &w, const WaveQueues &q, const FrameBuffers &f, bool first,
-                             hipStream_t st);
-hipError_t launch_resolve(uint32_t npix, uint32_t ns, const FrameBuffers &f, hipStream_t st);
+hipError_t launch_reduce(const StreamParams &w, const StreamQueues &q, const FrameBuffers &f, bool first,
+                         hipStream_t st);
+hipError_t launch_aov_init(uint32_t npix, const StreamQueues &q, hipStream_t st);
+hipError_t launch_resolve(const DevScene &s, const ShadeParams &sp, const StreamQueues &q, const FrameBuffers &f,
+                          uint32_t ns, uint32_t *fetch, uint32_t grid, hipStream_t st);
 
 // Seam A kernels (trace.hip).
 hipError_t launch_trace_compact(const DevScene &s, const float *org, const float *dir, const float *tmin,
