@@ -39,6 +39,7 @@ struct StreamParams {
   uint32_t capacity;     // path slots (queue length)
   uint32_t sample_base;  // sample index of s = 0 (sample_offset + first spp of the chunk)
   uint32_t chunk_s0;     // first spp index of the chunk (AOV bookkeeping)
+  uint32_t regen_when_empty;  // 1: start new samples only into an empty queue (pass by pass)
 };
 
 // Queue entries: o = (origin, tmin), d = (direction, tmax),

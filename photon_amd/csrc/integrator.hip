@@ -78,7 +78,7 @@ __global__ void k_reset(StreamQueues q) {
 __global__ __launch_bounds__(256) void k_regen(StreamParams w, StreamQueues q, uint32_t it) {
   const uint32_t used = q.counters[kRayCount + ring(it)];
   const uint32_t next = q.counters[kNextSample];
-  const uint32_t free_slots = w.capacity - used;
+  const uint32_t free_slots = (w.regen_when_empty && used != 0) ? 0u : w.capacity - used;
   const uint32_t left = w.total - next;
   const uint32_t take = free_slots < left ? free_slots : left;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -112,16 +112,6 @@ __global__ void k_regen_commit(StreamQueues q, uint32_t it) {
   }
 }
 
-// Next batch of 64 items for the wave; reads the counter before the atomic so
-// that exhausted queues cost one load per wave instead of one atomic.
-PHX_DEV uint32_t fetch_work(uint32_t *next, uint32_t count) {
-  uint32_t base = 0;
-  if (lane_id() == 0) {
-    base = __hip_atomic_load(next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (base < count) base = atomicAdd(next, 64u);
-  }
-  return __shfl(base, 0);
-}
 
 // Closest hit over the ray queue of iteration `it` (persistent waves).
 __global__ __launch_bounds__(256) void k_extend(DevScene s, StreamQueues q, uint32_t it) {
@@ -132,8 +122,9 @@ __global__ __launch_bounds__(256) void k_extend(DevScene s, StreamQueues q, uint
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], 256, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   const uint32_t lane = lane_id();
+  WorkCursor cur = first_chunk();
   while (true) {
-    const uint32_t base = fetch_work(&q.counters[kFetchExtend + ring(it)], count);
+    const uint32_t base = next_batch(cur, &q.counters[kFetchExtend + ring(it)], count);
     if (base >= count) break;
     const uint32_t i = base + lane;
     const bool valid = i < count;
@@ -202,8 +193,9 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene s, StreamParams w, Stre
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], 256, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   const uint32_t lane = lane_id();
+  WorkCursor cur = first_chunk();
   while (true) {
-    const uint32_t base = fetch_work(&q.counters[kFetchShadow + ring(it)], count);
+    const uint32_t base = next_batch(cur, &q.counters[kFetchShadow + ring(it)], count);
     if (base >= count) break;
     const uint32_t i = base + lane;
     const bool valid = i < count;
@@ -264,8 +256,9 @@ __global__ __launch_bounds__(256) void k_resolve(DevScene s, ShadeParams w, Stre
   LaneStack st{&lds[threadIdx.x], 256, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   const uint32_t lane = lane_id();
   const float inv = ns > 0 ? 1.f / float(ns) : 0.f;
+  WorkCursor cur = first_chunk();
   while (true) {
-    const uint32_t base = fetch_work(fetch, w.npix);
+    const uint32_t base = next_batch(cur, fetch, w.npix);
     if (base >= w.npix) break;
     const uint32_t p = base + lane;
     if (p >= w.npix) continue;

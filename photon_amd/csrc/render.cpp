@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 
 #include "context.h"
 
@@ -122,6 +124,10 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
 
   hipStream_t st = c->stream;
   const bool timed = c->tuning.time_kernels != 0;
+  // PHX_TRACE_COUNTS=N: read the queue length every N iterations and print it.
+  const char *tc = std::getenv("PHX_TRACE_COUNTS");
+  const bool trace_counts = tc && std::atoi(tc) > 0;
+  const uint32_t batch = trace_counts ? uint32_t(std::atoi(tc)) : kIterBatch;
   size_t ev = 0;
   auto mark = [&]() -> hipError_t {
     if (!timed) return hipSuccess;
@@ -144,11 +150,12 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
     w.capacity = std::min<uint32_t>(cap, w.total);
     w.sample_base = p->sample_offset + s0;
     w.chunk_s0 = s0;
+    w.regen_when_empty = std::getenv("PHX_REGEN_WHEN_EMPTY") ? 1u : 0u;
     HIP_TRY(c, launch_reset(q, st));
     HIP_TRY(c, launch_regen(c->dev, w, q, 0, st));
     uint32_t it = 0;
     while (true) {
-      for (uint32_t j = 0; j < kIterBatch; ++j, ++it) {
+      for (uint32_t j = 0; j < batch; ++j, ++it) {
         HIP_TRY(c, mark());
         HIP_TRY(c, launch_extend(c->dev, q, it, c->trace_grid, st));
         HIP_TRY(c, mark());
@@ -163,6 +170,9 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
       HIP_TRY(c, hipMemcpyAsync(&c->pinned[0], q.counters + kRayCount + ring(it), 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(c, hipMemcpyAsync(&c->pinned[1], q.counters + kNextSample, 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(c, hipStreamSynchronize(st));
+      if (trace_counts)
+        std::fprintf(stderr, "[phx] run %u iteration %u: queue %u, started %u of %u samples\n", s0, it, c->pinned[0],
+                     c->pinned[1], w.total);
       if (c->pinned[0] == 0 && c->pinned[1] >= w.total) break;
       if (it >= kIterationGuard)
         return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");

@@ -17,8 +17,9 @@ __global__ __launch_bounds__(256) void k_trace(DevScene s, const float *org, con
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], 256, spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   const uint32_t lane = lane_id();
+  WorkCursor cur = first_chunk();
   while (true) {
-    const uint32_t base = wave_fetch(fetch);
+    const uint32_t base = next_batch(cur, fetch, n);
     if (base >= n) break;
     const uint32_t i = base + lane;
     if (i >= n) continue;
