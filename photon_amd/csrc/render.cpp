@@ -70,7 +70,9 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
   const ShadeParams sp = shade_params(*cam, *p, max_depth, max_it);
   const uint32_t spp_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(spp, kSampleBudget / npix)));
   const uint64_t chunk_total = uint64_t(npix) * spp_chunk;
-  const uint32_t want = c->tuning.paths_in_flight ? c->tuning.paths_in_flight : (1u << 22);
+  // 2^25 path slots (about 6 GB of queues): fewer, fuller launches; HBM is
+  // plentiful on MI355X.
+  const uint32_t want = c->tuning.paths_in_flight ? c->tuning.paths_in_flight : (1u << 25);
   const uint32_t cap = uint32_t(std::min<uint64_t>(want, chunk_total));
 
   if (cap > c->q_capacity) {
