@@ -11,7 +11,10 @@ in HBM before the timed region.  Prints one JSON line.
             active closest ray (32 B ray read + 16 B hit written), divided by
             the kernel's HIP-event time over the timed steps
   cpu_baseline  the oracle (CPU restatement of the reference integrator) on a
-            64-row band of the same frame, on this host's cores
+            band of rows of the same frame, on this host's cores
+  gpu_baseline  the same frame rendered with the reference's algorithmic
+            structure on this GPU (phx_render_reference_structure), and the
+            speedup of the wavefront backend over it
 """
 import argparse
 import json
@@ -36,6 +39,8 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=256)
+    ap.add_argument("--baseline-steps", type=int, default=2,
+                    help="timed frames of the reference-structure GPU baseline (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -51,7 +56,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not offered: this build runs on one GPU "
-              "(the path shards by sample_offset ranges; see DESIGN.md)", file=sys.stderr)
+              "(the path shards by sample_offset ranges)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -115,6 +120,26 @@ def main():
     if os.path.exists(traffic_file):
         roofline["traffic"] = json.load(open(traffic_file)).get("hbm_bytes_per_launch")
 
+    # Same frame with the reference's algorithmic structure on this GPU (the
+    # stand-in for the reference's Kokkos-HIP build): binary median-split BVH,
+    # full-width bounce loop with a host readback per iteration.
+    gpu_base = None
+    if args.baseline_steps > 0:
+        pt.render(read=False, structure="reference")  # builds its BVH, warms up
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        for i in range(args.baseline_steps):
+            pt.render(read=False, structure="reference")
+            print(f"[bench] reference-structure step {i} {pt.ctx.stats().last_render_ms:.1f} ms",
+                  file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        bwall = time.perf_counter() - tb
+        bval = npix * args.spp * args.baseline_steps / bwall
+        gpu_base = {"kind": "reference-structure proxy (binary median BVH, 64-entry stack, full-width "
+                            "max_depth+9 bounce loop, per-iteration host readback)",
+                    "value": bval, "unit": "samples/s", "ms_per_step": bwall * 1e3 / args.baseline_steps,
+                    "steps": args.baseline_steps, "speedup": (samples / wall) / bval}
+
     cpu = None
     if not args.no_cpu_baseline:
         import oracle
@@ -151,6 +176,7 @@ def main():
                                "shadow": sdw_ms / args.steps},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "gpu_baseline": gpu_base,
         "bvh_build_s": build_s,
     }
     print(json.dumps(line))

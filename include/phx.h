@@ -328,6 +328,16 @@ int phx_read_result(phx_context *ctx, float *color_rgb, float *alpha, float *dep
 /* Device pointer of the resolved color (h*w*3 floats), valid until the next render. */
 int phx_result_device_color(phx_context *ctx, const float **d_color);
 
+/* Baseline: the same frame with the reference's algorithmic structure on this
+ * GPU, standing in for the reference's Kokkos-HIP build (which needs Kokkos):
+ * binary median-split BVH with 44-byte nodes (bvh.cpp:20-125), 64-entry
+ * per-thread stack traversal (mesh_intersector.h:57-124), max_depth + 9
+ * full-width iterations per sample of trace_closest (96-byte hit records),
+ * pt_shade, trace_occluded and pt_nee_accum_pack with a host readback of the
+ * active count after each (pathtracer.cpp:564-1159, kokkos_backend.cpp:21-86).
+ * Same result interface as phx_render. */
+int phx_render_reference_structure(phx_context *ctx, const phx_camera *camera, const phx_render_params *params);
+
 /* Disney BSDF of the shading kernels on n tuples in device memory: materials,
  * wo, wi, geometric normal, shading normal (3 floats each).  phx_bsdf_eval
  * replaces disney_bsdf_eval + disney_bsdf_pdf (disney_bsdf.h:288-355): f*cos

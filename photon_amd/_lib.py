@@ -42,6 +42,7 @@ def _sig(lib):
         "phx_bsdf_eval": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P]),
         "phx_bsdf_sample": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P, P, P]),
         "phx_render": (C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams)]),
+        "phx_render_reference_structure": (C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams)]),
         "phx_read_result": (C.c_int, [P, PF, PF, PF, PF, PF]),
         "phx_result_device_color": (C.c_int, [P, C.POINTER(P)]),
         "phx_camera_perspective": (C.c_int, [C.POINTER(abi.Camera), PF, PF, PF, C.c_float, C.c_float, C.c_float,

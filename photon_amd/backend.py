@@ -158,10 +158,19 @@ class PathTracer:
         self._desc = scene.desc()
         self.ctx.check(lib().phx_set_scene(self.ctx.h, C.byref(self._desc)))
 
-    def render(self, read=True):
+    def render(self, read=True, structure="wavefront"):
+        """Renders one frame.  structure="reference" runs the baseline with the
+        reference's algorithmic structure (full-width bounce loop over a binary
+        median-split BVH) instead of the streaming wavefront."""
         if self.camera is None:
             raise RuntimeError("PathTracer.render: no camera")
-        self.ctx.check(lib().phx_render(self.ctx.h, C.byref(self.camera), C.byref(self.params)))
+        if structure == "wavefront":
+            fn = lib().phx_render
+        elif structure == "reference":
+            fn = lib().phx_render_reference_structure
+        else:
+            raise ValueError(f"PathTracer.render: unknown structure {structure!r}")
+        self.ctx.check(fn(self.ctx.h, C.byref(self.camera), C.byref(self.params)))
         if not read:
             return None
         return self.read_result()

@@ -180,6 +180,9 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   }
   c->triangle_count = d->triangle_count;
   c->vertex_count = d->vertex_count;
+  c->host_positions.assign(d->positions, d->positions + size_t(d->vertex_count) * 3);
+  c->host_indices.assign(d->indices, d->indices + size_t(d->triangle_count) * 3);
+  c->px_built = false;
   // Bounds of the triangles' vertices (the BVH root box).
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (uint32_t i = 0; i < d->triangle_count * 3; ++i) {

@@ -82,6 +82,15 @@ struct phx_context {
   // render bridge (seam B)
   DBuf b_color, b_alpha, b_depth, b_normal, b_albedo;
   uint32_t b_w = 0, b_h = 0, b_samples = 0, b_counter = 0;
+
+  // reference-structure baseline (proxy.hip): host mesh copy, its median-split
+  // BVH (built on first use) and full-width per-slot state
+  std::vector<float> host_positions;
+  std::vector<uint32_t> host_indices;
+  bool px_built = false;
+  uint32_t px_root = 0, px_slots = 0;
+  DBuf px_nodes, px_prims, px_tris, px_o, px_d, px_tp, px_st, px_hit, px_sh_o, px_sh_d, px_sh_c, px_flags, px_accum,
+      px_count;
 };
 
 namespace phx_detail {

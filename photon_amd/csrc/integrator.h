@@ -66,6 +66,9 @@ struct FrameBuffers {
   float *color, *alpha, *depth, *normal, *albedo;
 };
 
+// RenderParams -> the shading constants of one frame (render.cpp).
+ShadeParams make_shade_params(const phx_camera &cam, const phx_render_params &p, uint32_t max_depth, uint32_t max_it);
+
 hipError_t launch_reset(const StreamQueues &q, hipStream_t st);
 hipError_t launch_regen(const DevScene &s, const StreamParams &w, const StreamQueues &q, uint32_t it, hipStream_t st);
 hipError_t launch_extend(const DevScene &s, const StreamQueues &q, uint32_t it, uint32_t grid, hipStream_t st);

@@ -24,7 +24,11 @@ constexpr uint32_t kIterBatch = 8;
 // turns a logic error into an error code instead of a hang.
 constexpr uint32_t kIterationGuard = 1u << 26;
 
-ShadeParams shade_params(const phx_camera &cam, const phx_render_params &p, uint32_t max_depth, uint32_t max_it) {
+}  // namespace
+
+namespace phx {
+
+ShadeParams make_shade_params(const phx_camera &cam, const phx_render_params &p, uint32_t max_depth, uint32_t max_it) {
   ShadeParams w{};
   w.cam = cam;
   w.width = p.width;
@@ -50,7 +54,7 @@ ShadeParams shade_params(const phx_camera &cam, const phx_render_params &p, uint
   return w;
 }
 
-}  // namespace
+}  // namespace phx
 
 extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_render_params *p) {
   if (!c || !cam || !p) return PHX_ERR_INVALID;
@@ -67,7 +71,7 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
   HIP_TRY(c, hipSetDevice(c->device));
   const auto t0 = std::chrono::steady_clock::now();
 
-  const ShadeParams sp = shade_params(*cam, *p, max_depth, max_it);
+  const ShadeParams sp = make_shade_params(*cam, *p, max_depth, max_it);
   const uint32_t spp_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(spp, kSampleBudget / npix)));
   const uint64_t chunk_total = uint64_t(npix) * spp_chunk;
   // 2^25 path slots (about 6 GB of queues): fewer, fuller launches; HBM is
