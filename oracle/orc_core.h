@@ -8,6 +8,19 @@
 // bit-for-bit where its arithmetic is IEEE single (no FMA contraction: build
 // with -ffp-contract=off, as the reference's x86-64 Release build has none);
 // T = double is the FP64 engine of the finite-difference gradient checks.
+//
+// Pinning (tests/test_oracle.py): RNG, hash, power heuristic, ONB, triangle
+// test, reference BVH (against brute force), Disney special cases (Lambert,
+// metal GGX, sheen, clearcoat, anisotropy limit) and invariants, env map,
+// camera, majorants, ratio/delta tracking and two integrator closed forms.
+// Parity unpinned (no reference vector or closed form fixes them):
+//   - the subsurface term of eval_diffuse (disney_bsdf.h:177-193);
+//   - the lobe-selection probabilities of mixed materials, disney_lobe_weights
+//     (disney_bsdf.h:148-175): only their self-consistency is checked;
+//   - transcendentals at the ulp level (the reference goes through Kokkos
+//     5.0.1 math; here host libm);
+//   - the exact random-number consumption of volume tracking (checked in
+//     expectation only).
 #pragma once
 
 #include <algorithm>

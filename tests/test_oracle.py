@@ -506,3 +506,29 @@ def test_float_and_double_engines_agree():
     b, _, _ = o.render(cam, p, use_double=True)
     # same sample streams: paths agree except rare rounding-driven divergences
     assert np.mean(np.abs(a - b) < 1e-3 * (1 + np.abs(b))) > 0.95
+
+
+def test_emissive_quad_irradiance_closed_form():
+    # NEE over an emissive mesh (emissive CDF, area sampling, area-to-solid-angle
+    # pdf, one-sided emission) combined by MIS with BSDF hits on the emitter:
+    # a white Lambertian floor of albedo a under a downward-facing square
+    # emitter of radiance Le (black, so nothing bounces back) reflects
+    # a * Le * F at the point below the centre, F the differential-area to
+    # parallel-rectangle form factor (sum of four corner rectangles).
+    a, le, h, half = 0.5, 3.0, 1.0, 0.5
+    fv, ft = scenes._quad((-50, 0, 50), (100, 0, 0), (0, 0, -100))
+    ev, et = scenes._quad((-half, h, -half), (2 * half, 0, 0), (0, 0, 2 * half))
+    v = np.concatenate([fv, ev]).astype(np.float32)
+    t = np.concatenate([ft, et + len(fv)]).astype(np.uint32)
+    mats = [scenes.diffuse((a, a, a)), scenes.emitter((le, le, le))]
+    s = scenes.Scene(v, t, mats, material_ids=np.array([0, 0, 1, 1], np.uint32))
+    o = oracle.OracleScene(s)
+    cam = oracle.camera_perspective((0, 0.5, 3.0), (0, 0, 0), (0, 1, 0), 0.5, 1.0)
+    p = abi.default_render_params(4, 4, 4096, 3)
+    img, _, _ = o.render(cam, p)
+
+    def corner(x, y):
+        return (x / math.sqrt(1 + x * x) * math.atan(y / math.sqrt(1 + x * x)) +
+                y / math.sqrt(1 + y * y) * math.atan(x / math.sqrt(1 + y * y))) / (2 * math.pi)
+    f = 4 * corner(half / h, half / h)
+    assert img.mean() == pytest.approx(a * le * f, rel=0.01)
