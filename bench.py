@@ -28,6 +28,78 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def batched_form(pt_c1, c1, args):
+    """16 views in one phx_render_batch against the same 16 views as plain
+    launches: resolutions from {64..1024}^2 and spp from {4..64} (seeded); views
+    0-7 are the 737K-triangle scene from orbit cameras, views 8-15 Cornell
+    boxes with random wall colours.  Returns the figures for the JSON line."""
+    import numpy as np
+    import torch
+
+    from photon_amd import abi, scenes
+    from photon_amd.backend import BatchRenderer, HipBackend, PathTracer, make_camera
+
+    rng = np.random.default_rng(2024)
+    res = rng.choice([64, 128, 256, 512, 1024], 16)
+    spp = rng.choice([4, 8, 16, 32, 64], 16)
+    views = []
+    for v in range(16):
+        w = h = int(res[v])
+        if v < 8:
+            pt, s = pt_c1, c1
+            ang = rng.uniform(0, 2 * np.pi)
+        else:
+            walls = tuple(tuple(rng.uniform(0.05, 0.8, 3)) for _ in range(3))
+            s = scenes.cornell_box(wall_colors=walls)
+            pt = PathTracer(HipBackend(device=0))
+            pt.set_scene(s)
+            ang = rng.uniform(-0.3, 0.3)
+        eye = np.asarray(s.camera["look_from"], np.float64)
+        at = np.asarray(s.camera["look_at"], np.float64)
+        d = eye - at
+        c, sn = np.cos(ang), np.sin(ang)
+        eye = at + np.array([c * d[0] + sn * d[2], d[1], -sn * d[0] + c * d[2]])
+        cam = make_camera(eye, at, s.camera["vup"], s.camera["vfov"], 1.0)
+        views.append((pt, cam, abi.default_render_params(w, h, int(spp[v]), args.depth)))
+    total = sum(p.width * p.height * p.samples_per_pixel for _, _, p in views)
+
+    def plain_pass():
+        t = time.perf_counter()
+        for pt, cam, p in views:
+            pt.camera, pt.params = cam, p
+            pt.render(read=False)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    plain_pass()  # warm-up: allocations
+    seq = min(plain_pass() for _ in range(args.batch_reps))
+    br = BatchRenderer(device=0)
+    tuning = None
+    if not args.no_autotune:
+        t, ms = br.autotune(views, repeats=1)
+        tuning = {f: getattr(t, f) for f, _ in t._fields_}
+        print(f"[bench] autotuned batch shape {tuning}: {ms:.1f} ms", file=sys.stderr, flush=True)
+    br.render(views)  # warm-up: allocations, graph capture
+    torch.cuda.synchronize()
+    times = []
+    for i in range(args.batch_reps):
+        t = time.perf_counter()
+        br.render(views)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        print(f"[bench] batched step {i} {times[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    tb = min(times)
+    # each view of the batch equals its plain render (checked on the smallest)
+    small = min(range(16), key=lambda v: views[v][2].width * views[v][2].samples_per_pixel)
+    pt, cam, p = views[small]
+    pt.camera, pt.params = cam, p
+    same = bool(np.array_equal(pt.render().color, br.read(small).color))
+    return {"views": 16, "samples": total, "ms": tb * 1e3, "samples_per_s": total / tb,
+            "sequential_ms": seq * 1e3, "sequential_samples_per_s": total / seq, "speedup": seq / tb,
+            "resolutions": [int(x) for x in res], "spp": [int(x) for x in spp], "tuning": tuning,
+            "equal_to_plain": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +113,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=256)
     ap.add_argument("--baseline-steps", type=int, default=2,
                     help="timed frames of the reference-structure GPU baseline (0: skip)")
+    ap.add_argument("--batch-reps", type=int, default=3,
+                    help="timed repetitions of the batched 16-view launch and of its 16 plain launches (0: skip)")
+    ap.add_argument("--no-autotune", action="store_true", help="batched form with the default launch shape")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -153,6 +228,8 @@ def main():
                "sample": f"rows {y0}-{y0 + rows - 1} of the same 1024x1024x64spp frame ({cpu_samples} samples, "
                          f"{secs:.1f} s)"}
 
+    batched = batched_form(pt, scene, args) if args.batch_reps > 0 else None
+
     line = {
         "metric": "samples/s & Mrays/s at 1024^2 64spp (plain)",
         "value": samples / wall,
@@ -177,6 +254,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_baseline": gpu_base,
+        "batched": batched,
         "bvh_build_s": build_s,
     }
     print(json.dumps(line))

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PHX_ABI_VERSION 1
+#define PHX_ABI_VERSION 2
 
 enum {
   PHX_OK = 0,
@@ -249,15 +249,17 @@ typedef struct phx_hit_compact {
   float b2;
 } phx_hit_compact;
 
-/* Launch shape and queue sizing, chosen by the autotuner or by hand. */
+/* Launch shape and queue sizing, chosen by the autotuner (phx_autotune) or by
+ * hand.  0 in a field selects its default. */
 typedef struct phx_tuning {
-  uint32_t trace_block;      /* work-group size of the traversal kernels (64..1024) */
-  uint32_t shade_block;      /* work-group size of the shading kernels */
-  uint32_t paths_in_flight;  /* path slots per wavefront pass (0: auto) */
-  uint32_t sort_by_material; /* 1: partition the shade queue by material key */
-  uint32_t use_graph;        /* 1: replay the bounce loop as a captured hipGraph */
-  uint32_t time_kernels;     /* 1: time each kernel launch with HIP events (phx_stats) */
-  uint32_t pad_[2];
+  uint32_t trace_block;          /* work-group size of the traversal kernels: 128 or 256 (256) */
+  uint32_t trace_groups_per_cu;  /* persistent traversal work-groups per CU, 1..32 (8) */
+  uint32_t reserved0;            /* must be 0 */
+  uint32_t queue_chunk;          /* rays a wave takes per queue fetch, 64..4096, multiple of 64 (256) */
+  uint32_t paths_in_flight;      /* path slots of the wavefront (2^25) */
+  uint32_t sort_by_material;     /* reserved, must be 0 */
+  uint32_t use_graph;            /* 1: phx_render_batch replays its bounce loop as a captured hipGraph */
+  uint32_t time_kernels;         /* 1: time each kernel launch with HIP events (phx_stats) */
 } phx_tuning;
 
 /* Counters kept by the kernels (active rays only). */
@@ -327,6 +329,31 @@ int phx_read_result(phx_context *ctx, float *color_rgb, float *alpha, float *dep
                     float *albedo_rgb);
 /* Device pointer of the resolved color (h*w*3 floats), valid until the next render. */
 int phx_result_device_color(phx_context *ctx, const float **d_color);
+
+/* ---- batched views ---------------------------------------------------------
+ * Renders many views, of one or several scenes, in one wavefront: all views
+ * share the path queues, so the kernels stay full across views of uneven size
+ * and the per-frame tail is paid once.  The reference renders one view per
+ * PathTracer::render call (pathtracer.cpp:564-1159); each view's result here
+ * equals phx_render of that view bit for bit (same sample streams, same
+ * summation order).  `scene` names the context holding the view's scene
+ * (phx_set_scene); it must live on ctx's device and stay alive during the
+ * call; it may be ctx itself.  With phx_tuning.use_graph = 1 the bounce loop
+ * is captured once as a hipGraph and replayed. */
+typedef struct phx_view {
+  phx_context *scene;
+  phx_camera camera;
+  phx_render_params params;
+} phx_view;
+int phx_render_batch(phx_context *ctx, const phx_view *views, uint32_t count);
+/* Result of view `view` of the last phx_render_batch, as phx_read_result. */
+int phx_read_batch_result(phx_context *ctx, uint32_t view, float *color_rgb, float *alpha, float *depth,
+                          float *normal_xyz, float *albedo_rgb);
+/* Times phx_render_batch of `views` under a set of launch shapes (traversal
+ * work-group size and groups per CU, queue chunk, path slots) and keeps the fastest in ctx's tuning.  `best` (optional) receives it,
+ * `best_ms` (optional) its time.  Each candidate runs `repeats` times (>= 1). */
+int phx_autotune(phx_context *ctx, const phx_view *views, uint32_t count, uint32_t repeats, phx_tuning *best,
+                 double *best_ms);
 
 /* Baseline: the same frame with the reference's algorithmic structure on this
  * GPU, standing in for the reference's Kokkos-HIP build (which needs Kokkos):

@@ -44,6 +44,10 @@ def _sig(lib):
         "phx_render": (C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams)]),
         "phx_render_reference_structure": (C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams)]),
         "phx_read_result": (C.c_int, [P, PF, PF, PF, PF, PF]),
+        "phx_render_batch": (C.c_int, [P, C.POINTER(abi.View), C.c_uint32]),
+        "phx_read_batch_result": (C.c_int, [P, C.c_uint32, PF, PF, PF, PF, PF]),
+        "phx_autotune": (C.c_int, [P, C.POINTER(abi.View), C.c_uint32, C.c_uint32, C.POINTER(abi.Tuning),
+                                   C.POINTER(C.c_double)]),
         "phx_result_device_color": (C.c_int, [P, C.POINTER(P)]),
         "phx_camera_perspective": (C.c_int, [C.POINTER(abi.Camera), PF, PF, PF, C.c_float, C.c_float, C.c_float,
                                              C.c_float]),

@@ -113,9 +113,9 @@ class HitCompact(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [
-        ("trace_block", C.c_uint32), ("shade_block", C.c_uint32), ("paths_in_flight", C.c_uint32),
-        ("sort_by_material", C.c_uint32), ("use_graph", C.c_uint32), ("time_kernels", C.c_uint32),
-        ("pad_", C.c_uint32 * 2),
+        ("trace_block", C.c_uint32), ("trace_groups_per_cu", C.c_uint32), ("reserved0", C.c_uint32),
+        ("queue_chunk", C.c_uint32), ("paths_in_flight", C.c_uint32), ("sort_by_material", C.c_uint32),
+        ("use_graph", C.c_uint32), ("time_kernels", C.c_uint32),
     ]
 
 
@@ -152,13 +152,12 @@ class RenderSettings(C.Structure):
 
 
 class View(C.Structure):
-    _fields_ = [
-        ("camera", Camera), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
-        ("sample_offset", C.c_uint32), ("scene_slot", C.c_uint32), ("pad_", C.c_uint32 * 3),
-    ]
+    """phx_view: one view of phx_render_batch."""
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("params", RenderParams)]
 
 
-EXPECTED_SIZES = {Material: 144, Light: 96, TextureInfo: 16, Volume: 144, Hit: 96, HitCompact: 16, Camera: 104}
+EXPECTED_SIZES = {Material: 144, Light: 96, TextureInfo: 16, Volume: 144, Hit: 96, HitCompact: 16, Camera: 104,
+                  Tuning: 32}
 
 
 def check_layout():

@@ -139,6 +139,50 @@ def make_camera(look_from, look_at, vup=(0.0, 1.0, 0.0), vfov=40.0, aspect=1.0, 
     return cam
 
 
+class BatchRenderer:
+    """phx_render_batch: many views (of one or several scenes) in one wavefront
+    with shared queues.  A view is (scene, camera, params) where scene is a
+    PathTracer (or Context) holding the view's scene."""
+
+    def __init__(self, context=None, device=0):
+        self.ctx = context if context is not None else Context(device)
+        self._views = None
+
+    @staticmethod
+    def _table(views):
+        arr = (abi.View * len(views))()
+        for i, (scene, cam, params) in enumerate(views):
+            ctx = scene.ctx if hasattr(scene, "ctx") else scene
+            arr[i].scene = ctx.h
+            arr[i].camera = cam
+            arr[i].params = params
+        return arr
+
+    def render(self, views):
+        arr = self._table(views)
+        self.ctx.check(lib().phx_render_batch(self.ctx.h, arr, len(views)))
+        self._views = [(p.width, p.height) for _, _, p in views]
+
+    def read(self, v):
+        w, h = self._views[v]
+        color = np.empty((h, w, 3), np.float32)
+        alpha = np.empty((h, w), np.float32)
+        depth = np.empty((h, w), np.float32)
+        normal = np.empty((h, w, 3), np.float32)
+        albedo = np.empty((h, w, 3), np.float32)
+        self.ctx.check(lib().phx_read_batch_result(self.ctx.h, v, fptr(color), fptr(alpha), fptr(depth),
+                                                   fptr(normal), fptr(albedo)))
+        return RenderResult(color, alpha, depth, normal, albedo)
+
+    def autotune(self, views, repeats=1):
+        """phx_autotune over `views`: returns (tuning, milliseconds) and keeps it."""
+        arr = self._table(views)
+        t = abi.Tuning()
+        ms = C.c_double()
+        self.ctx.check(lib().phx_autotune(self.ctx.h, arr, len(views), repeats, C.byref(t), C.byref(ms)))
+        return t, ms.value
+
+
 class RenderResult:
     def __init__(self, color, alpha, depth, normal, albedo):
         self.color, self.alpha, self.depth, self.normal, self.albedo = color, alpha, depth, normal, albedo

@@ -95,11 +95,8 @@ int phx_create(phx_context **out, int device) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     c->cu_count = cus;
-  c->tuning.trace_block = 256;
-  c->tuning.shade_block = 256;
-  c->tuning.paths_in_flight = 0;
-  c->tuning.sort_by_material = 0;
-  c->tuning.use_graph = 0;
+  c->tuning = phx_tuning{};
+  c->tuning.use_graph = 1;
   *out = c;
   return PHX_OK;
 }
@@ -111,6 +108,7 @@ void phx_destroy(phx_context *c) {
   // DBuf members free their memory in the destructor.
   hipStream_t s = c->stream;
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
   if (c->pinned) (void)hipHostFree(c->pinned);
   delete c;
   if (s) (void)hipStreamDestroy(s);
@@ -120,11 +118,15 @@ const char *phx_last_error(const phx_context *c) { return c ? c->error.c_str() :
 
 int phx_set_tuning(phx_context *c, const phx_tuning *t) {
   if (!c || !t) return PHX_ERR_INVALID;
-  if (t->trace_block != 0 && t->trace_block != 256) return fail(c, PHX_ERR_UNSUPPORTED, "trace_block must be 256");
-  if (t->shade_block != 0 && t->shade_block != 256) return fail(c, PHX_ERR_UNSUPPORTED, "shade_block must be 256");
+  if (t->trace_block != 0 && t->trace_block != 128 && t->trace_block != 256)
+    return fail(c, PHX_ERR_UNSUPPORTED, "set_tuning: trace_block must be 128 or 256");
+  if (t->trace_groups_per_cu > 32) return fail(c, PHX_ERR_INVALID, "set_tuning: at most 32 work-groups per CU");
+  if (t->reserved0 != 0) return fail(c, PHX_ERR_INVALID, "set_tuning: reserved field must be 0");
+  if (t->queue_chunk != 0 && (t->queue_chunk % 64 != 0 || t->queue_chunk > 4096))
+    return fail(c, PHX_ERR_INVALID, "set_tuning: queue_chunk must be a multiple of 64 up to 4096");
+  if (t->paths_in_flight > (1u << 28)) return fail(c, PHX_ERR_INVALID, "set_tuning: more than 2^28 path slots");
+  if (t->sort_by_material != 0) return fail(c, PHX_ERR_UNSUPPORTED, "set_tuning: sort_by_material is reserved");
   c->tuning = *t;
-  if (!c->tuning.trace_block) c->tuning.trace_block = 256;
-  if (!c->tuning.shade_block) c->tuning.shade_block = 256;
   return PHX_OK;
 }
 int phx_get_tuning(const phx_context *c, phx_tuning *t) {

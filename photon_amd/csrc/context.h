@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -70,7 +71,8 @@ struct phx_context {
   // wavefront storage
   DBuf q_o[2], q_d[2], q_tp[2], q_st[2], q_hit, sh_o, sh_d, sh_c, sample_L, aov_s, counters, stats_buf, spill, fetch;
   uint32_t q_capacity = 0;
-  uint32_t trace_grid = 0;
+  uint32_t trace_grid = 0;  // persistent grid of the seam-A trace kernels and the resolve pass
+  size_t spill_threads = 0;
   std::vector<hipEvent_t> events;  // kernel timing (phx_tuning.time_kernels)
   uint32_t *pinned = nullptr;      // host-visible counter readback (hipHostMalloc)
 
@@ -78,6 +80,17 @@ struct phx_context {
   DBuf f_accum, f_color, f_alpha, f_depth, f_normal, f_albedo;
   uint32_t f_w = 0, f_h = 0;
   bool have_result = false;
+
+  // batched views (phx_render_batch): run tables, concatenated frame buffers
+  DBuf bt_items, bt_first, bt_scenes, bt_accum, bt_color, bt_alpha, bt_depth, bt_normal, bt_albedo, bt_aov;
+  struct BatchView {
+    uint32_t width, height, pix_base;
+  };
+  std::vector<BatchView> bt_views;
+  bool bt_have_result = false;
+  // captured bounce loop of the batch and the launch arguments it was captured with
+  hipGraphExec_t graph = nullptr;
+  std::vector<unsigned char> graph_key;
 
   // render bridge (seam B)
   DBuf b_color, b_alpha, b_depth, b_normal, b_albedo;
@@ -120,11 +133,16 @@ inline int upload(phx_context *c, phx_detail::DBuf &b, const T *src, size_t coun
   return PHX_OK;
 }
 
-// Persistent traversal grid (8 work-groups of 256 per CU) and its per-thread
-// stack spill area.
-inline int ensure_trace_scratch(phx_context *c) {
+// Persistent traversal grid of the seam-A kernels and the resolve pass (8
+// work-groups of 256 per CU), and a per-thread stack spill area for at least
+// `threads` threads (0: that grid).
+inline int ensure_trace_scratch(phx_context *c, size_t threads = 0) {
   if (c->trace_grid == 0) c->trace_grid = uint32_t(c->cu_count) * 8u;
-  HIP_TRY(c, c->spill.reserve(size_t(c->trace_grid) * 256 * (phx::kMaxStack - phx::kShortStack) * sizeof(int)));
+  threads = std::max(threads, size_t(c->trace_grid) * 256);
+  if (threads > c->spill_threads) {
+    HIP_TRY(c, c->spill.reserve(threads * (phx::kMaxStack - phx::kShortStack) * sizeof(int)));
+    c->spill_threads = threads;
+  }
   HIP_TRY(c, c->fetch.reserve(64));
   return PHX_OK;
 }

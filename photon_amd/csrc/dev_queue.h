@@ -31,27 +31,28 @@ __device__ __forceinline__ void wave_count(unsigned long long *counter, bool pre
 }
 
 // Work distribution of the persistent kernels: each wave owns a chunk of
-// kChunk items at a time.  The first chunk of every wave is static (wave index
-// * kChunk); later chunks come from one atomic per wave and chunk on a counter
-// that starts after the static ones.  (A plain atomic at L2: a coherent
-// pre-read of the counter would bypass L2 and serialize at memory.)
-constexpr uint32_t kChunk = 256;
+// `chunk` items (a multiple of 64) at a time.  The first chunk of every wave is
+// static (wave index * chunk); later chunks come from one atomic per wave and
+// chunk on a counter that starts after the static ones.  (A plain atomic at
+// L2: a coherent pre-read of the counter would bypass L2 and serialize at
+// memory.)
+constexpr uint32_t kChunk = 256;  // default chunk
 struct WorkCursor {
   uint32_t base;  // first item of the current chunk
   uint32_t sub;   // next 64-item sub-batch within the chunk
 };
-__device__ __forceinline__ WorkCursor first_chunk() {
+__device__ __forceinline__ WorkCursor first_chunk(uint32_t chunk) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  return WorkCursor{wave * kChunk, 0u};
+  return WorkCursor{wave * chunk, 0u};
 }
 // Base of the wave's next 64-item batch; >= count when the work is done
 // (every later batch is then beyond count too).  The counter starts at 0.
-__device__ __forceinline__ uint32_t next_batch(WorkCursor &c, uint32_t *counter, uint32_t count) {
-  if (c.sub == kChunk) {
+__device__ __forceinline__ uint32_t next_batch(WorkCursor &c, uint32_t *counter, uint32_t count, uint32_t chunk) {
+  if (c.sub == chunk) {
     uint32_t b = 0;
-    if (lane_id() == 0) b = atomicAdd(counter, kChunk);
+    if (lane_id() == 0) b = atomicAdd(counter, chunk);
     b = __shfl(b, 0);
-    c.base = gridDim.x * blockDim.x / 64u * kChunk + b;
+    c.base = gridDim.x * blockDim.x / 64u * chunk + b;
     c.sub = 0;
   }
   const uint32_t r = c.base + c.sub;

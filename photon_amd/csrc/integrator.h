@@ -1,4 +1,5 @@
-// Streaming wavefront integrator: state layout and kernel launchers.
+// Streaming wavefront integrator: state layout, the view policies the kernels
+// are instantiated for, and the kernel launchers.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,11 +11,13 @@ namespace phx {
 // Counter words of one streaming run, in rings of kRing entries indexed by
 // ring(it) for wavefront iteration `it` (a global step of the wavefront, not
 // of a path): the ray queue of `it` is counted by kRayCount + ring(it), the
-// shadow queue it fills by kShadowCount + ring(it); the persistent traversal
-// kernels fetch work through kFetch* + ring(it).  kNextSample is the next
-// sample to start.  k_reset clears everything at the start of a run and
-// k_regen_commit clears the entries of it + 1 before they are used, so no
-// host write or memset sits inside the bounce loop.
+// shadow queue it fills by kShadowCount + ring(it); the persistent kernels
+// fetch work through kFetch* + ring(it).  kNextSample is the next sample to
+// start, kIter the current iteration, kTotal / kCapacity the run's sample
+// count and path slots.  The kernels read the iteration from kIter (so a
+// captured sequence of iterations can be replayed unchanged); k_run_start
+// clears everything and k_commit clears the entries of it + 1 before they are
+// used, so no host write or memset sits inside the bounce loop.
 constexpr uint32_t kMaxIterations = 255;  // per path (RenderParams bound)
 constexpr uint32_t kRing = 16;
 constexpr uint32_t kRayCount = 0;
@@ -23,7 +26,10 @@ constexpr uint32_t kFetchExtend = kShadowCount + kRing;
 constexpr uint32_t kFetchShadow = kFetchExtend + kRing;
 constexpr uint32_t kRegenTake = kFetchShadow + kRing;
 constexpr uint32_t kNextSample = kRegenTake + kRing;
-constexpr uint32_t kCounterWords = kNextSample + 4;
+constexpr uint32_t kIter = kNextSample + 1;
+constexpr uint32_t kTotal = kIter + 1;
+constexpr uint32_t kCapacity = kTotal + 1;
+constexpr uint32_t kCounterWords = kCapacity + 1;
 __host__ __device__ inline uint32_t ring(uint32_t it) { return it & (kRing - 1); }
 
 constexpr uint32_t kStatClosest = 0;
@@ -31,15 +37,48 @@ constexpr uint32_t kStatShadow = 1;
 constexpr uint32_t kStatShade = 2;
 constexpr uint32_t kStatWords = 8;
 
-// A streaming run renders samples [0, total) of the chunk: sample id
-// sid = s * npix + pixel, with sample index sample_base + s.
-struct StreamParams {
-  ShadeParams sp;
-  uint32_t total;        // samples in this run
-  uint32_t capacity;     // path slots (queue length)
-  uint32_t sample_base;  // sample index of s = 0 (sample_offset + first spp of the chunk)
-  uint32_t chunk_s0;     // first spp index of the chunk (AOV bookkeeping)
-  uint32_t regen_when_empty;  // 1: start new samples only into an empty queue (pass by pass)
+// One view's samples [s0, s0 + nspp) inside a streaming run.  A run streams
+// the run-global sample ids [0, total); item k owns [first, first + npix *
+// nspp) with local id l = s * npix + pixel and sample index sample_base + s.
+struct RunItem {
+  ShadeParams sp;        // the view's shading constants
+  uint32_t scene;        // index into the scene table
+  uint32_t first;        // run-global id of the item's first sample
+  uint32_t sample_base;  // sample index of s = 0 (sample_offset + s0)
+  uint32_t s0;           // first spp index of the item (AOV bookkeeping)
+  uint32_t pix_base;     // first pixel of the view in the frame buffers
+  uint32_t view;         // view index (diagnostics)
+};
+
+// View policies.  OneItem: a single view and scene passed as kernel
+// arguments (uniform, scalar loads).  ItemTable: many views in device tables,
+// looked up per lane from the sample id; kOneScene keeps the scene a kernel
+// argument when every view renders the same scene.
+struct OneItem {
+  DevScene scene;
+  RunItem item;
+  PHX_DEV uint32_t find(uint32_t) const { return 0; }
+  PHX_DEV const RunItem &item_at(uint32_t) const { return item; }
+  PHX_DEV const DevScene &scene_of(const RunItem &) const { return scene; }
+};
+template <bool kOneScene>
+struct ItemTable {
+  DevScene scene;          // used when kOneScene
+  const DevScene *scenes;  // used otherwise
+  const RunItem *items;
+  const uint32_t *first;   // items[k].first, ascending, first[0] = 0
+  uint32_t count;
+  PHX_DEV uint32_t find(uint32_t g) const {
+    uint32_t lo = 0, hi = count;  // largest k with first[k] <= g
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (first[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+    return lo;
+  }
+  PHX_DEV const RunItem &item_at(uint32_t k) const { return items[k]; }
+  PHX_DEV const DevScene &scene_of(const RunItem &r) const { return kOneScene ? scene : scenes[r.scene]; }
 };
 
 // Queue entries: o = (origin, tmin), d = (direction, tmax),
@@ -64,20 +103,44 @@ struct StreamQueues {
 struct FrameBuffers {
   float4 *accum;  // (radiance sum, alpha sum)
   float *color, *alpha, *depth, *normal, *albedo;
+  FrameBuffers at(uint32_t pix) const {
+    return {accum + pix, color + size_t(pix) * 3, alpha + pix, depth + pix, normal + size_t(pix) * 3,
+            albedo + size_t(pix) * 3};
+  }
+};
+
+// Launch shape of the bounce loop (phx_tuning).
+struct LoopShape {
+  uint32_t trace_block;  // 128 or 256
+  uint32_t trace_grid;   // work-groups of the persistent traversal kernels
+  uint32_t shade_grid;   // work-groups (of 256) of the shade kernel: covers the path slots
+  uint32_t regen_grid;   // work-groups (of 256) of the regeneration kernel
+  uint32_t chunk;        // work items a wave takes per fetch (multiple of 64)
 };
 
 // RenderParams -> the shading constants of one frame (render.cpp).
 ShadeParams make_shade_params(const phx_camera &cam, const phx_render_params &p, uint32_t max_depth, uint32_t max_it);
 
-hipError_t launch_reset(const StreamQueues &q, hipStream_t st);
-hipError_t launch_regen(const DevScene &s, const StreamParams &w, const StreamQueues &q, uint32_t it, hipStream_t st);
-hipError_t launch_extend(const DevScene &s, const StreamQueues &q, uint32_t it, uint32_t grid, hipStream_t st);
-hipError_t launch_shade(const DevScene &s, const StreamParams &w, const StreamQueues &q, uint32_t it, hipStream_t st);
-hipError_t launch_shadow(const DevScene &s, const StreamParams &w, const StreamQueues &q, uint32_t it, uint32_t grid,
+// The view arguments of one launch: exactly one of the three is used.
+struct ViewArgs {
+  int kind;  // 0: OneItem, 1: ItemTable<true>, 2: ItemTable<false>
+  OneItem one;
+  ItemTable<true> shared;
+  ItemTable<false> many;
+};
+
+hipError_t launch_run_start(const StreamQueues &q, uint32_t total, uint32_t capacity, hipStream_t st);
+// ahead = 0: fill the queue of the current iteration (start of a run);
+// ahead = 1: fill the next iteration's queue and advance the iteration.
+hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead, const LoopShape &ls,
+                        hipStream_t st);
+hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
+hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
+hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
+// Sums the item's samples of each pixel into accum (first: overwrite).
+hipError_t launch_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum, bool first,
                          hipStream_t st);
-hipError_t launch_reduce(const StreamParams &w, const StreamQueues &q, const FrameBuffers &f, bool first,
-                         hipStream_t st);
-hipError_t launch_aov_init(uint32_t npix, const StreamQueues &q, hipStream_t st);
+hipError_t launch_aov_init(uint32_t npix, uint32_t *aov_s, hipStream_t st);
 hipError_t launch_resolve(const DevScene &s, const ShadeParams &sp, const StreamQueues &q, const FrameBuffers &f,
                           uint32_t ns, uint32_t *fetch, uint32_t grid, hipStream_t st);
 

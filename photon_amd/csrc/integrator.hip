@@ -20,6 +20,8 @@
 // summed in sample order: results do not depend on scheduling.
 #include "integrator.h"
 
+#include <type_traits>
+
 #include "dev_queue.h"
 #include "dev_trace.h"
 
@@ -69,37 +71,60 @@ PHX_DEV void camera_path(const ShadeParams &w, uint32_t pixel, uint32_t sample, 
   p.path_dist = 0.f;
 }
 
-__global__ void k_reset(StreamQueues q) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kCounterWords) q.counters[i] = 0u;
+// Clears the counters of a run and records its size.
+__global__ void k_run_start(StreamQueues q, uint32_t total, uint32_t capacity) {
+  const uint32_t i = threadIdx.x;
+  if (i < kCounterWords) q.counters[i] = i == kTotal ? total : (i == kCapacity ? capacity : 0u);
 }
 
-// Fills the free slots of iteration `it`'s queue with new camera paths.
-__global__ __launch_bounds__(256) void k_regen(StreamParams w, StreamQueues q, uint32_t it) {
+// Traversal scene of a lane: the uniform kernel-argument scene, or the two
+// pointers traversal reads, copied out of the lane's scene-table entry.
+template <class P>
+PHX_DEV DevScene trace_scene(const P &v, uint32_t g) {
+  if constexpr (std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value) {
+    return v.scene;
+  } else {
+    const DevScene &sc = v.scene_of(v.item_at(v.find(g)));
+    DevScene ls{};
+    ls.nodes = sc.nodes;
+    ls.tris = sc.tris;
+    return ls;
+  }
+}
+
+// Fills the free slots of a queue with new camera paths (pt_init_sample).
+// ahead = 0: the current iteration's queue; 1: the next one.
+template <class P>
+__global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahead) {
+  const uint32_t it = q.counters[kIter] + ahead;
   const uint32_t used = q.counters[kRayCount + ring(it)];
   const uint32_t next = q.counters[kNextSample];
-  const uint32_t free_slots = (w.regen_when_empty && used != 0) ? 0u : w.capacity - used;
-  const uint32_t left = w.total - next;
+  const uint32_t free_slots = q.counters[kCapacity] - used;
+  const uint32_t left = q.counters[kTotal] - next;
   const uint32_t take = free_slots < left ? free_slots : left;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) q.counters[kRegenTake + ring(it)] = take;
-  if (i >= take) return;
-  const uint32_t sid = next + i;
-  const uint32_t pixel = sid % w.sp.npix;
-  const uint32_t sample = w.sample_base + sid / w.sp.npix;
-  PathState p;
-  camera_path(w.sp, pixel, sample, p);
-  const uint32_t slot = used + i;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid == 0) q.counters[kRegenTake + ring(it)] = take;
   const int buf = it & 1;
-  q.o[buf][slot] = f4(p.ro, p.tmin);
-  q.d[buf][slot] = f4(p.rd, p.tmax);
-  q.tp[buf][slot] = f4(p.tp, p.prev_pdf);
-  q.st[buf][slot] = make_uint4(sid, pack_flags(p), p.kinds, __float_as_uint(p.path_dist));
-  q.sample_L[sid] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t i = tid; i < take; i += gridDim.x * blockDim.x) {
+    const uint32_t g = next + i;
+    const RunItem &r = v.item_at(v.find(g));
+    const uint32_t l = g - r.first;
+    PathState p;
+    camera_path(r.sp, l % r.sp.npix, r.sample_base + l / r.sp.npix, p);
+    const uint32_t slot = used + i;
+    q.o[buf][slot] = f4(p.ro, p.tmin);
+    q.d[buf][slot] = f4(p.rd, p.tmax);
+    q.tp[buf][slot] = f4(p.tp, p.prev_pdf);
+    q.st[buf][slot] = make_uint4(g, pack_flags(p), p.kinds, __float_as_uint(p.path_dist));
+    q.sample_L[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
-__global__ void k_regen_commit(StreamQueues q, uint32_t it) {
+// Commits the regenerated paths, clears the ring entries of the following
+// iteration and (ahead = 1) advances the iteration.
+__global__ void k_commit(StreamQueues q, uint32_t ahead) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const uint32_t it = q.counters[kIter] + ahead;
     const uint32_t take = q.counters[kRegenTake + ring(it)];
     q.counters[kRayCount + ring(it)] += take;
     q.counters[kNextSample] += take;
@@ -109,22 +134,24 @@ __global__ void k_regen_commit(StreamQueues q, uint32_t it) {
     q.counters[kShadowCount + n] = 0u;
     q.counters[kFetchExtend + n] = 0u;
     q.counters[kFetchShadow + n] = 0u;
+    q.counters[kIter] = it;
   }
 }
 
-
-// Closest hit over the ray queue of iteration `it` (persistent waves).
-__global__ __launch_bounds__(256) void k_extend(DevScene s, StreamQueues q, uint32_t it) {
+// Closest hit over the ray queue of the current iteration (persistent waves).
+template <class P, int B>
+__global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chunk) {
+  const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[kRayCount + ring(it)];
   if (count == 0) return;
-  __shared__ int lds[kShortStack * 256];
+  __shared__ int lds[kShortStack * B];
   const int buf = it & 1;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  LaneStack st{&lds[threadIdx.x], 256, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
+  LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   const uint32_t lane = lane_id();
-  WorkCursor cur = first_chunk();
+  WorkCursor cur = first_chunk(chunk);
   while (true) {
-    const uint32_t base = next_batch(cur, &q.counters[kFetchExtend + ring(it)], count);
+    const uint32_t base = next_batch(cur, &q.counters[kFetchExtend + ring(it)], count, chunk);
     if (base >= count) break;
     const uint32_t i = base + lane;
     const bool valid = i < count;
@@ -132,15 +159,22 @@ __global__ __launch_bounds__(256) void k_extend(DevScene s, StreamQueues q, uint
     if (valid) {
       const float4 o = q.o[buf][i], d = q.d[buf][i];
       live = d.w > o.w;
+      const DevScene sc = trace_scene(v, q.st[buf][i].x);
       TraceHit h;
-      traverse<false>(s, xyz(o), xyz(d), o.w, d.w, st, h);
+      traverse<false>(sc, xyz(o), xyz(d), o.w, d.w, st, h);
       q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
     }
     wave_count(&q.stats[kStatClosest], live);
   }
 }
 
-__global__ __launch_bounds__(256) void k_shade(DevScene s, StreamParams w, StreamQueues q, uint32_t it) {
+// pt_shade over the ray queue, one thread per path (a persistent loop would
+// keep the shading constants live across iterations and double the register
+// use); continuing paths and shadow rays go to compacted queues, one atomic
+// per wave.  The grid covers the path slots; threads beyond the queue exit.
+template <class P>
+__global__ __launch_bounds__(256) void k_shade(P v, StreamQueues q) {
+  const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[kRayCount + ring(it)];
   if (blockIdx.x * blockDim.x >= count) return;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -149,24 +183,27 @@ __global__ __launch_bounds__(256) void k_shade(DevScene s, StreamParams w, Strea
   ShadeOut out;
   out.cont = false;
   out.shadow = false;
-  uint32_t sid = 0;
+  uint32_t g = 0;
   if (valid) {
-    const uint4 st = q.st[buf][i];
-    sid = st.x;
-    const PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], st);
+    const uint4 stv = q.st[buf][i];
+    g = stv.x;
+    const RunItem &r = v.item_at(v.find(g));
+    const DevScene &s = v.scene_of(r);
+    const PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], stv);
     const float4 hit = q.hit[i];
     const uint32_t htri = __float_as_uint(hit.w);
-    const uint32_t pixel = sid % w.sp.npix;
-    const uint32_t s_local = sid / w.sp.npix;
-    Rng rng = make_rng(pixel, w.sample_base + s_local, 1u + ps.path_it);
-    shade_path(s, w.sp, ps, htri != 0xffffffffu, htri, hit.x, hit.y, hit.z, rng, out);
+    const uint32_t l = g - r.first;
+    const uint32_t pixel = l % r.sp.npix;
+    const uint32_t s_local = l / r.sp.npix;
+    Rng rng = make_rng(pixel, r.sample_base + s_local, 1u + ps.path_it);
+    shade_path(s, r.sp, ps, htri != 0xffffffffu, htri, hit.x, hit.y, hit.z, rng, out);
     if (out.add.x != 0.f || out.add.y != 0.f || out.add.z != 0.f || out.add_alpha != 0.f) {
-      const float4 L = q.sample_L[sid];
-      q.sample_L[sid] = make_float4(L.x + out.add.x, L.y + out.add.y, L.z + out.add.z, L.w + out.add_alpha);
+      const float4 L = q.sample_L[g];
+      q.sample_L[g] = make_float4(L.x + out.add.x, L.y + out.add.y, L.z + out.add.z, L.w + out.add_alpha);
     }
-    if (out.primary_event) atomicMin(&q.aov_s[pixel], w.chunk_s0 + s_local);
+    if (out.primary_event) atomicMin(&q.aov_s[r.pix_base + pixel], r.s0 + s_local);
     // the reference's loop bound (pathtracer.cpp:682,762)
-    if (out.next.path_it >= w.sp.max_it) out.cont = false;
+    if (out.next.path_it >= r.sp.max_it) out.cont = false;
   }
   wave_count(&q.stats[kStatShade], valid);
   const uint32_t ci = wave_alloc(&q.counters[kRayCount + ring(it + 1)], out.cont);
@@ -175,27 +212,31 @@ __global__ __launch_bounds__(256) void k_shade(DevScene s, StreamParams w, Strea
     q.o[nbuf][ci] = f4(n.ro, n.tmin);
     q.d[nbuf][ci] = f4(n.rd, n.tmax);
     q.tp[nbuf][ci] = f4(n.tp, n.prev_pdf);
-    q.st[nbuf][ci] = make_uint4(sid, pack_flags(n), n.kinds, __float_as_uint(n.path_dist));
+    q.st[nbuf][ci] = make_uint4(g, pack_flags(n), n.kinds, __float_as_uint(n.path_dist));
   }
   const uint32_t si = wave_alloc(&q.counters[kShadowCount + ring(it)], out.shadow);
   if (out.shadow) {
     // the shading path's iteration keys the transmittance stream
     q.sh_o[si] = f4(out.so, __uint_as_float(out.next.path_it - 1u));
     q.sh_d[si] = f4(out.sd, out.stmax);
-    q.sh_c[si] = f4(out.sc, __uint_as_float(sid));
+    q.sh_c[si] = f4(out.sc, __uint_as_float(g));
   }
 }
 
-__global__ __launch_bounds__(256) void k_shadow(DevScene s, StreamParams w, StreamQueues q, uint32_t it) {
+// Any hit + NEE accumulation over the shadow queue (trace_occluded,
+// pt_nee_accum_pack; persistent waves).
+template <class P, int B>
+__global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chunk) {
+  const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[kShadowCount + ring(it)];
   if (count == 0) return;
-  __shared__ int lds[kShortStack * 256];
+  __shared__ int lds[kShortStack * B];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  LaneStack st{&lds[threadIdx.x], 256, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
+  LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   const uint32_t lane = lane_id();
-  WorkCursor cur = first_chunk();
+  WorkCursor cur = first_chunk(chunk);
   while (true) {
-    const uint32_t base = next_batch(cur, &q.counters[kFetchShadow + ring(it)], count);
+    const uint32_t base = next_batch(cur, &q.counters[kFetchShadow + ring(it)], count, chunk);
     if (base >= count) break;
     const uint32_t i = base + lane;
     const bool valid = i < count;
@@ -203,20 +244,22 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene s, StreamParams w, Stre
     if (valid) {
       const float4 o = q.sh_o[i], d = q.sh_d[i], c = q.sh_c[i];
       live = d.w > 0.f;
+      const uint32_t g = __float_as_uint(c.w);
+      const DevScene sc = trace_scene(v, g);
       TraceHit h;
-      const bool occ = traverse<true>(s, xyz(o), xyz(d), 0.f, d.w, st, h);
+      const bool occ = traverse<true>(sc, xyz(o), xyz(d), 0.f, d.w, st, h);
       if (!occ) {
-        const uint32_t sid = __float_as_uint(c.w);
         vec3 contrib = xyz(c);
+        const RunItem &r = v.item_at(v.find(g));
+        const DevScene &s = v.scene_of(r);
         if (s.vol_count > 0) {
-          const uint32_t pixel = sid % w.sp.npix;
-          const uint32_t sample = w.sample_base + sid / w.sp.npix;
-          Rng rng = make_rng(pixel, sample, 0x10000u + __float_as_uint(o.w));
+          const uint32_t l = g - r.first;
+          Rng rng = make_rng(l % r.sp.npix, r.sample_base + l / r.sp.npix, 0x10000u + __float_as_uint(o.w));
           contrib = contrib * vol_transmittance(s, xyz(o), xyz(d), 0.f, d.w, rng);
         }
         if (finite3(contrib)) {
-          const float4 L = q.sample_L[sid];
-          q.sample_L[sid] = make_float4(L.x + contrib.x, L.y + contrib.y, L.z + contrib.z, L.w);
+          const float4 L = q.sample_L[g];
+          q.sample_L[g] = make_float4(L.x + contrib.x, L.y + contrib.y, L.z + contrib.z, L.w);
         }
       }
     }
@@ -224,26 +267,25 @@ __global__ __launch_bounds__(256) void k_shadow(DevScene s, StreamParams w, Stre
   }
 }
 
-// Sums the run's samples of each pixel into the frame, in sample order.
-__global__ __launch_bounds__(256) void k_reduce(StreamParams w, StreamQueues q, FrameBuffers f, uint32_t first) {
+// Sums an item's samples of each pixel into the frame, in sample order.
+__global__ __launch_bounds__(256) void k_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum,
+                                                uint32_t first) {
   const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t npix = w.sp.npix;
   if (pixel >= npix) return;
-  float4 acc = first ? make_float4(0.f, 0.f, 0.f, 0.f) : f.accum[pixel];
-  const uint32_t ns = w.total / npix;
-  for (uint32_t s = 0; s < ns; ++s) {
-    const float4 L = q.sample_L[size_t(s) * npix + pixel];
+  float4 acc = first ? make_float4(0.f, 0.f, 0.f, 0.f) : accum[pixel];
+  for (uint32_t s = 0; s < nspp; ++s) {
+    const float4 L = sample_L[size_t(s) * npix + pixel];
     acc.x += L.x;
     acc.y += L.y;
     acc.z += L.z;
     acc.w += L.w;
   }
-  f.accum[pixel] = acc;
+  accum[pixel] = acc;
 }
 
-__global__ void k_aov_init(uint32_t npix, StreamQueues q) {
+__global__ void k_aov_init(uint32_t npix, uint32_t *aov_s) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < npix) q.aov_s[p] = 0xffffffffu;
+  if (p < npix) aov_s[p] = 0xffffffffu;
 }
 
 // pt_resolve plus the AOVs: the first sample whose camera ray ended at a
@@ -256,9 +298,9 @@ __global__ __launch_bounds__(256) void k_resolve(DevScene s, ShadeParams w, Stre
   LaneStack st{&lds[threadIdx.x], 256, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   const uint32_t lane = lane_id();
   const float inv = ns > 0 ? 1.f / float(ns) : 0.f;
-  WorkCursor cur = first_chunk();
+  WorkCursor cur = first_chunk(kChunk);
   while (true) {
-    const uint32_t base = next_batch(cur, fetch, w.npix);
+    const uint32_t base = next_batch(cur, fetch, w.npix, kChunk);
     if (base >= w.npix) break;
     const uint32_t p = base + lane;
     if (p >= w.npix) continue;
@@ -336,38 +378,67 @@ __global__ __launch_bounds__(256) void k_resolve(DevScene s, ShadeParams w, Stre
 }
 
 // ---- host launchers -----------------------------------------------------------
-hipError_t launch_reset(const StreamQueues &q, hipStream_t st) {
-  hipLaunchKernelGGL(k_reset, dim3((kCounterWords + 255) / 256), dim3(256), 0, st, q);
+namespace {
+template <class F>
+hipError_t with_view(const ViewArgs &v, F f) {
+  switch (v.kind) {
+    case 0: return f(v.one);
+    case 1: return f(v.shared);
+    default: return f(v.many);
+  }
+}
+}  // namespace
+
+hipError_t launch_run_start(const StreamQueues &q, uint32_t total, uint32_t capacity, hipStream_t st) {
+  hipLaunchKernelGGL(k_run_start, dim3(1), dim3(128), 0, st, q, total, capacity);
   return hipGetLastError();
 }
-hipError_t launch_regen(const DevScene &, const StreamParams &w, const StreamQueues &q, uint32_t it, hipStream_t st) {
-  hipLaunchKernelGGL(k_regen, dim3((w.capacity + 255) / 256), dim3(256), 0, st, w, q, it);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(64), 0, st, q, it);
-  return hipGetLastError();
-}
-hipError_t launch_extend(const DevScene &s, const StreamQueues &q, uint32_t it, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_extend, dim3(grid), dim3(256), 0, st, s, q, it);
-  return hipGetLastError();
-}
-hipError_t launch_shade(const DevScene &s, const StreamParams &w, const StreamQueues &q, uint32_t it,
+hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead, const LoopShape &ls,
                         hipStream_t st) {
-  hipLaunchKernelGGL(k_shade, dim3((w.capacity + 255) / 256), dim3(256), 0, st, s, w, q, it);
-  return hipGetLastError();
+  return with_view(v, [&](const auto &p) {
+    using P = std::decay_t<decltype(p)>;
+    hipLaunchKernelGGL(k_regen<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q, ahead);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, q, ahead);
+    return hipGetLastError();
+  });
 }
-hipError_t launch_shadow(const DevScene &s, const StreamParams &w, const StreamQueues &q, uint32_t it, uint32_t grid,
+hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st) {
+  return with_view(v, [&](const auto &p) {
+    using P = std::decay_t<decltype(p)>;
+    if (ls.trace_block == 128)
+      hipLaunchKernelGGL((k_extend<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk);
+    else
+      hipLaunchKernelGGL((k_extend<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk);
+    return hipGetLastError();
+  });
+}
+hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st) {
+  return with_view(v, [&](const auto &p) {
+    using P = std::decay_t<decltype(p)>;
+    hipLaunchKernelGGL(k_shade<P>, dim3(ls.shade_grid), dim3(256), 0, st, p, q);
+    return hipGetLastError();
+  });
+}
+hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st) {
+  return with_view(v, [&](const auto &p) {
+    using P = std::decay_t<decltype(p)>;
+    if (ls.trace_block == 128)
+      hipLaunchKernelGGL((k_shadow<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk);
+    else
+      hipLaunchKernelGGL((k_shadow<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk);
+    return hipGetLastError();
+  });
+}
+hipError_t launch_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum, bool first,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(256), 0, st, s, w, q, it);
+  hipLaunchKernelGGL(k_reduce, dim3((npix + 255) / 256), dim3(256), 0, st, npix, nspp, sample_L, accum,
+                     first ? 1u : 0u);
   return hipGetLastError();
 }
-hipError_t launch_reduce(const StreamParams &w, const StreamQueues &q, const FrameBuffers &f, bool first,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce, dim3((w.sp.npix + 255) / 256), dim3(256), 0, st, w, q, f, first ? 1u : 0u);
-  return hipGetLastError();
-}
-hipError_t launch_aov_init(uint32_t npix, const StreamQueues &q, hipStream_t st) {
-  hipLaunchKernelGGL(k_aov_init, dim3((npix + 255) / 256), dim3(256), 0, st, npix, q);
+hipError_t launch_aov_init(uint32_t npix, uint32_t *aov_s, hipStream_t st) {
+  hipLaunchKernelGGL(k_aov_init, dim3((npix + 255) / 256), dim3(256), 0, st, npix, aov_s);
   return hipGetLastError();
 }
 hipError_t launch_resolve(const DevScene &s, const ShadeParams &sp, const StreamQueues &q, const FrameBuffers &f,

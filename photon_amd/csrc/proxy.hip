@@ -285,7 +285,7 @@ extern "C" int phx_render_reference_structure(phx_context *c, const phx_camera *
   StreamQueues q{};
   q.aov_s = st.aov_s;
   q.spill = c->spill.as<int>();
-  HIP_TRY(c, launch_aov_init(npix, q, stm));
+  HIP_TRY(c, launch_aov_init(npix, q.aov_s, stm));
   for (uint32_t s = 0; s < spp; ++s) {
     hipLaunchKernelGGL(k_px_init, grd, blk, 0, stm, sp, st, s, s == 0 ? 1u : 0u);
     HIP_TRY(c, hipGetLastError());

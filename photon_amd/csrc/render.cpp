@@ -1,30 +1,17 @@
-// phx_render: PathTracer::render (pathtracer.cpp:564-1159) on the streaming
-// wavefront kernels of integrator.hip.
+// phx_render and phx_render_batch: PathTracer::render (pathtracer.cpp:564-1159)
+// on the streaming wavefront kernels of integrator.hip, for one view or for
+// many views sharing the path queues; phx_autotune picks their launch shape.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <utility>
 
 #include "context.h"
-
-namespace {
-
-using namespace phx;
-using namespace phx_detail;
-
-// Per-sample radiance buffer budget of one streaming run (16 B per sample):
-// larger frames are rendered in several runs of whole samples per pixel.
-constexpr uint64_t kSampleBudget = uint64_t(1) << 27;
-// Iterations launched between two reads of the queue counters.
-constexpr uint32_t kIterBatch = 8;
-// Every path ends within max_it iterations and at least one sample starts per
-// iteration, so a run needs at most total + max_it iterations; this guard only
-// turns a logic error into an error code instead of a hang.
-constexpr uint32_t kIterationGuard = 1u << 26;
-
-}  // namespace
+#include "dev_queue.h"
 
 namespace phx {
 
@@ -56,29 +43,193 @@ ShadeParams make_shade_params(const phx_camera &cam, const phx_render_params &p,
 
 }  // namespace phx
 
-extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_render_params *p) {
-  if (!c || !cam || !p) return PHX_ERR_INVALID;
-  if (!c->have_scene) return fail(c, PHX_ERR_NO_SCENE, "render before set_scene");
-  if (p->width == 0 || p->height == 0) return fail(c, PHX_ERR_INVALID, "render: empty frame");
-  if (p->adaptive_sampling) return fail(c, PHX_ERR_UNSUPPORTED, "render: adaptive sampling is not offered");
-  const uint64_t npix64 = uint64_t(p->width) * p->height;
-  if (npix64 > (uint64_t(1) << 26)) return fail(c, PHX_ERR_INVALID, "render: frame larger than 2^26 pixels");
-  const uint32_t npix = uint32_t(npix64);
-  const uint32_t spp = std::max<uint32_t>(p->samples_per_pixel, 1);
+namespace {
+
+using namespace phx;
+using namespace phx_detail;
+
+// Per-sample radiance buffer budget of one streaming run (16 B per sample):
+// larger jobs are rendered in several runs of whole samples per pixel.
+constexpr uint64_t kSampleBudget = uint64_t(1) << 27;
+// Iterations launched (or replayed as one graph) between two reads of the
+// queue counters.
+constexpr uint32_t kIterBatch = 8;
+// Every path ends within max_it iterations and at least one sample starts per
+// iteration, so a run needs at most total + max_it iterations; this guard only
+// turns a logic error into an error code instead of a hang.
+constexpr uint32_t kIterationGuard = 1u << 26;
+constexpr uint32_t kDefaultPaths = 1u << 25;
+
+// One view of a render or a batch.
+struct ViewJob {
+  const phx_context *scene;
+  uint32_t scene_index;
+  ShadeParams sp;
+  uint32_t spp;
+  uint32_t pix_base;
+};
+
+// Item of a run on the host side: the run item plus what the reduce needs.
+struct HostItem {
+  RunItem r;
+  uint32_t nspp;
+};
+struct Run {
+  std::vector<HostItem> items;
+  uint32_t total = 0;
+};
+
+int check_params(phx_context *c, const phx_render_params *p, const char *who) {
+  if (p->width == 0 || p->height == 0) return fail(c, PHX_ERR_INVALID, std::string(who) + ": empty frame");
+  if (p->adaptive_sampling)
+    return fail(c, PHX_ERR_UNSUPPORTED, std::string(who) + ": adaptive sampling is not offered");
+  if (uint64_t(p->width) * p->height > (uint64_t(1) << 26))
+    return fail(c, PHX_ERR_INVALID, std::string(who) + ": frame larger than 2^26 pixels");
   const uint32_t max_depth = p->max_depth > 0 ? p->max_depth : 1u;
+  if (max_depth + 9u > kMaxIterations) return fail(c, PHX_ERR_INVALID, std::string(who) + ": max_depth too large");
+  return PHX_OK;
+}
+
+ViewJob make_job(const phx_context *scene, uint32_t scene_index, const phx_camera &cam, const phx_render_params &p,
+                 uint32_t pix_base) {
+  const uint32_t max_depth = p.max_depth > 0 ? p.max_depth : 1u;
   const uint32_t max_it = max_depth + 1u + 8u;  // pathtracer.cpp:682
-  if (max_it > kMaxIterations) return fail(c, PHX_ERR_INVALID, "render: max_depth too large");
-  HIP_TRY(c, hipSetDevice(c->device));
-  const auto t0 = std::chrono::steady_clock::now();
+  return ViewJob{scene, scene_index, make_shade_params(cam, p, max_depth, max_it),
+                 std::max<uint32_t>(p.samples_per_pixel, 1), pix_base};
+}
 
-  const ShadeParams sp = make_shade_params(*cam, *p, max_depth, max_it);
-  const uint32_t spp_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(spp, kSampleBudget / npix)));
-  const uint64_t chunk_total = uint64_t(npix) * spp_chunk;
-  // 2^25 path slots (about 6 GB of queues): fewer, fuller launches; HBM is
-  // plentiful on MI355X.
-  const uint32_t want = c->tuning.paths_in_flight ? c->tuning.paths_in_flight : (1u << 25);
-  const uint32_t cap = uint32_t(std::min<uint64_t>(want, chunk_total));
+// Splits the views' samples into runs of at most kSampleBudget samples, each
+// a list of (view, first spp, spp count) items in view order.
+std::vector<Run> plan_runs(const std::vector<ViewJob> &views) {
+  // PHX_SAMPLE_BUDGET=N (diagnostic): a smaller run budget, to exercise the
+  // splitting of large jobs at small sizes.
+  uint64_t budget = kSampleBudget;
+  if (const char *e = std::getenv("PHX_SAMPLE_BUDGET")) {
+    const long long b = std::atoll(e);
+    if (b > 0) budget = std::min<uint64_t>(budget, uint64_t(b));
+  }
+  std::vector<Run> runs(1);
+  for (uint32_t v = 0; v < views.size(); ++v) {
+    const ViewJob &j = views[v];
+    const uint32_t npix = j.sp.npix;
+    uint32_t s0 = 0;
+    while (s0 < j.spp) {
+      Run &run = runs.back();
+      const uint64_t room = budget > run.total ? budget - run.total : 0;
+      uint32_t nfit = uint32_t(std::min<uint64_t>(j.spp - s0, room / npix));
+      if (nfit == 0) {
+        if (!run.items.empty()) {
+          runs.emplace_back();
+          continue;
+        }
+        nfit = 1;  // a view whose single sample exceeds the budget runs alone
+      }
+      HostItem h{};
+      h.r.sp = j.sp;
+      h.r.scene = j.scene_index;
+      h.r.first = run.total;
+      h.r.sample_base = j.sp.sample_offset + s0;
+      h.r.s0 = s0;
+      h.r.pix_base = j.pix_base;
+      h.r.view = v;
+      h.nspp = nfit;
+      run.items.push_back(h);
+      run.total += npix * nfit;
+      s0 += nfit;
+    }
+  }
+  return runs;
+}
 
+LoopShape loop_shape(const phx_context *c, uint32_t capacity) {
+  const phx_tuning &t = c->tuning;
+  LoopShape ls{};
+  ls.trace_block = t.trace_block == 128 ? 128u : 256u;
+  ls.trace_grid = uint32_t(c->cu_count) * (t.trace_groups_per_cu ? t.trace_groups_per_cu : 8u);
+  ls.shade_grid = std::max<uint32_t>(1, (capacity + 255) / 256);
+  ls.regen_grid = std::max<uint32_t>(1, std::min<uint32_t>((capacity + 255) / 256, uint32_t(c->cu_count) * 16u));
+  ls.chunk = t.queue_chunk ? t.queue_chunk : kChunk;
+  return ls;
+}
+
+struct Timer {
+  phx_context *c;
+  hipStream_t st;
+  bool on;
+  size_t n = 0;
+  hipError_t mark() {
+    if (!on) return hipSuccess;
+    if (n == c->events.size()) {
+      hipEvent_t e;
+      hipError_t err = hipEventCreate(&e);
+      if (err != hipSuccess) return err;
+      c->events.push_back(e);
+    }
+    return hipEventRecord(c->events[n++], st);
+  }
+};
+
+hipError_t launch_iterations(const ViewArgs &va, const StreamQueues &q, const LoopShape &ls, uint32_t n, Timer *tm,
+                             hipStream_t st) {
+  for (uint32_t j = 0; j < n; ++j) {
+    hipError_t e;
+    if (tm && (e = tm->mark()) != hipSuccess) return e;
+    if ((e = launch_extend(va, q, ls, st)) != hipSuccess) return e;
+    if (tm && (e = tm->mark()) != hipSuccess) return e;
+    if (tm && (e = tm->mark()) != hipSuccess) return e;
+    if ((e = launch_shade(va, q, ls, st)) != hipSuccess) return e;
+    if (tm && (e = tm->mark()) != hipSuccess) return e;
+    if (tm && (e = tm->mark()) != hipSuccess) return e;
+    if ((e = launch_shadow(va, q, ls, st)) != hipSuccess) return e;
+    if (tm && (e = tm->mark()) != hipSuccess) return e;
+    if ((e = launch_regen(va, q, 1, ls, st)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Captures kIterBatch iterations of the bounce loop as a graph, unless the
+// graph already held was captured with the same launch arguments.
+int ensure_graph(phx_context *c, const ViewArgs &va, const StreamQueues &q, const LoopShape &ls) {
+  std::vector<unsigned char> key(sizeof va + sizeof q + sizeof ls);
+  std::memcpy(key.data(), &va, sizeof va);
+  std::memcpy(key.data() + sizeof va, &q, sizeof q);
+  std::memcpy(key.data() + sizeof va + sizeof q, &ls, sizeof ls);
+  if (c->graph && key == c->graph_key) return PHX_OK;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const hipError_t le = launch_iterations(va, q, ls, kIterBatch, nullptr, c->stream);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_fail(c, le != hipSuccess ? le : ce, "capturing the bounce loop");
+  }
+  const hipError_t ie = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    c->graph = nullptr;
+    return hip_fail(c, ie, "hipGraphInstantiate");
+  }
+  c->graph_key = std::move(key);
+  return PHX_OK;
+}
+
+// Streams every view's samples through the shared queues and resolves each
+// view into `fb` at its pix_base (AOV sample indices in aov[pix_base..]).
+int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::vector<const phx_context *> &scenes,
+                 const FrameBuffers &fb, uint32_t *aov, uint32_t total_pixels, bool use_graph) {
+  const std::vector<Run> runs = plan_runs(views);
+  uint32_t max_total = 0;
+  size_t max_items = 1;
+  for (const Run &r : runs) {
+    max_total = std::max(max_total, r.total);
+    max_items = std::max(max_items, r.items.size());
+  }
+  const uint32_t want = c->tuning.paths_in_flight ? c->tuning.paths_in_flight : kDefaultPaths;
+  const uint32_t cap = std::max<uint32_t>(1, std::min<uint32_t>(want, max_total));
   if (cap > c->q_capacity) {
     for (int b = 0; b < 2; ++b) {
       HIP_TRY(c, c->q_o[b].reserve(size_t(cap) * 16));
@@ -92,23 +243,31 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
     HIP_TRY(c, c->sh_c.reserve(size_t(cap) * 16));
     c->q_capacity = cap;
   }
-  HIP_TRY(c, c->sample_L.reserve(size_t(chunk_total) * 16));
-  HIP_TRY(c, c->aov_s.reserve(size_t(npix) * 4));
+  HIP_TRY(c, c->sample_L.reserve(size_t(max_total) * 16));
   HIP_TRY(c, c->counters.reserve(kCounterWords * 4));
   HIP_TRY(c, c->stats_buf.reserve(kStatWords * 8));
   if (!c->pinned) HIP_TRY(c, hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 64, hipHostMallocDefault));
-  int r = ensure_trace_scratch(c);
+  const LoopShape ls = loop_shape(c, cap);
+  int r = ensure_trace_scratch(c, size_t(ls.trace_grid) * ls.trace_block);
   if (r) return r;
-  if (c->f_w != p->width || c->f_h != p->height || !c->f_accum.p) {
-    HIP_TRY(c, c->f_accum.reserve(size_t(npix) * 16));
-    HIP_TRY(c, c->f_color.reserve(size_t(npix) * 12));
-    HIP_TRY(c, c->f_alpha.reserve(size_t(npix) * 4));
-    HIP_TRY(c, c->f_depth.reserve(size_t(npix) * 4));
-    HIP_TRY(c, c->f_normal.reserve(size_t(npix) * 12));
-    HIP_TRY(c, c->f_albedo.reserve(size_t(npix) * 12));
-    c->f_w = p->width;
-    c->f_h = p->height;
+
+  // Item tables of the runs that have several items: first[] is padded to a
+  // power of two with ~0u so the per-lane binary search has a fixed length.
+  size_t pad = 1;
+  while (pad < max_items) pad <<= 1;
+  const bool one_scene = scenes.size() == 1;
+  if (max_items > 1) {
+    HIP_TRY(c, c->bt_items.reserve(pad * sizeof(RunItem)));
+    HIP_TRY(c, c->bt_first.reserve(pad * sizeof(uint32_t)));
+    if (!one_scene) {
+      std::vector<DevScene> table(scenes.size());
+      for (size_t i = 0; i < scenes.size(); ++i) table[i] = scenes[i]->dev;
+      HIP_TRY(c, c->bt_scenes.reserve(table.size() * sizeof(DevScene)));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+      HIP_TRY(c, hipMemcpy(c->bt_scenes.p, table.data(), table.size() * sizeof(DevScene), hipMemcpyHostToDevice));
+    }
   }
+
   StreamQueues q{};
   for (int b = 0; b < 2; ++b) {
     q.o[b] = c->q_o[b].as<float4>();
@@ -121,85 +280,103 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
   q.sh_d = c->sh_d.as<float4>();
   q.sh_c = c->sh_c.as<float4>();
   q.sample_L = c->sample_L.as<float4>();
-  q.aov_s = c->aov_s.as<uint32_t>();
+  q.aov_s = aov;
   q.counters = c->counters.as<uint32_t>();
   q.stats = c->stats_buf.as<unsigned long long>();
   q.spill = c->spill.as<int>();
-  FrameBuffers fb{c->f_accum.as<float4>(), c->f_color.as<float>(), c->f_alpha.as<float>(),
-                  c->f_depth.as<float>(), c->f_normal.as<float>(), c->f_albedo.as<float>()};
 
   hipStream_t st = c->stream;
-  const bool timed = c->tuning.time_kernels != 0;
   // PHX_TRACE_COUNTS=N: read the queue length every N iterations and print it.
   const char *tc = std::getenv("PHX_TRACE_COUNTS");
   const bool trace_counts = tc && std::atoi(tc) > 0;
   const uint32_t batch = trace_counts ? uint32_t(std::atoi(tc)) : kIterBatch;
-  size_t ev = 0;
-  auto mark = [&]() -> hipError_t {
-    if (!timed) return hipSuccess;
-    if (ev == c->events.size()) {
-      hipEvent_t e;
-      hipError_t err = hipEventCreate(&e);
-      if (err != hipSuccess) return err;
-      c->events.push_back(e);
-    }
-    return hipEventRecord(c->events[ev++], st);
-  };
+  Timer tm{c, st, c->tuning.time_kernels != 0};
 
   HIP_TRY(c, launch_clear_u32(reinterpret_cast<uint32_t *>(q.stats), kStatWords * 2, st));
-  HIP_TRY(c, launch_aov_init(npix, q, st));
-  for (uint32_t s0 = 0; s0 < spp; s0 += spp_chunk) {
-    StreamParams w{};
-    w.sp = sp;
-    const uint32_t nspp = std::min(spp_chunk, spp - s0);
-    w.total = npix * nspp;
-    w.capacity = std::min<uint32_t>(cap, w.total);
-    w.sample_base = p->sample_offset + s0;
-    w.chunk_s0 = s0;
-    w.regen_when_empty = std::getenv("PHX_REGEN_WHEN_EMPTY") ? 1u : 0u;
-    HIP_TRY(c, launch_reset(q, st));
-    HIP_TRY(c, launch_regen(c->dev, w, q, 0, st));
+  HIP_TRY(c, launch_aov_init(total_pixels, aov, st));
+  std::vector<RunItem> items(pad);
+  std::vector<uint32_t> first(pad);
+  for (const Run &run : runs) {
+    if (run.items.empty()) continue;
+    ViewArgs va;
+    std::memset(&va, 0, sizeof va);
+    if (run.items.size() == 1) {
+      va.kind = 0;
+      va.one.scene = scenes[run.items[0].r.scene]->dev;
+      va.one.item = run.items[0].r;
+    } else {
+      for (size_t k = 0; k < pad; ++k) {
+        items[k] = k < run.items.size() ? run.items[k].r : run.items.back().r;
+        first[k] = k < run.items.size() ? run.items[k].r.first : 0xffffffffu;
+      }
+      // the previous run's kernels must be done reading the tables
+      HIP_TRY(c, hipStreamSynchronize(st));
+      HIP_TRY(c, hipMemcpy(c->bt_items.p, items.data(), pad * sizeof(RunItem), hipMemcpyHostToDevice));
+      HIP_TRY(c, hipMemcpy(c->bt_first.p, first.data(), pad * sizeof(uint32_t), hipMemcpyHostToDevice));
+      if (one_scene) {
+        va.kind = 1;
+        va.shared.scene = scenes[0]->dev;
+        va.shared.items = c->bt_items.as<RunItem>();
+        va.shared.first = c->bt_first.as<uint32_t>();
+        va.shared.count = uint32_t(pad);
+      } else {
+        va.kind = 2;
+        va.many.scenes = c->bt_scenes.as<DevScene>();
+        va.many.items = c->bt_items.as<RunItem>();
+        va.many.first = c->bt_first.as<uint32_t>();
+        va.many.count = uint32_t(pad);
+      }
+    }
+    const bool graph = use_graph && va.kind != 0 && !trace_counts && !tm.on;
+    if (graph && (r = ensure_graph(c, va, q, ls))) return r;
+    HIP_TRY(c, launch_run_start(q, run.total, std::min(cap, run.total), st));
+    HIP_TRY(c, launch_regen(va, q, 0, ls, st));
     uint32_t it = 0;
     while (true) {
-      for (uint32_t j = 0; j < batch; ++j, ++it) {
-        HIP_TRY(c, mark());
-        HIP_TRY(c, launch_extend(c->dev, q, it, c->trace_grid, st));
-        HIP_TRY(c, mark());
-        HIP_TRY(c, mark());
-        HIP_TRY(c, launch_shade(c->dev, w, q, it, st));
-        HIP_TRY(c, mark());
-        HIP_TRY(c, mark());
-        HIP_TRY(c, launch_shadow(c->dev, w, q, it, c->trace_grid, st));
-        HIP_TRY(c, mark());
-        HIP_TRY(c, launch_regen(c->dev, w, q, it + 1, st));
+      if (graph) {
+        HIP_TRY(c, hipGraphLaunch(c->graph, st));
+        it += kIterBatch;
+      } else {
+        HIP_TRY(c, launch_iterations(va, q, ls, batch, tm.on ? &tm : nullptr, st));
+        it += batch;
       }
-      HIP_TRY(c, hipMemcpyAsync(&c->pinned[0], q.counters + kRayCount + ring(it), 4, hipMemcpyDeviceToHost, st));
+      // the iteration counter, the samples started, then that iteration's queue
+      HIP_TRY(c, hipMemcpyAsync(&c->pinned[0], q.counters + kIter, 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(c, hipMemcpyAsync(&c->pinned[1], q.counters + kNextSample, 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(c, hipMemcpyAsync(&c->pinned[2], q.counters + kRayCount + ring(it), 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(c, hipStreamSynchronize(st));
+      if (c->pinned[0] != it)
+        return fail(c, PHX_ERR_OVERFLOW, "render: wavefront iteration counter out of step");
       if (trace_counts)
-        std::fprintf(stderr, "[phx] run %u iteration %u: queue %u, started %u of %u samples\n", s0, it, c->pinned[0],
-                     c->pinned[1], w.total);
-      if (c->pinned[0] == 0 && c->pinned[1] >= w.total) break;
+        std::fprintf(stderr, "[phx] iteration %u: queue %u, started %u of %u samples\n", it, c->pinned[2],
+                     c->pinned[1], run.total);
+      if (c->pinned[2] == 0 && c->pinned[1] >= run.total) break;
       if (it >= kIterationGuard)
         return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");
     }
-    HIP_TRY(c, launch_reduce(w, q, fb, s0 == 0, st));
+    for (const HostItem &h : run.items)
+      HIP_TRY(c, launch_reduce(h.r.sp.npix, h.nspp, q.sample_L + h.r.first, fb.accum + h.r.pix_base, h.r.s0 == 0, st));
   }
-  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
-  HIP_TRY(c, launch_resolve(c->dev, sp, q, fb, spp, c->fetch.as<uint32_t>(), c->trace_grid, st));
+  for (const ViewJob &j : views) {
+    StreamQueues qv = q;
+    qv.aov_s = aov + j.pix_base;
+    HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
+    HIP_TRY(c, launch_resolve(j.scene->dev, j.sp, qv, fb.at(j.pix_base), j.spp, c->fetch.as<uint32_t>(),
+                              c->trace_grid, st));
+  }
   unsigned long long hs[kStatWords];
   HIP_TRY(c, hipMemcpyAsync(hs, q.stats, sizeof hs, hipMemcpyDeviceToHost, st));
   HIP_TRY(c, hipStreamSynchronize(st));
-  const auto t1 = std::chrono::steady_clock::now();
   c->stats.closest_rays = hs[kStatClosest];
   c->stats.shadow_rays = hs[kStatShadow];
   c->stats.shade_events = hs[kStatShade];
-  c->stats.samples = uint64_t(npix) * spp;
-  c->stats.pixels = npix;
-  c->stats.last_render_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  uint64_t samples = 0;
+  for (const ViewJob &j : views) samples += uint64_t(j.sp.npix) * j.spp;
+  c->stats.samples = samples;
+  c->stats.pixels = total_pixels;
   c->stats.extend_launches = c->stats.shade_launches = c->stats.shadow_launches = 0;
   c->stats.extend_ms = c->stats.shade_ms = c->stats.shadow_ms = 0.0;
-  for (size_t e = 0; e + 5 < ev; e += 6) {
+  for (size_t e = 0; e + 5 < tm.n; e += 6) {
     float a = 0.f, b = 0.f, d = 0.f;
     HIP_TRY(c, hipEventElapsedTime(&a, c->events[e], c->events[e + 1]));
     HIP_TRY(c, hipEventElapsedTime(&b, c->events[e + 2], c->events[e + 3]));
@@ -211,6 +388,157 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
     c->stats.shade_launches++;
     c->stats.shadow_launches++;
   }
+  return PHX_OK;
+}
+
+int reserve_frame(phx_context *c, DBuf &accum, DBuf &color, DBuf &alpha, DBuf &depth, DBuf &normal, DBuf &albedo,
+                  size_t npix) {
+  HIP_TRY(c, accum.reserve(npix * 16));
+  HIP_TRY(c, color.reserve(npix * 12));
+  HIP_TRY(c, alpha.reserve(npix * 4));
+  HIP_TRY(c, depth.reserve(npix * 4));
+  HIP_TRY(c, normal.reserve(npix * 12));
+  HIP_TRY(c, albedo.reserve(npix * 12));
+  return PHX_OK;
+}
+
+}  // namespace
+
+extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_render_params *p) {
+  if (!c || !cam || !p) return PHX_ERR_INVALID;
+  if (!c->have_scene) return fail(c, PHX_ERR_NO_SCENE, "render before set_scene");
+  int r = check_params(c, p, "render");
+  if (r) return r;
+  HIP_TRY(c, hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t npix = p->width * p->height;
+  if (c->f_w != p->width || c->f_h != p->height || !c->f_accum.p) {
+    if ((r = reserve_frame(c, c->f_accum, c->f_color, c->f_alpha, c->f_depth, c->f_normal, c->f_albedo, npix)))
+      return r;
+    c->f_w = p->width;
+    c->f_h = p->height;
+  }
+  HIP_TRY(c, c->aov_s.reserve(size_t(npix) * 4));
+  c->have_result = false;
+  const std::vector<ViewJob> views{make_job(c, 0, *cam, *p, 0)};
+  const FrameBuffers fb{c->f_accum.as<float4>(), c->f_color.as<float>(), c->f_alpha.as<float>(),
+                        c->f_depth.as<float>(), c->f_normal.as<float>(), c->f_albedo.as<float>()};
+  if ((r = stream_views(c, views, {c}, fb, c->aov_s.as<uint32_t>(), npix, false))) return r;
+  c->stats.last_render_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->have_result = true;
+  return PHX_OK;
+}
+
+extern "C" int phx_render_batch(phx_context *c, const phx_view *views, uint32_t count) {
+  if (!c || (count && !views)) return PHX_ERR_INVALID;
+  if (count == 0) return fail(c, PHX_ERR_INVALID, "render_batch: no views");
+  int r;
+  std::vector<const phx_context *> scenes;
+  std::vector<ViewJob> jobs;
+  uint64_t pix = 0;
+  for (uint32_t v = 0; v < count; ++v) {
+    const phx_view &w = views[v];
+    if (!w.scene) return fail(c, PHX_ERR_INVALID, "render_batch: view without a scene");
+    if (!w.scene->have_scene) return fail(c, PHX_ERR_NO_SCENE, "render_batch: a view's scene is not set");
+    if (w.scene->device != c->device) return fail(c, PHX_ERR_INVALID, "render_batch: scene on another device");
+    if ((r = check_params(c, &w.params, "render_batch"))) return r;
+    uint32_t si = 0;
+    while (si < scenes.size() && scenes[si] != w.scene) ++si;
+    if (si == scenes.size()) scenes.push_back(w.scene);
+    jobs.push_back(make_job(w.scene, si, w.camera, w.params, uint32_t(pix)));
+    pix += uint64_t(w.params.width) * w.params.height;
+    if (pix > (uint64_t(1) << 28)) return fail(c, PHX_ERR_INVALID, "render_batch: more than 2^28 pixels in all");
+  }
+  HIP_TRY(c, hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  c->bt_have_result = false;
+  if ((r = reserve_frame(c, c->bt_accum, c->bt_color, c->bt_alpha, c->bt_depth, c->bt_normal, c->bt_albedo, pix)))
+    return r;
+  HIP_TRY(c, c->bt_aov.reserve(pix * 4));
+  const FrameBuffers fb{c->bt_accum.as<float4>(), c->bt_color.as<float>(), c->bt_alpha.as<float>(),
+                        c->bt_depth.as<float>(), c->bt_normal.as<float>(), c->bt_albedo.as<float>()};
+  if ((r = stream_views(c, jobs, scenes, fb, c->bt_aov.as<uint32_t>(), uint32_t(pix), c->tuning.use_graph != 0)))
+    return r;
+  c->bt_views.clear();
+  for (uint32_t v = 0; v < count; ++v)
+    c->bt_views.push_back({views[v].params.width, views[v].params.height, jobs[v].pix_base});
+  c->stats.last_render_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  c->bt_have_result = true;
+  return PHX_OK;
+}
+
+extern "C" int phx_read_batch_result(phx_context *c, uint32_t view, float *color, float *alpha, float *depth,
+                                     float *normal, float *albedo) {
+  if (!c) return PHX_ERR_INVALID;
+  if (!c->bt_have_result) return fail(c, PHX_ERR_NO_FRAME, "read_batch_result before render_batch");
+  if (view >= c->bt_views.size()) return fail(c, PHX_ERR_INVALID, "read_batch_result: no such view");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const auto &v = c->bt_views[view];
+  const size_t n = size_t(v.width) * v.height, b = v.pix_base;
+  if (color) HIP_TRY(c, hipMemcpy(color, c->bt_color.as<float>() + b * 3, n * 12, hipMemcpyDeviceToHost));
+  if (alpha) HIP_TRY(c, hipMemcpy(alpha, c->bt_alpha.as<float>() + b, n * 4, hipMemcpyDeviceToHost));
+  if (depth) HIP_TRY(c, hipMemcpy(depth, c->bt_depth.as<float>() + b, n * 4, hipMemcpyDeviceToHost));
+  if (normal) HIP_TRY(c, hipMemcpy(normal, c->bt_normal.as<float>() + b * 3, n * 12, hipMemcpyDeviceToHost));
+  if (albedo) HIP_TRY(c, hipMemcpy(albedo, c->bt_albedo.as<float>() + b * 3, n * 12, hipMemcpyDeviceToHost));
+  return PHX_OK;
+}
+
+extern "C" int phx_autotune(phx_context *c, const phx_view *views, uint32_t count, uint32_t repeats, phx_tuning *best,
+                            double *best_ms) {
+  if (!c) return PHX_ERR_INVALID;
+  repeats = std::max<uint32_t>(repeats, 1);
+  const phx_tuning start = c->tuning;
+  auto time_it = [&](const phx_tuning &t, double &ms) -> int {
+    int r = phx_set_tuning(c, &t);
+    if (r) return r;
+    if ((r = phx_render_batch(c, views, count))) return r;  // warm-up: allocations, graph capture
+    ms = 1e300;
+    for (uint32_t i = 0; i < repeats; ++i) {
+      if ((r = phx_render_batch(c, views, count))) return r;
+      ms = std::min(ms, c->stats.last_render_ms);
+    }
+    return PHX_OK;
+  };
+  phx_tuning cur = start;
+  cur.time_kernels = 0;
+  if (!cur.trace_block) cur.trace_block = 256;
+  if (!cur.trace_groups_per_cu) cur.trace_groups_per_cu = 8;
+  if (!cur.queue_chunk) cur.queue_chunk = kChunk;
+  if (!cur.paths_in_flight) cur.paths_in_flight = kDefaultPaths;
+  double cur_ms;
+  int r = time_it(cur, cur_ms);
+  if (r) {
+    (void)phx_set_tuning(c, &start);
+    return r;
+  }
+  // one pass of coordinate descent over the launch-shape knobs
+  const std::vector<std::pair<uint32_t phx_tuning::*, std::vector<uint32_t>>> knobs = {
+      {&phx_tuning::trace_block, {128, 256}},
+      {&phx_tuning::trace_groups_per_cu, {4, 8, 16, 32}},
+      {&phx_tuning::queue_chunk, {64, 128, 256, 512, 1024}},
+      {&phx_tuning::paths_in_flight, {1u << 23, 1u << 24, 1u << 25}},
+  };
+  for (const auto &k : knobs) {
+    const uint32_t was = cur.*(k.first);
+    for (uint32_t val : k.second) {
+      if (val == was) continue;
+      phx_tuning t = cur;
+      t.*(k.first) = val;
+      double ms;
+      if ((r = time_it(t, ms))) {
+        (void)phx_set_tuning(c, &start);
+        return r;
+      }
+      if (ms < cur_ms) {
+        cur = t;
+        cur_ms = ms;
+      }
+    }
+  }
+  if ((r = phx_set_tuning(c, &cur))) return r;
+  if (best) *best = cur;
+  if (best_ms) *best_ms = cur_ms;
   return PHX_OK;
 }

@@ -17,9 +17,9 @@ __global__ __launch_bounds__(256) void k_trace(DevScene s, const float *org, con
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], 256, spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   const uint32_t lane = lane_id();
-  WorkCursor cur = first_chunk();
+  WorkCursor cur = first_chunk(kChunk);
   while (true) {
-    const uint32_t base = next_batch(cur, fetch, n);
+    const uint32_t base = next_batch(cur, fetch, n, kChunk);
     if (base >= n) break;
     const uint32_t i = base + lane;
     if (i >= n) continue;

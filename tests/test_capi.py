@@ -46,7 +46,7 @@ def test_struct_layouts_match_header():
 
 def test_backend_name_and_abi_version():
     assert lib().phx_backend_name() == b"hip"
-    assert lib().phx_abi_version() == 1
+    assert lib().phx_abi_version() == 2
 
 
 def test_errors_without_gpu_or_arguments():
@@ -113,3 +113,28 @@ def test_scene_generators():
     g = scenes.sphere_grid(level=2, grid=2, env_size=(32, 16))
     assert g.triangle_count == 4 * 320 + 2 + 8
     assert g.env_pixels.shape == (16, 32, 3)
+
+
+def test_ctypes_layouts_equal_the_c_header(tmp_path):
+    # sizeof / offsetof from the C compiler against the ctypes mirror
+    structs = {"phx_material": abi.Material, "phx_light": abi.Light, "phx_texture_info": abi.TextureInfo,
+               "phx_volume": abi.Volume, "phx_scene_desc": abi.SceneDesc, "phx_camera": abi.Camera,
+               "phx_render_params": abi.RenderParams, "phx_hit": abi.Hit, "phx_hit_compact": abi.HitCompact,
+               "phx_tuning": abi.Tuning, "phx_stats": abi.Stats, "phx_view": abi.View,
+               "phx_camera_data": abi.CameraData, "phx_render_settings": abi.RenderSettings}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "phx.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("  return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(l.rsplit(" ", 1) for l in out if l)
+    for cname, cls in structs.items():
+        assert int(got[cname]) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(cls, fname).offset, f"{cname}.{fname}"

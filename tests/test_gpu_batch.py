@@ -1,0 +1,133 @@
+"""phx_render_batch: many views of several scenes in one wavefront.  Each view's
+result must equal phx_render of that view bit for bit (same sample streams,
+same summation order), with or without the captured graph, however the views
+are split into runs, and whatever the launch shape."""
+import os
+
+import numpy as np
+import pytest
+
+from photon_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _tracer(ctx_factory, s):
+    from photon_amd.backend import HipBackend, PathTracer
+    pt = PathTracer(HipBackend(context=ctx_factory()))
+    pt.set_scene(s)
+    return pt
+
+
+def _view(pt, s, w, h, spp, depth, offset=0, orbit=0.0):
+    from photon_amd.backend import make_camera
+    eye = np.asarray(s.camera["look_from"], np.float64)
+    at = np.asarray(s.camera["look_at"], np.float64)
+    d = eye - at
+    c, sn = np.cos(orbit), np.sin(orbit)
+    eye = at + np.array([c * d[0] + sn * d[2], d[1], -sn * d[0] + c * d[2]])
+    cam = make_camera(eye, at, s.camera["vup"], s.camera["vfov"], w / h)
+    p = abi.default_render_params(w, h, spp, depth)
+    p.sample_offset = offset
+    return (pt, cam, p)
+
+
+@pytest.fixture(scope="module")
+def batch_setup():
+    from photon_amd.backend import Context
+    ctxs = []
+
+    def make():
+        ctxs.append(Context(0))
+        return ctxs[-1]
+    a = scenes.cornell_box()
+    b = scenes.cornell_box(wall_colors=((0.2, 0.6, 0.7), (0.7, 0.6, 0.1), (0.5, 0.5, 0.5)))
+    c = scenes.sphere_grid(seed=7, grid=2, level=2, env_size=(64, 32))
+    pa, pb, pc = _tracer(make, a), _tracer(make, b), _tracer(make, c)
+    views = [
+        _view(pa, a, 32, 24, 4, 8),
+        _view(pc, c, 40, 40, 3, 6, offset=5, orbit=0.7),
+        _view(pb, b, 16, 16, 8, 8),
+        _view(pc, c, 24, 8, 1, 8, orbit=-1.1),
+        _view(pa, a, 8, 8, 16, 4, offset=3, orbit=0.3),
+        _view(pb, b, 33, 17, 2, 8, orbit=-0.4),
+    ]
+    plain = []
+    for pt, cam, p in views:
+        pt.camera = cam
+        pt.params = p
+        plain.append(pt.render())
+    yield views, plain, make
+
+
+def _check_equal(res, ref, tag):
+    for f in ("color", "alpha", "depth", "normal", "albedo"):
+        np.testing.assert_array_equal(getattr(res, f), getattr(ref, f), err_msg=f"{tag}: {f}")
+
+
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_batch_equals_plain_renders(batch_setup, use_graph):
+    from photon_amd.backend import BatchRenderer
+    views, plain, make = batch_setup
+    br = BatchRenderer(make())
+    br.ctx.set_tuning(use_graph=use_graph)
+    for rep in range(2):  # the second call replays the captured graph
+        br.render(views)
+        for v in range(len(views)):
+            _check_equal(br.read(v), plain[v], f"graph={use_graph} rep={rep} view {v}")
+    st = br.ctx.stats()
+    assert st.samples == sum(p.width * p.height * p.samples_per_pixel for _, _, p in views)
+
+
+def test_batch_split_into_runs(batch_setup, monkeypatch):
+    # a small run budget splits views across runs and spp ranges
+    from photon_amd.backend import BatchRenderer
+    views, plain, make = batch_setup
+    monkeypatch.setenv("PHX_SAMPLE_BUDGET", "3000")
+    br = BatchRenderer(make())
+    br.render(views)
+    for v in range(len(views)):
+        _check_equal(br.read(v), plain[v], f"split view {v}")
+    # the plain path splits the same way and still gives the same frame
+    pt, cam, p = views[1]
+    pt.camera, pt.params = cam, p
+    _check_equal(pt.render(), plain[1], "plain split")
+
+
+def test_launch_shape_does_not_change_results(batch_setup):
+    views, plain, make = batch_setup
+    pt, cam, p = views[1]
+    pt.camera, pt.params = cam, p
+    for kw in (dict(trace_block=128), dict(queue_chunk=64, trace_groups_per_cu=3), dict(paths_in_flight=777),
+               dict(trace_block=256, queue_chunk=1024, trace_groups_per_cu=16, paths_in_flight=0)):
+        pt.ctx.set_tuning(**kw)
+        _check_equal(pt.render(), plain[1], f"tuning {kw}")
+    pt.ctx.set_tuning(trace_block=0, queue_chunk=0, trace_groups_per_cu=0, paths_in_flight=0)
+
+
+def test_autotune_keeps_a_valid_shape(batch_setup):
+    from photon_amd.backend import BatchRenderer
+    views, plain, make = batch_setup
+    br = BatchRenderer(make())
+    t, ms = br.autotune(views[:3], repeats=1)
+    assert ms > 0
+    assert t.trace_block in (128, 256) and t.queue_chunk % 64 == 0
+    br.render(views)
+    for v in range(len(views)):
+        _check_equal(br.read(v), plain[v], f"after autotune view {v}")
+
+
+def test_batch_errors(batch_setup):
+    from photon_amd._lib import PhxError
+    from photon_amd.backend import BatchRenderer, Context
+    views, _, make = batch_setup
+    br = BatchRenderer(make())
+    with pytest.raises(PhxError):
+        br.render([])
+    empty = Context(0)  # no scene set
+    with pytest.raises(PhxError):
+        br.render([(empty, views[0][1], views[0][2])])
+    bad = abi.default_render_params(8, 8, 1, 4)
+    bad.adaptive_sampling = 1
+    with pytest.raises(PhxError):
+        br.render([(views[0][0], views[0][1], bad)])
