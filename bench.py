@@ -15,6 +15,10 @@ in HBM before the timed region.  Prints one JSON line.
   gpu_baseline  the same frame rendered with the reference's algorithmic
             structure on this GPU (phx_render_reference_structure), and the
             speedup of the wavefront backend over it
+  batched   16 views of uneven size in one phx_render_batch against the same
+            16 views as separate renders
+  adjoint   path-replay adjoint time over primal time on the differentiable
+            workload
 """
 import argparse
 import json
@@ -100,6 +104,47 @@ def batched_form(pt_c1, c1, args):
             "equal_to_plain": same}
 
 
+def adjoint_form(args):
+    """Path-replay adjoint against its primal on the differentiable workload:
+    Cornell box with a textured floor (512^2 RGB albedo, initialised to 0.5)
+    and 32 trainable Disney materials; target from hidden seeded parameters and
+    a 3-octave coloured-noise texture; L2 loss at 256^2, 32 primal spp, 32
+    decorrelated adjoint spp, max depth 6.  Times are the library's own (the
+    call, synchronised)."""
+    import torch
+
+    from photon_amd import scenes
+    from photon_amd.backend import make_camera
+    from photon_amd.diff import DiffRenderer
+
+    tex_target = scenes.colored_noise_texture(512, 99)
+    target_scene = scenes.textured_cornell(n_materials=32, tex_size=512, texture=tex_target, seed=77)
+    scene = scenes.textured_cornell(n_materials=32, tex_size=512, seed=5)
+    cam = make_camera(scene.camera["look_from"], scene.camera["look_at"], scene.camera["vup"], scene.camera["vfov"],
+                      1.0)
+    tgt = DiffRenderer(target_scene, cam, 256, 256, spp=32, max_depth=6, n_train=32)
+    target = tgt.render(*tgt.initial_params(target_scene)).detach()
+    dr = DiffRenderer(scene, cam, 256, 256, spp=32, max_depth=6, adjoint_spp=32, n_train=32)
+    params, tex = dr.initial_params(scene)
+    params.requires_grad_(True)
+    tex.requires_grad_(True)
+    primal, adjoint = [], []
+    for i in range(1 + args.adjoint_reps):
+        params.grad = None
+        tex.grad = None
+        loss = ((dr.render(params, tex) - target) ** 2).mean()
+        loss.backward()
+        torch.cuda.synchronize()
+        if i > 0:  # the first pass warms up allocations
+            primal.append(dr.primal_ms)
+            adjoint.append(dr.adjoint_ms)
+    p_ms, a_ms = min(primal), min(adjoint)
+    print(f"[bench] adjoint: primal {p_ms:.2f} ms, adjoint {a_ms:.2f} ms", file=sys.stderr, flush=True)
+    return {"workload": "cornell + 512^2 textured floor, 32 Disney materials, 256x256, 32+32 spp, max depth 6",
+            "primal_ms": p_ms, "adjoint_ms": a_ms, "ratio": a_ms / p_ms, "loss": float(loss.item()),
+            "params": int(params.numel() + tex.numel())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +161,7 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=3,
                     help="timed repetitions of the batched 16-view launch and of its 16 plain launches (0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="batched form with the default launch shape")
+    ap.add_argument("--adjoint-reps", type=int, default=3, help="timed primal+adjoint passes (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -229,6 +275,7 @@ def main():
                          f"{secs:.1f} s)"}
 
     batched = batched_form(pt, scene, args) if args.batch_reps > 0 else None
+    adjoint = adjoint_form(args) if args.adjoint_reps > 0 else None
 
     line = {
         "metric": "samples/s & Mrays/s at 1024^2 64spp (plain)",
@@ -255,6 +302,7 @@ def main():
         "cpu_baseline": cpu,
         "gpu_baseline": gpu_base,
         "batched": batched,
+        "adjoint": adjoint,
         "bvh_build_s": build_s,
     }
     print(json.dumps(line))

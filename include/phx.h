@@ -322,7 +322,8 @@ int phx_synchronize(phx_context *ctx);
  * phx_set_scene replaces PathTracer::set_scene + build_accel
  * (pathtracer.cpp:393-503, RenderBridge.cpp:822-825).  phx_render replaces
  * PathTracer::render (pathtracer.cpp:564-1159); its result is read with
- * phx_read_result (RenderResult, pathtracer.h:57-63; rows as rendered). */
+ * phx_read_result (RenderResult, pathtracer.h:57-63; rows as rendered) into
+ * host or device memory (NULL skips an output). */
 int phx_set_scene(phx_context *ctx, const phx_scene_desc *scene);
 int phx_render(phx_context *ctx, const phx_camera *camera, const phx_render_params *params);
 int phx_read_result(phx_context *ctx, float *color_rgb, float *alpha, float *depth, float *normal_xyz,
@@ -355,6 +356,31 @@ int phx_read_batch_result(phx_context *ctx, uint32_t view, float *color_rgb, flo
 int phx_autotune(phx_context *ctx, const phx_view *views, uint32_t count, uint32_t repeats, phx_tuning *best,
                  double *best_ms);
 
+/* ---- differentiable form: path-replay adjoint ------------------------------
+ * No reference counterpart (the reference renders only).  Adds to
+ * d_grad_materials[m * 5 + k] (k = base colour r, g, b, roughness, metallic of
+ * material m; material_count * 5 floats) and, when d_grad_texture is not NULL,
+ * to d_grad_texture[p * 3 + c] (texture-atlas texel p) the gradient of a
+ * loss of the frame (camera, params) given d_adjoint = dLoss/dImage (RGB per
+ * pixel, the layout of phx_read_result's color; all device memory).  The frame
+ * is rendered with its own sample streams (params->sample_offset picks them;
+ * use samples disjoint from the primal's for a decorrelated gradient) and
+ * every path is replayed from its seed: sampling decisions are held fixed and
+ * derivatives flow through the BSDF weights of next-event estimation and of
+ * the continuing BSDF sample (delta and transmission lobes carry none).
+ * Memory beyond the render's is one radiance value per sample in flight.
+ * Roughness and metallic gradients are zero outside (0, 1), where the
+ * material resolve clamps them; a texture-mapped base colour sends its
+ * gradient to the texels.  Clamps (throughput, direct/indirect) are not
+ * differentiated. */
+int phx_adjoint(phx_context *ctx, const phx_camera *camera, const phx_render_params *params,
+                const float *d_adjoint, float *d_grad_materials, float *d_grad_texture);
+/* Writes base colour, roughness and metallic of materials 0..count-1 from a
+ * device array of count * 5 floats (the gradient layout of phx_adjoint). */
+int phx_set_material_params(phx_context *ctx, const float *d_params, uint32_t count);
+/* Overwrites texture-atlas texels [first, first + count) from device RGB floats. */
+int phx_set_texture_pixels(phx_context *ctx, const float *d_pixels, uint32_t first, uint32_t count);
+
 /* Baseline: the same frame with the reference's algorithmic structure on this
  * GPU, standing in for the reference's Kokkos-HIP build (which needs Kokkos):
  * binary median-split BVH with 44-byte nodes (bvh.cpp:20-125), 64-entry
@@ -376,6 +402,13 @@ int phx_bsdf_eval(phx_context *ctx, const phx_material *d_materials, const float
 int phx_bsdf_sample(phx_context *ctx, const phx_material *d_materials, const float *d_wo, const float *d_ng,
                     const float *d_sn, const uint32_t *d_seeds, uint32_t count, float *d_wi, float *d_f,
                     float *d_pdf, uint32_t *d_flags, void *stream);
+
+/* Derivatives of phx_bsdf_eval's f*cos with respect to base colour r, g, b,
+ * roughness and metallic (the adjoint's parameters): df[i*15 + k*3 + c] =
+ * d f_c / d param_k of tuple i; f as phx_bsdf_eval.  No reference
+ * counterpart; a check of the path-replay adjoint's derivative code. */
+int phx_bsdf_eval_grad(phx_context *ctx, const phx_material *d_materials, const float *d_wo, const float *d_wi,
+                       const float *d_ng, const float *d_sn, uint32_t count, float *d_f, float *d_df, void *stream);
 
 /* Camera::make_perspective / make_orthographic (camera.h:32-77). */
 int phx_camera_perspective(phx_camera *out, const float look_from[3], const float look_at[3],

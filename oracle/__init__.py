@@ -57,6 +57,9 @@ def lib():
         L.orc_volume_transmittance.argtypes = [P, P, P, C.c_float, C.c_float, P, C.c_uint32, P]
         L.orc_volume_scatter.argtypes = [P, P, P, C.c_float, C.c_float, P, C.c_uint32, P]
         L.orc_render.restype = C.c_double
+        L.orc_bsdf_eval_grad_fd.argtypes = [P, P, P, P, P, C.c_uint32, C.c_double, P]
+        L.orc_render_detached.restype = C.c_double
+        L.orc_render_detached.argtypes = [P, P, P, P, C.c_uint32, P, P, P, C.c_int]
         L.orc_render.argtypes = [P, P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_uint32, C.c_uint32]
         _lib = L
     return _lib
@@ -175,6 +178,37 @@ class OracleScene:
                                 _p(extras.get("albedo")), int(threads), int(use_double), int(pixel_range[0]),
                                 int(pixel_range[1]))
         return color, extras, secs
+
+
+def _render_detached(self, camera, params, mat_params=None, texels=None, delta=None, image=False, threads=0):
+    """The detached estimator of the path-replay adjoint in double (see
+    detached_pixel in oracle.cpp): float sampling decisions, double weights with
+    `mat_params` (materials x 5: base r, g, b, roughness, metallic) and atlas
+    `texels` (pixels x 3) replacing the scene's.  Returns (sum delta * image or
+    None, image (h, w, 3) float64 or None)."""
+    h, w = params.height, params.width
+    mp = None if mat_params is None else np.ascontiguousarray(mat_params, np.float64).reshape(-1)
+    tx = None if texels is None else np.ascontiguousarray(texels, np.float64).reshape(-1)
+    dl = None if delta is None else np.ascontiguousarray(delta, np.float32).reshape(-1)
+    img = np.zeros((h, w, 3), np.float64) if image else None
+    total = lib().orc_render_detached(self.h, C.addressof(camera), C.addressof(params), _p(mp),
+                                      0 if mp is None else len(mp) // 5, _p(tx), _p(dl), _p(img), int(threads))
+    return (total if delta is not None else None), img
+
+
+OracleScene.render_detached = _render_detached
+
+
+def bsdf_eval_grad_fd(mats, wo, wi, ng, sn, h=1e-4):
+    """Central differences in double of the BSDF eval with respect to base r, g,
+    b, roughness, metallic: (n, 5, 3)."""
+    from photon_amd import abi
+    n = len(mats)
+    arr = (abi.Material * n)(*mats)
+    wo, wi, ng, sn = (_c(x, np.float32) for x in (wo, wi, ng, sn))
+    df = np.zeros((n, 5, 3), np.float64)
+    lib().orc_bsdf_eval_grad_fd(C.addressof(arr), _p(wo), _p(wi), _p(ng), _p(sn), n, h, _p(df))
+    return df
 
 
 def bsdf_eval(mats, wo, wi, ng, sn, use_double=False):

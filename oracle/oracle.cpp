@@ -784,6 +784,288 @@ static void render_pixel(const Scene &s, const Params &P, uint32_t idx, PixelOut
   }
 }
 
+// ---- detached finite-difference engine ----------------------------------------
+// The estimator the path-replay adjoint differentiates: every sampling
+// decision is the float integrator's (render_pixel<float>, the scene's
+// materials), while the weights that depend on the trainable parameters are
+// evaluated in double with `eval` materials and texels: the BSDF f * cos at
+// next-event estimation and at the continuing sample (f / pdf with the float
+// pdf), times the parameter-free factors (emission, MIS weights, light pdfs,
+// Russian-roulette survival, volume albedo and transmittance).  Delta and
+// transmission samples keep their float weight.  Clamps are not applied.
+// Central differences of this estimator in double are the reference the
+// adjoint's gradients are checked against.
+struct DetachedEval {
+  const double *mat_params;  // material_count * 5 (base r, g, b, roughness, metallic) or null
+  uint32_t mat_count;
+  const double *texels;      // atlas pixels * 3 or null (the scene's)
+};
+static Mat<double> detached_material(const Scene &s, const DetachedEval &e, uint32_t mid, const V2<float> &uv,
+                                     bool has_color, const V3<float> &color) {
+  Mat<double> m = s.mats.empty() ? Mat<double>{} : mat_from<double>(s.mats[mid]);
+  if (e.mat_params && mid < e.mat_count) {
+    const double *q = e.mat_params + size_t(mid) * 5;
+    m.base_color = {q[0], q[1], q[2]};
+    m.roughness = q[3];
+    m.metallic = q[4];
+  }
+  const bool tex = !s.atlas.infos.empty();
+  const V2<double> uvd{double(uv.x), double(uv.y)};
+  if (m.base_color_tex >= 0 && tex)
+    m.base_color = e.texels ? s.atlas.sample_from<double>(e.texels, m.base_color_tex, uvd.x, uvd.y)
+                            : s.atlas.sample<double>(m.base_color_tex, uvd.x, uvd.y);
+  else if (has_color)
+    m.base_color = V3<double>(color);
+  if (m.roughness_tex >= 0 && tex) m.roughness = s.atlas.sample<double>(m.roughness_tex, uvd.x, uvd.y).x;
+  if (m.metallic_tex >= 0 && tex) m.metallic = s.atlas.sample<double>(m.metallic_tex, uvd.x, uvd.y).x;
+  auto c01 = [](double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); };
+  m.metallic = c01(m.metallic);
+  m.roughness = c01(m.roughness);
+  m.transmission = c01(m.transmission);
+  return m;
+}
+
+static V3<double> detached_pixel(const Scene &s, const Params &P, uint32_t idx, const DetachedEval &E) {
+  const phx_render_params &rp = P.p;
+  const uint32_t w = rp.width, h = rp.height;
+  const uint32_t max_depth = rp.max_depth > 0 ? rp.max_depth : 1u;
+  const uint32_t strategies = uint32_t(s.nee_lights.size()) + (s.emissive_count > 0 ? 1u : 0u) + (s.has_env ? 1u : 0u);
+  const uint32_t max_it = max_depth + 1u + 8u;
+  const bool has_vol = !s.vol.empty();
+  const float eps = s.ray_eps;
+  const V3<float> ambient = v3<float>(rp.ambient_color) * rp.ambient_radiance;
+  const V3<float> bg = v3<float>(rp.background_color);
+  const uint32_t x = idx % w, y = idx / w;
+  V3<double> acc{};
+  auto wv = [](const V3<float> &v) { return V3<double>(v); };
+  for (uint32_t sidx = 0; sidx < rp.samples_per_pixel; ++sidx) {
+    const uint32_t sample = sidx + rp.sample_offset;
+    Rng crng = make_rng(idx, sample, 0u);
+    float u = (float(x) + draw<float>(crng)) / float(w);
+    float v = (float(y) + draw<float>(crng)) / float(h);
+    if (!rp.bottom_up) v = 1.f - v;
+    u = rp.image_region[0] + (rp.image_region[2] - rp.image_region[0]) * u;
+    v = rp.image_region[1] + (rp.image_region[3] - rp.image_region[1]) * v;
+    V3<float> ro, rd;
+    camera_ray<float>(P.cam, u, v, crng, ro, rd);
+    float rtmin, rtmax;
+    clip_range<float>(P.cam, rd, rtmin, rtmax);
+    V3<float> tp{1.f, 1.f, 1.f};
+    V3<double> tw{1.0, 1.0, 1.0};  // the parameter-dependent throughput
+    bool active = true, primary = true;
+    float prev_pdf = -1.f, path_dist = 0.f;
+    uint32_t depth = 0, kinds = 0;
+    for (uint32_t it = 0; it < max_it && active; ++it) {
+      const Hit<float> hr = closest<float>(s, ro, rd, rtmin, rtmax);
+      Rng rng = make_rng(idx, sample, 1u + it);
+      const bool emission_only = (kinds & PATH_EMISSION_ONLY) != 0u;
+      bool nee = false;
+      V3<double> nee_w{};
+      V3<float> so, sd;
+      float stmin = 1.f, stmax = 0.f;
+      if (has_vol) {
+        V3<float> alb{};
+        const float t_end = hr.hit ? hr.t : rtmax;
+        const float ts = s.vol.sample_scatter<float>(ro, rd, rtmin, t_end, rng, alb);
+        if (ts >= 0.f) {
+          const V3<float> p = ro + rd * ts;
+          if (primary) primary = false;
+          if (depth >= max_depth || emission_only) {
+            active = false;
+          } else {
+            const V3<float> tpv = tp * alb;
+            const V3<double> twv = tw * wv(alb);
+            const float phase = 0.25f * K<float>::INV_PI;
+            const Direct<float> ds = sample_direct<float>(s, p, rng, strategies, nullptr);
+            if (ds.valid) {
+              const float wm = ds.mis ? power_heuristic(1, ds.pdf, 1, phase) : 1.f;
+              nee_w = twv * wv(ds.L) * (double(phase) * double(wm) / double(ds.pdf));
+              nee = true;
+              so = p;
+              sd = ds.wi;
+              stmin = 0.f;
+              stmax = ds.dist < 1e29f ? ds.dist * (1.f - 1e-4f) : 1e30f;
+            }
+            V3<float> tn = tpv;
+            V3<double> twn = twv;
+            bool killed = false;
+            if (depth >= 2) {
+              const float mt = fmaxT(tpv.x, fmaxT(tpv.y, tpv.z));
+              const float surv = fminT(fmaxT(mt, 0.05f), 0.95f);
+              if (draw<float>(rng) >= surv) killed = true;
+              else {
+                tn = tpv * (1.f / surv);
+                twn = twv * (1.0 / double(surv));
+              }
+            }
+            if (killed) {
+              active = false;
+            } else {
+              tp = tn;
+              tw = twn;
+              prev_pdf = phase;
+              depth += 1;
+              if (count_bounce(kinds, 24u, rp.max_volume_bounces)) kinds |= PATH_EMISSION_ONLY;
+              ro = p;
+              rd = sample_uniform_sphere<float>(rng);
+              rtmin = 0.f;
+              rtmax = 1e30f;
+            }
+          }
+          goto pack;
+        }
+      }
+      if (!hr.hit) {
+        V3<float> L{};
+        if (primary) {
+          L = (s.has_env && s.env_visible) ? s.env.evaluate<float>(rd) : bg * rp.background_alpha;
+        } else {
+          L = ambient;
+          if (s.has_env) {
+            float wm = 1.f;
+            if (prev_pdf > 0.f) wm = power_heuristic(1, prev_pdf, 1, s.env.pdf<float>(rd) / float(strategies));
+            L = L + s.env.evaluate<float>(rd) * wm;
+          }
+        }
+        acc = acc + tw * wv(L);
+        active = false;
+        goto pack;
+      }
+      {
+        const V3<float> p = hr.p;
+        const V3<float> wo = -rd;
+        V3<float> hsn = hr.sn;
+        if (!(dot(hsn, hsn) > 0.25f)) hsn = hr.n;
+        else if (dot(hsn, hr.n) < 0.f) hsn = hsn * -1.f;
+        float alpha = 1.f;
+        const Mat<float> mat = resolve_material<float>(s, hr.mat, hr.uv, hr.has_color, hr.color, alpha, nullptr);
+        const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0 && primary;
+        if (hidden || (alpha < 1.f && draw<float>(rng) >= alpha)) {
+          ro = p;
+          rtmin = eps;
+          rtmax = 1e30f;
+          path_dist += hr.t;
+          if (primary) {
+            float tn, tf;
+            clip_range<float>(P.cam, rd, tn, tf);
+            if (tf < 1e30f) {
+              rtmax = tf - path_dist;
+              if (rtmax < eps) {
+                rtmin = 1.f;
+                rtmax = 0.f;
+              }
+            }
+          }
+          prev_pdf = -1.f;
+          goto pack;
+        }
+        V3<float> n = hr.n, sn = hsn;
+        const bool front = dot(n, wo) >= 0.f;
+        if (!(mat.transmission > 0.f) && !front) {
+          n = n * -1.f;
+          sn = sn * -1.f;
+        }
+        if (mat.normal_tex >= 0 && !s.atlas.infos.empty()) {
+          V3<float> nm = s.atlas.sample<float>(mat.normal_tex, hr.uv.x, hr.uv.y);
+          nm = nm * 2.f - V3<float>{1.f, 1.f, 1.f};
+          V3<float> ta, bi;
+          onb_from_normal(sn, ta, bi);
+          sn = normalize(ta * nm.x + bi * nm.y + sn * nm.z);
+        }
+        const Mat<double> mw = detached_material(s, E, hr.mat, hr.uv, hr.has_color, hr.color);
+        if (primary) primary = false;
+        if (material_may_emit(mat) && emits_on_side(mat, front)) {
+          float wm = 1.f;
+          if (prev_pdf > 0.f && s.emissive_count > 0) {
+            const float pa = s.prim_light_pdf[hr.prim];
+            const float cl = std::fabs(dot(hr.n, wo));
+            if (pa > 0.f && cl > 0.f) wm = power_heuristic(1, prev_pdf, 1, pa * hr.t * hr.t / cl / float(strategies));
+          }
+          acc = acc + tw * wv(material_emission(mat) * wm);
+        }
+        if ((mat.flags & PHX_MATERIAL_FLAG_EMITTER_ONLY) != 0 || depth >= max_depth || emission_only) {
+          active = false;
+          goto pack;
+        }
+        {
+          const Direct<float> ds = sample_direct<float>(s, p, rng, strategies, nullptr);
+          if (ds.valid) {
+            const V3<float> f = disney_bsdf_eval(mat, wo, ds.wi, n, sn);
+            if (f.x > 0.f || f.y > 0.f || f.z > 0.f) {
+              float wm = 1.f;
+              if (ds.mis) wm = power_heuristic(1, ds.pdf, 1, disney_bsdf_pdf(mat, wo, ds.wi, n, sn));
+              const V3<double> fw = disney_bsdf_eval(mw, wv(wo), wv(ds.wi), wv(n), wv(sn));
+              nee_w = tw * fw * wv(ds.L) * (double(wm) / double(ds.pdf));
+              nee = true;
+              const V3<float> sorg = p + n * eps;
+              so = sorg;
+              stmin = 0.f;
+              if (ds.dist < 1e29f) {
+                const V3<float> tl = p + ds.wi * ds.dist - sorg;
+                const float sdist = length(tl);
+                sd = sdist > 0.f ? tl * (1.f / sdist) : ds.wi;
+                stmax = sdist * (1.f - 1e-3f);
+              } else {
+                sd = ds.wi;
+                stmax = 1e30f;
+              }
+            }
+          }
+        }
+        const BsdfSample<float> bs = disney_bsdf_sample(mat, wo, n, sn, rng);
+        if (bs.pdf <= 0.f || (bs.f.x <= 0.f && bs.f.y <= 0.f && bs.f.z <= 0.f)) {
+          active = false;
+          goto pack;
+        }
+        V3<float> tpn = tp * (bs.f * (1.f / bs.pdf));
+        const V3<double> fw =
+            bs.is_specular ? wv(bs.f) : disney_bsdf_eval(mw, wv(wo), wv(bs.wi), wv(n), wv(sn));
+        V3<double> twn = tw * fw * (1.0 / double(bs.pdf));
+        if (depth >= 2) {
+          const float mt = fmaxT(tpn.x, fmaxT(tpn.y, tpn.z));
+          const float surv = fminT(fmaxT(mt, 0.05f), 0.95f);
+          if (draw<float>(rng) >= surv) {
+            active = false;
+            goto pack;
+          }
+          tpn = tpn * (1.f / surv);
+          twn = twn * (1.0 / double(surv));
+        }
+        if (!(tpn.x == tpn.x && tpn.y == tpn.y && tpn.z == tpn.z)) {
+          active = false;
+          goto pack;
+        }
+        tp = tpn;
+        tw = twn;
+        prev_pdf = bs.is_specular ? -1.f : bs.pdf;
+        depth += 1;
+        const V3<float> wi = normalize(bs.wi);
+        const float side = dot(wi, hr.n) < 0.f ? -1.f : 1.f;
+        const bool through = (dot(wi, hr.n) < 0.f) != (dot(wo, hr.n) < 0.f);
+        const bool exceeded = through ? count_bounce(kinds, 16u, rp.max_transmission_bounces)
+                                      : (bs.is_diffuse ? count_bounce(kinds, 0u, rp.max_diffuse_bounces)
+                                                       : count_bounce(kinds, 8u, rp.max_glossy_bounces));
+        if (exceeded) kinds |= PATH_EMISSION_ONLY;
+        ro = p + hr.n * (side * eps);
+        rd = wi;
+        rtmin = 0.f;
+        rtmax = 1e30f;
+      }
+    pack:
+      if (nee && !occluded(s, so, sd, stmin, stmax)) {
+        double tr = 1.0;
+        if (has_vol) {
+          Rng trng = make_rng(idx, sample, 0x10000u + it);
+          tr = double(s.vol.transmittance<float>(so, sd, stmin, stmax, trng));
+        }
+        const V3<double> c = nee_w * tr;
+        if (c.x == c.x && c.y == c.y && c.z == c.z) acc = acc + c;
+      }
+    }
+  }
+  return rp.samples_per_pixel ? acc * (1.0 / double(rp.samples_per_pixel)) : acc;
+}
+
 // AOVs are written by a separate, simpler pass: the first camera-ray event of
 // the first sample (pathtracer.cpp:800-809, 974-983).  They do not feed the
 // color, so the color pass above leaves them out.
@@ -952,6 +1234,31 @@ void orc_bsdf_eval(const phx_material *mats, const float *wo, const float *wi, c
     }
   }
 }
+// Central differences in double of disney_bsdf_eval with respect to base
+// colour r, g, b, roughness and metallic (step h): df[i*15 + k*3 + c].
+void orc_bsdf_eval_grad_fd(const phx_material *mats, const float *wo, const float *wi, const float *ng, const float *sn,
+                           uint32_t n, double h, double *df) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const V3<double> a = v3<double>(&wo[i * 3]), b = v3<double>(&wi[i * 3]), c = v3<double>(&ng[i * 3]),
+                     d = v3<double>(&sn[i * 3]);
+    for (int k = 0; k < 5; ++k) {
+      V3<double> f[2];
+      for (int sgn = 0; sgn < 2; ++sgn) {
+        Mat<double> m = mat_from<double>(mats[i]);
+        const double dh = sgn ? -h : h;
+        if (k == 0) m.base_color.x += dh;
+        if (k == 1) m.base_color.y += dh;
+        if (k == 2) m.base_color.z += dh;
+        if (k == 3) m.roughness += dh;
+        if (k == 4) m.metallic += dh;
+        f[sgn] = disney_bsdf_eval(m, a, b, c, d);
+      }
+      df[i * 15 + k * 3] = (f[0].x - f[1].x) / (2 * h);
+      df[i * 15 + k * 3 + 1] = (f[0].y - f[1].y) / (2 * h);
+      df[i * 15 + k * 3 + 2] = (f[0].z - f[1].z) / (2 * h);
+    }
+  }
+}
 // disney_bsdf_sample with a per-tuple seed: wi (3), f (3), pdf, flags
 // (bit 0 specular, bit 1 diffuse).
 void orc_bsdf_sample(const phx_material *mats, const float *wo, const float *ng, const float *sn, const uint32_t *seeds,
@@ -1038,6 +1345,31 @@ void orc_volume_scatter(void *sp, const float *org, const float *dir, float tmin
     V3<float> alb{};
     t_out[i] = s->vol.sample_scatter<float>(v3<float>(org), v3<float>(dir), tmin, tmax, r, alb);
   }
+}
+
+// The detached estimator (detached_pixel) of every pixel in double: the image
+// when `image` is not NULL, and the returned sum over pixels and channels of
+// delta[p * 3 + c] * image[p * 3 + c] when `delta` is not NULL.
+double orc_render_detached(void *sp, const phx_camera *cam, const phx_render_params *rp, const double *mat_params,
+                           uint32_t mat_count, const double *texels, const float *delta, double *image, int threads) {
+  Scene *s = static_cast<Scene *>(sp);
+  Params P{*rp, *cam};
+  const DetachedEval E{mat_params, mat_count, texels};
+  const uint32_t npix = rp->width * rp->height;
+  if (threads > 0) omp_set_num_threads(threads);
+  double total = 0.0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+  for (int64_t i = 0; i < int64_t(npix); ++i) {
+    const uint32_t idx = uint32_t(i);
+    const V3<double> c = detached_pixel(*s, P, idx, E);
+    if (image) {
+      image[idx * 3] = c.x;
+      image[idx * 3 + 1] = c.y;
+      image[idx * 3 + 2] = c.z;
+    }
+    if (delta) total += double(delta[idx * 3]) * c.x + double(delta[idx * 3 + 1]) * c.y + double(delta[idx * 3 + 2]) * c.z;
+  }
+  return total;
 }
 
 // PathTracer::render (pathtracer.cpp:564-1159) on the CPU with `threads`

@@ -222,7 +222,9 @@ struct EnvMap {
 struct Atlas {
   std::vector<float> pixels;
   std::vector<phx_texture_info> infos;
-  template <class T> V3<T> sample(int id, T u, T v) const {
+  template <class T> V3<T> sample(int id, T u, T v) const { return sample_from<T>(pixels.data(), id, u, v); }
+  // The same lookup over another pixel array of the atlas layout (float or double).
+  template <class T, class P> V3<T> sample_from(const P *pix, int id, T u, T v) const {
     if (id < 0 || uint32_t(id) >= infos.size()) return {T(1), T(1), T(1)};
     const phx_texture_info ti = infos[id];
     if (!ti.width || !ti.height) return {T(1), T(1), T(1)};
@@ -234,7 +236,10 @@ struct Atlas {
     auto wrap = [](int i, uint32_t sz) { const int s = int(sz); return uint32_t(((i % s) + s) % s); };
     const uint32_t x0 = wrap(ix, ti.width), x1 = wrap(ix + 1, ti.width);
     const uint32_t y0 = wrap(iy, ti.height), y1 = wrap(iy + 1, ti.height);
-    auto px = [&](uint32_t x, uint32_t y) { return v3<T>(&pixels[(size_t(ti.offset) + y * ti.width + x) * 3]); };
+    auto px = [&](uint32_t x, uint32_t y) {
+      const P *q = &pix[(size_t(ti.offset) + y * ti.width + x) * 3];
+      return V3<T>{T(q[0]), T(q[1]), T(q[2])};
+    };
     const V3<T> top = px(x0, y0) * (T(1) - tx) + px(x1, y0) * tx;
     const V3<T> bot = px(x0, y1) * (T(1) - tx) + px(x1, y1) * tx;
     return top * (T(1) - ty) + bot * ty;

@@ -41,6 +41,10 @@ def _sig(lib):
         "phx_synchronize": (C.c_int, [P]),
         "phx_bsdf_eval": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P]),
         "phx_bsdf_sample": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P, P, P]),
+        "phx_bsdf_eval_grad": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P]),
+        "phx_adjoint": (C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams), P, P, P]),
+        "phx_set_material_params": (C.c_int, [P, P, C.c_uint32]),
+        "phx_set_texture_pixels": (C.c_int, [P, P, C.c_uint32, C.c_uint32]),
         "phx_render": (C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams)]),
         "phx_render_reference_structure": (C.c_int, [P, C.POINTER(abi.Camera), C.POINTER(abi.RenderParams)]),
         "phx_read_result": (C.c_int, [P, PF, PF, PF, PF, PF]),

@@ -3,6 +3,7 @@
 // disney_bsdf_eval / disney_bsdf_pdf / disney_bsdf_sample (disney_bsdf.h:288-521).
 #include "context.h"
 #include "dev_bsdf.h"
+#include "dev_bsdf_ad.h"
 
 namespace {
 
@@ -40,9 +41,35 @@ __global__ void k_bsdf_sample(const phx_material *mats, const float *wo, const f
   flags[i] = (s.specular ? 1u : 0u) | (s.diffuse ? 2u : 0u);
 }
 
+__global__ void k_bsdf_eval_grad(const phx_material *mats, const float *wo, const float *wi, const float *ng,
+                                 const float *sn, uint32_t n, float *f, float *df) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const phx_material m = mats[i];
+  const DV3 r = bsdf_eval_d(m, ldv(wo, i), ldv(wi, i), ldv(ng, i), ldv(sn, i));
+  f[i * 3] = r.x.v;
+  f[i * 3 + 1] = r.y.v;
+  f[i * 3 + 2] = r.z.v;
+  for (int k = 0; k < kGradParams; ++k) {
+    df[i * 15 + k * 3] = r.x.d[k];
+    df[i * 15 + k * 3 + 1] = r.y.d[k];
+    df[i * 15 + k * 3 + 2] = r.z.d[k];
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int phx_bsdf_eval_grad(phx_context *c, const phx_material *mats, const float *wo, const float *wi, const float *ng,
+                       const float *sn, uint32_t n, float *f, float *df, void *stream) {
+  if (!c || (n && (!mats || !wo || !wi || !ng || !sn || !f || !df))) return PHX_ERR_INVALID;
+  if (n == 0) return PHX_OK;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  hipLaunchKernelGGL(k_bsdf_eval_grad, dim3((n + 255) / 256), dim3(256), 0, st, mats, wo, wi, ng, sn, n, f, df);
+  HIP_TRY(c, hipGetLastError());
+  return PHX_OK;
+}
 
 int phx_bsdf_eval(phx_context *c, const phx_material *mats, const float *wo, const float *wi, const float *ng,
                   const float *sn, uint32_t n, float *f, float *pdf, void *stream) {

@@ -341,6 +341,7 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
     if ((r = upload(c, c->tex_infos, d->texture_infos, d->texture_count))) return r;
     s.tex_count = d->texture_count;
   }
+  c->tex_pixel_count = s.tex_count ? d->texture_pixel_count : 0;
   s.tex_pixels = c->tex_pixels.as<float>();
   s.tex_infos = c->tex_infos.as<phx_texture_info>();
 
@@ -506,11 +507,11 @@ int phx_read_result(phx_context *c, float *color, float *alpha, float *depth, fl
   if (!c->have_result) return fail(c, PHX_ERR_NO_FRAME, "no frame has been rendered");
   const size_t n = size_t(c->f_w) * c->f_h;
   hipStream_t st = c->stream;
-  if (color) HIP_TRY(c, hipMemcpyAsync(color, c->f_color.p, n * 12, hipMemcpyDeviceToHost, st));
-  if (alpha) HIP_TRY(c, hipMemcpyAsync(alpha, c->f_alpha.p, n * 4, hipMemcpyDeviceToHost, st));
-  if (depth) HIP_TRY(c, hipMemcpyAsync(depth, c->f_depth.p, n * 4, hipMemcpyDeviceToHost, st));
-  if (normal) HIP_TRY(c, hipMemcpyAsync(normal, c->f_normal.p, n * 12, hipMemcpyDeviceToHost, st));
-  if (albedo) HIP_TRY(c, hipMemcpyAsync(albedo, c->f_albedo.p, n * 12, hipMemcpyDeviceToHost, st));
+  if (color) HIP_TRY(c, hipMemcpyAsync(color, c->f_color.p, n * 12, hipMemcpyDefault, st));
+  if (alpha) HIP_TRY(c, hipMemcpyAsync(alpha, c->f_alpha.p, n * 4, hipMemcpyDefault, st));
+  if (depth) HIP_TRY(c, hipMemcpyAsync(depth, c->f_depth.p, n * 4, hipMemcpyDefault, st));
+  if (normal) HIP_TRY(c, hipMemcpyAsync(normal, c->f_normal.p, n * 12, hipMemcpyDefault, st));
+  if (albedo) HIP_TRY(c, hipMemcpyAsync(albedo, c->f_albedo.p, n * 12, hipMemcpyDefault, st));
   HIP_TRY(c, hipStreamSynchronize(st));
   return PHX_OK;
 }

@@ -67,6 +67,10 @@ struct phx_context {
   DBuf materials, lights, em_ids, em_cdf, prim_light_pdf, env_pixels, env_marginal, env_cond, tex_pixels, tex_infos,
       volumes, vol_values, vol_tf, majorants;
   phx::DevScene dev{};
+  uint32_t tex_pixel_count = 0;  // texels of the atlas
+
+  // adjoint gradient accumulators (double)
+  DBuf adj_gm, adj_gt;
 
   // wavefront storage
   DBuf q_o[2], q_d[2], q_tp[2], q_st[2], q_hit, sh_o, sh_d, sh_c, sample_L, aov_s, counters, stats_buf, spill, fetch;

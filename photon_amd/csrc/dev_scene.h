@@ -235,6 +235,38 @@ PHX_DEV vec3 tex_sample(const DevScene &s, int id, float u, float v) {
   return top * (1.f - ty) + bot * ty;
 }
 
+// The four atlas texels tex_sample blends at (u, v) and their weights.
+struct TexTaps {
+  uint32_t idx[4];  // atlas pixel indices: (x0,y0), (x1,y0), (x0,y1), (x1,y1)
+  float w[4];
+  bool valid;
+};
+PHX_DEV TexTaps tex_taps(const DevScene &s, int id, float u, float v) {
+  TexTaps t{};
+  t.valid = false;
+  if (id < 0 || uint32_t(id) >= s.tex_count) return t;
+  const phx_texture_info ti = s.tex_infos[id];
+  if (!ti.width || !ti.height) return t;
+  u = u - floorf(u);
+  v = 1.f - (v - floorf(v));
+  const float fx = u * float(ti.width) - 0.5f, fy = v * float(ti.height) - 0.5f;
+  const int ix = int(floorf(fx)), iy = int(floorf(fy));
+  const float tx = fx - floorf(fx), ty = fy - floorf(fy);
+  const int W = int(ti.width), H = int(ti.height);
+  const uint32_t x0 = uint32_t(((ix % W) + W) % W), x1 = uint32_t((((ix + 1) % W) + W) % W);
+  const uint32_t y0 = uint32_t(((iy % H) + H) % H), y1 = uint32_t((((iy + 1) % H) + H) % H);
+  t.idx[0] = ti.offset + y0 * ti.width + x0;
+  t.idx[1] = ti.offset + y0 * ti.width + x1;
+  t.idx[2] = ti.offset + y1 * ti.width + x0;
+  t.idx[3] = ti.offset + y1 * ti.width + x1;
+  t.w[0] = (1.f - tx) * (1.f - ty);
+  t.w[1] = tx * (1.f - ty);
+  t.w[2] = (1.f - tx) * ty;
+  t.w[3] = tx * ty;
+  t.valid = true;
+  return t;
+}
+
 // Material at a hit with atlas textures applied (pathtracer.cpp:186-247).
 PHX_DEV phx_material resolve_material(const DevScene &s, uint32_t mid, vec2 uv, bool has_color, vec3 color,
                                       float &alpha) {

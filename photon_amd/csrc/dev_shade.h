@@ -105,8 +105,19 @@ PHX_DEV SurfaceHit surface_at(const DevScene &s, uint32_t tri, float t, float b1
   return h;
 }
 
+// Observers of the weights that depend on the material parameters, called by
+// shade_path at next-event estimation (f * cos toward the light and the
+// constant factor Le * w_mis / pdf_light) and at the BSDF sample that
+// continues the path.  The primal kernels pass NoHooks; the path-replay
+// adjoint records derivatives there.
+struct NoHooks {
+  PHX_DEV void nee(const phx_material &, const SurfaceHit &, vec3, vec3, vec3, vec3, vec3, vec3) {}
+  PHX_DEV void bsdf(const phx_material &, const SurfaceHit &, vec3, vec3, vec3, vec3, const BsdfSample &) {}
+};
+
+template <class Hooks>
 PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState &ps, bool hr_hit, uint32_t htri,
-                        float ht, float b1, float b2, Rng &rng, ShadeOut &out) {
+                        float ht, float b1, float b2, Rng &rng, ShadeOut &out, Hooks &hooks) {
   out.cont = false;
   out.shadow = false;
   out.add = vec3{0.f, 0.f, 0.f};
@@ -260,6 +271,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
         float wm = 1.f;
         if (ds.mis) wm = power_heuristic(ds.pdf, bsdf_pdf(mat, wo, ds.wi, n, sn));
         out.sc = clamp_contribution(tp * f * ds.L * (wm / ds.pdf), int(depth), w.clamp_direct, w.clamp_indirect);
+        hooks.nee(mat, sh, wo, ds.wi, n, sn, tp, ds.L * (wm / ds.pdf));
         const vec3 sorg = p + n * s.ray_eps;
         vec3 sdir = ds.wi;
         float stmax = 1e30f;
@@ -297,6 +309,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
                                 : (bs.diffuse ? count_bounce(kinds, 0u, w.max_diffuse)
                                               : count_bounce(kinds, 8u, w.max_glossy));
   if (exceeded) kinds |= kEmissionOnly;
+  hooks.bsdf(mat, sh, wo, n, sn, tp, bs);
   out.cont = true;
   out.next.ro = p + ng * (side * s.ray_eps);
   out.next.rd = wi;
@@ -307,6 +320,11 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
   out.next.depth = depth + 1u;
   out.next.kinds = kinds;
   out.next.primary = false;
+}
+PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState &ps, bool hr_hit, uint32_t htri,
+                        float ht, float b1, float b2, Rng &rng, ShadeOut &out) {
+  NoHooks h;
+  shade_path(s, w, ps, hr_hit, htri, ht, b1, b2, rng, out, h);
 }
 
 }  // namespace phx

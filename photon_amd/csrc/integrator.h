@@ -132,8 +132,9 @@ struct ViewArgs {
 hipError_t launch_run_start(const StreamQueues &q, uint32_t total, uint32_t capacity, hipStream_t st);
 // ahead = 0: fill the queue of the current iteration (start of a run);
 // ahead = 1: fill the next iteration's queue and advance the iteration.
+// zero_L = false keeps each new sample's stored radiance (the adjoint replay).
 hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead, const LoopShape &ls,
-                        hipStream_t st);
+                        hipStream_t st, bool zero_L = true);
 hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
@@ -143,6 +144,23 @@ hipError_t launch_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, f
 hipError_t launch_aov_init(uint32_t npix, uint32_t *aov_s, hipStream_t st);
 hipError_t launch_resolve(const DevScene &s, const ShadeParams &sp, const StreamQueues &q, const FrameBuffers &f,
                           uint32_t ns, uint32_t *fetch, uint32_t grid, hipStream_t st);
+
+// Path-replay adjoint (adjoint.hip).  Gradients are accumulated in double:
+// grad_mat[material * 5 + k] for k = base r, g, b, roughness, metallic, and
+// grad_tex[atlas pixel * 3 + c].  `adjoint` is dLoss/dImage (RGB per pixel,
+// frame-buffer layout at each view's pix_base).
+constexpr uint32_t kAdjLdsMaterials = 128;  // materials accumulated per work-group in LDS
+struct AdjointArgs {
+  const float *adjoint;
+  double *grad_mat;
+  double *grad_tex;  // null: texel gradients are not wanted
+  uint32_t lds_mats;  // min(material count, kAdjLdsMaterials)
+};
+hipError_t launch_shade_adjoint(const ViewArgs &v, const StreamQueues &q, const AdjointArgs &a, const LoopShape &ls,
+                                hipStream_t st);
+hipError_t launch_grad_out(const double *src, float *dst, uint32_t n, hipStream_t st);
+hipError_t launch_clear_f64(double *p, uint32_t n, hipStream_t st);
+hipError_t launch_set_material_params(phx_material *mats, const float *p, uint32_t count, hipStream_t st);
 
 // Seam A kernels (trace.hip).
 hipError_t launch_trace_compact(const DevScene &s, const float *org, const float *dir, const float *tmin,

@@ -20,56 +20,10 @@
 // summed in sample order: results do not depend on scheduling.
 #include "integrator.h"
 
-#include <type_traits>
-
 #include "dev_queue.h"
-#include "dev_trace.h"
+#include "dev_stream.h"
 
 namespace phx {
-
-constexpr uint32_t kPrimaryBit = 0x10000u;
-
-PHX_DEV vec3 xyz(float4 v) { return {v.x, v.y, v.z}; }
-PHX_DEV float4 f4(vec3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
-
-PHX_DEV PathState unpack(float4 o, float4 d, float4 tp, uint4 st) {
-  PathState p;
-  p.ro = xyz(o);
-  p.tmin = o.w;
-  p.rd = xyz(d);
-  p.tmax = d.w;
-  p.tp = xyz(tp);
-  p.prev_pdf = tp.w;
-  p.depth = st.y & 0xffu;
-  p.path_it = (st.y >> 8) & 0xffu;
-  p.primary = (st.y & kPrimaryBit) != 0u;
-  p.kinds = st.z;
-  p.path_dist = __uint_as_float(st.w);
-  return p;
-}
-PHX_DEV uint32_t pack_flags(const PathState &p) {
-  return (p.depth & 0xffu) | ((p.path_it & 0xffu) << 8) | (p.primary ? kPrimaryBit : 0u);
-}
-
-// Camera ray of sample s of a pixel (pathtracer.cpp:727-759).
-PHX_DEV void camera_path(const ShadeParams &w, uint32_t pixel, uint32_t sample, PathState &p) {
-  const uint32_t x = pixel % w.width, y = pixel / w.width;
-  Rng rng = make_rng(pixel, sample, 0u);
-  float u = (float(x) + rng.next()) / float(w.width);
-  float v = (float(y) + rng.next()) / float(w.height);
-  if (!w.bottom_up) v = 1.f - v;
-  u = w.region[0] + (w.region[2] - w.region[0]) * u;
-  v = w.region[1] + (w.region[3] - w.region[1]) * v;
-  camera_ray(w.cam, u, v, rng, p.ro, p.rd);
-  clip_range(w.cam, p.rd, p.tmin, p.tmax);
-  p.tp = vec3{1.f, 1.f, 1.f};
-  p.prev_pdf = -1.f;
-  p.depth = 0;
-  p.path_it = 0;
-  p.kinds = 0;
-  p.primary = true;
-  p.path_dist = 0.f;
-}
 
 // Clears the counters of a run and records its size.
 __global__ void k_run_start(StreamQueues q, uint32_t total, uint32_t capacity) {
@@ -77,25 +31,10 @@ __global__ void k_run_start(StreamQueues q, uint32_t total, uint32_t capacity) {
   if (i < kCounterWords) q.counters[i] = i == kTotal ? total : (i == kCapacity ? capacity : 0u);
 }
 
-// Traversal scene of a lane: the uniform kernel-argument scene, or the two
-// pointers traversal reads, copied out of the lane's scene-table entry.
-template <class P>
-PHX_DEV DevScene trace_scene(const P &v, uint32_t g) {
-  if constexpr (std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value) {
-    return v.scene;
-  } else {
-    const DevScene &sc = v.scene_of(v.item_at(v.find(g)));
-    DevScene ls{};
-    ls.nodes = sc.nodes;
-    ls.tris = sc.tris;
-    return ls;
-  }
-}
-
 // Fills the free slots of a queue with new camera paths (pt_init_sample).
 // ahead = 0: the current iteration's queue; 1: the next one.
 template <class P>
-__global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahead) {
+__global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahead, uint32_t zero_L) {
   const uint32_t it = q.counters[kIter] + ahead;
   const uint32_t used = q.counters[kRayCount + ring(it)];
   const uint32_t next = q.counters[kNextSample];
@@ -116,7 +55,7 @@ __global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahe
     q.d[buf][slot] = f4(p.rd, p.tmax);
     q.tp[buf][slot] = f4(p.tp, p.prev_pdf);
     q.st[buf][slot] = make_uint4(g, pack_flags(p), p.kinds, __float_as_uint(p.path_dist));
-    q.sample_L[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (zero_L) q.sample_L[g] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -394,10 +333,10 @@ hipError_t launch_run_start(const StreamQueues &q, uint32_t total, uint32_t capa
   return hipGetLastError();
 }
 hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead, const LoopShape &ls,
-                        hipStream_t st) {
+                        hipStream_t st, bool zero_L) {
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
-    hipLaunchKernelGGL(k_regen<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q, ahead);
+    hipLaunchKernelGGL(k_regen<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q, ahead, zero_L ? 1u : 0u);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, q, ahead);

@@ -59,6 +59,7 @@ constexpr uint32_t kIterBatch = 8;
 // turns a logic error into an error code instead of a hang.
 constexpr uint32_t kIterationGuard = 1u << 26;
 constexpr uint32_t kDefaultPaths = 1u << 25;
+constexpr uint32_t kGradParamsAbi = 5;  // base r, g, b, roughness, metallic
 
 // One view of a render or a batch.
 struct ViewJob {
@@ -187,6 +188,19 @@ hipError_t launch_iterations(const ViewArgs &va, const StreamQueues &q, const Lo
   return hipSuccess;
 }
 
+// Iterations of the adjoint replay: closest hit, replayed shading with inline
+// NEE visibility and gradients, regeneration that keeps the stored radiance.
+hipError_t launch_replay_iterations(const ViewArgs &va, const StreamQueues &q, const AdjointArgs &adj,
+                                    const LoopShape &ls, uint32_t n, hipStream_t st) {
+  for (uint32_t j = 0; j < n; ++j) {
+    hipError_t e;
+    if ((e = launch_extend(va, q, ls, st)) != hipSuccess) return e;
+    if ((e = launch_shade_adjoint(va, q, adj, ls, st)) != hipSuccess) return e;
+    if ((e = launch_regen(va, q, 1, ls, st, false)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 // Captures kIterBatch iterations of the bounce loop as a graph, unless the
 // graph already held was captured with the same launch arguments.
 int ensure_graph(phx_context *c, const ViewArgs &va, const StreamQueues &q, const LoopShape &ls) {
@@ -219,8 +233,11 @@ int ensure_graph(phx_context *c, const ViewArgs &va, const StreamQueues &q, cons
 
 // Streams every view's samples through the shared queues and resolves each
 // view into `fb` at its pix_base (AOV sample indices in aov[pix_base..]).
+// With `adj`, each run is followed by its adjoint replay and nothing is
+// resolved: the result is the gradients.
 int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::vector<const phx_context *> &scenes,
-                 const FrameBuffers &fb, uint32_t *aov, uint32_t total_pixels, bool use_graph) {
+                 const FrameBuffers &fb, uint32_t *aov, uint32_t total_pixels, bool use_graph,
+                 const AdjointArgs *adj = nullptr) {
   const std::vector<Run> runs = plan_runs(views);
   uint32_t max_total = 0;
   size_t max_items = 1;
@@ -327,37 +344,45 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
         va.many.count = uint32_t(pad);
       }
     }
-    const bool graph = use_graph && va.kind != 0 && !trace_counts && !tm.on;
+    const bool graph = use_graph && !adj && va.kind != 0 && !trace_counts && !tm.on;
     if (graph && (r = ensure_graph(c, va, q, ls))) return r;
-    HIP_TRY(c, launch_run_start(q, run.total, std::min(cap, run.total), st));
-    HIP_TRY(c, launch_regen(va, q, 0, ls, st));
-    uint32_t it = 0;
-    while (true) {
-      if (graph) {
-        HIP_TRY(c, hipGraphLaunch(c->graph, st));
-        it += kIterBatch;
-      } else {
-        HIP_TRY(c, launch_iterations(va, q, ls, batch, tm.on ? &tm : nullptr, st));
-        it += batch;
+    for (int pass = 0; pass < (adj ? 2 : 1); ++pass) {
+      const bool replay = pass == 1;
+      HIP_TRY(c, launch_run_start(q, run.total, std::min(cap, run.total), st));
+      HIP_TRY(c, launch_regen(va, q, 0, ls, st, !replay));
+      uint32_t it = 0;
+      while (true) {
+        if (replay) {
+          HIP_TRY(c, launch_replay_iterations(va, q, *adj, ls, batch, st));
+          it += batch;
+        } else if (graph) {
+          HIP_TRY(c, hipGraphLaunch(c->graph, st));
+          it += kIterBatch;
+        } else {
+          HIP_TRY(c, launch_iterations(va, q, ls, batch, tm.on ? &tm : nullptr, st));
+          it += batch;
+        }
+        // the iteration counter, the samples started, then that iteration's queue
+        HIP_TRY(c, hipMemcpyAsync(&c->pinned[0], q.counters + kIter, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipMemcpyAsync(&c->pinned[1], q.counters + kNextSample, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipMemcpyAsync(&c->pinned[2], q.counters + kRayCount + ring(it), 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipStreamSynchronize(st));
+        if (c->pinned[0] != it)
+          return fail(c, PHX_ERR_OVERFLOW, "render: wavefront iteration counter out of step");
+        if (trace_counts)
+          std::fprintf(stderr, "[phx] %s iteration %u: queue %u, started %u of %u samples\n",
+                       replay ? "replay" : "render", it, c->pinned[2], c->pinned[1], run.total);
+        if (c->pinned[2] == 0 && c->pinned[1] >= run.total) break;
+        if (it >= kIterationGuard)
+          return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");
       }
-      // the iteration counter, the samples started, then that iteration's queue
-      HIP_TRY(c, hipMemcpyAsync(&c->pinned[0], q.counters + kIter, 4, hipMemcpyDeviceToHost, st));
-      HIP_TRY(c, hipMemcpyAsync(&c->pinned[1], q.counters + kNextSample, 4, hipMemcpyDeviceToHost, st));
-      HIP_TRY(c, hipMemcpyAsync(&c->pinned[2], q.counters + kRayCount + ring(it), 4, hipMemcpyDeviceToHost, st));
-      HIP_TRY(c, hipStreamSynchronize(st));
-      if (c->pinned[0] != it)
-        return fail(c, PHX_ERR_OVERFLOW, "render: wavefront iteration counter out of step");
-      if (trace_counts)
-        std::fprintf(stderr, "[phx] iteration %u: queue %u, started %u of %u samples\n", it, c->pinned[2],
-                     c->pinned[1], run.total);
-      if (c->pinned[2] == 0 && c->pinned[1] >= run.total) break;
-      if (it >= kIterationGuard)
-        return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");
     }
+    if (adj) continue;
     for (const HostItem &h : run.items)
       HIP_TRY(c, launch_reduce(h.r.sp.npix, h.nspp, q.sample_L + h.r.first, fb.accum + h.r.pix_base, h.r.s0 == 0, st));
   }
   for (const ViewJob &j : views) {
+    if (adj) break;
     StreamQueues qv = q;
     qv.aov_s = aov + j.pix_base;
     HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
@@ -477,11 +502,11 @@ extern "C" int phx_read_batch_result(phx_context *c, uint32_t view, float *color
   HIP_TRY(c, hipSetDevice(c->device));
   const auto &v = c->bt_views[view];
   const size_t n = size_t(v.width) * v.height, b = v.pix_base;
-  if (color) HIP_TRY(c, hipMemcpy(color, c->bt_color.as<float>() + b * 3, n * 12, hipMemcpyDeviceToHost));
-  if (alpha) HIP_TRY(c, hipMemcpy(alpha, c->bt_alpha.as<float>() + b, n * 4, hipMemcpyDeviceToHost));
-  if (depth) HIP_TRY(c, hipMemcpy(depth, c->bt_depth.as<float>() + b, n * 4, hipMemcpyDeviceToHost));
-  if (normal) HIP_TRY(c, hipMemcpy(normal, c->bt_normal.as<float>() + b * 3, n * 12, hipMemcpyDeviceToHost));
-  if (albedo) HIP_TRY(c, hipMemcpy(albedo, c->bt_albedo.as<float>() + b * 3, n * 12, hipMemcpyDeviceToHost));
+  if (color) HIP_TRY(c, hipMemcpy(color, c->bt_color.as<float>() + b * 3, n * 12, hipMemcpyDefault));
+  if (alpha) HIP_TRY(c, hipMemcpy(alpha, c->bt_alpha.as<float>() + b, n * 4, hipMemcpyDefault));
+  if (depth) HIP_TRY(c, hipMemcpy(depth, c->bt_depth.as<float>() + b, n * 4, hipMemcpyDefault));
+  if (normal) HIP_TRY(c, hipMemcpy(normal, c->bt_normal.as<float>() + b * 3, n * 12, hipMemcpyDefault));
+  if (albedo) HIP_TRY(c, hipMemcpy(albedo, c->bt_albedo.as<float>() + b * 3, n * 12, hipMemcpyDefault));
   return PHX_OK;
 }
 
@@ -540,5 +565,61 @@ extern "C" int phx_autotune(phx_context *c, const phx_view *views, uint32_t coun
   if ((r = phx_set_tuning(c, &cur))) return r;
   if (best) *best = cur;
   if (best_ms) *best_ms = cur_ms;
+  return PHX_OK;
+}
+
+extern "C" int phx_adjoint(phx_context *c, const phx_camera *cam, const phx_render_params *p, const float *d_adjoint,
+                           float *d_grad_materials, float *d_grad_texture) {
+  if (!c || !cam || !p || !d_adjoint || !d_grad_materials) return PHX_ERR_INVALID;
+  if (!c->have_scene) return fail(c, PHX_ERR_NO_SCENE, "adjoint before set_scene");
+  int r = check_params(c, p, "adjoint");
+  if (r) return r;
+  HIP_TRY(c, hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t npix = p->width * p->height;
+  const uint32_t nm = c->dev.material_count * kGradParamsAbi;
+  const uint32_t nt = d_grad_texture ? c->tex_pixel_count * 3 : 0;
+  HIP_TRY(c, c->adj_gm.reserve(size_t(nm) * 8));
+  if (nt) HIP_TRY(c, c->adj_gt.reserve(size_t(nt) * 8));
+  HIP_TRY(c, c->aov_s.reserve(size_t(npix) * 4));
+  hipStream_t st = c->stream;
+  HIP_TRY(c, launch_clear_f64(c->adj_gm.as<double>(), nm, st));
+  if (nt) HIP_TRY(c, launch_clear_f64(c->adj_gt.as<double>(), nt, st));
+  AdjointArgs a{};
+  a.adjoint = d_adjoint;
+  a.grad_mat = c->adj_gm.as<double>();
+  a.grad_tex = nt ? c->adj_gt.as<double>() : nullptr;
+  a.lds_mats = std::min<uint32_t>(c->dev.material_count, kAdjLdsMaterials);
+  const std::vector<ViewJob> views{make_job(c, 0, *cam, *p, 0)};
+  const FrameBuffers fb{};
+  if ((r = stream_views(c, views, {c}, fb, c->aov_s.as<uint32_t>(), npix, false, &a))) return r;
+  HIP_TRY(c, launch_grad_out(c->adj_gm.as<double>(), d_grad_materials, nm, st));
+  if (nt) HIP_TRY(c, launch_grad_out(c->adj_gt.as<double>(), d_grad_texture, nt, st));
+  HIP_TRY(c, hipStreamSynchronize(st));
+  c->stats.last_render_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return PHX_OK;
+}
+
+extern "C" int phx_set_material_params(phx_context *c, const float *d_params, uint32_t count) {
+  if (!c || (count && !d_params)) return PHX_ERR_INVALID;
+  if (!c->have_scene) return fail(c, PHX_ERR_NO_SCENE, "set_material_params before set_scene");
+  if (count > c->dev.material_count) return fail(c, PHX_ERR_INVALID, "set_material_params: more than the scene's materials");
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, launch_set_material_params(c->materials.as<phx_material>(), d_params, count, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PHX_OK;
+}
+
+extern "C" int phx_set_texture_pixels(phx_context *c, const float *d_pixels, uint32_t first, uint32_t count) {
+  if (!c || (count && !d_pixels)) return PHX_ERR_INVALID;
+  if (!c->have_scene) return fail(c, PHX_ERR_NO_SCENE, "set_texture_pixels before set_scene");
+  if (uint64_t(first) + count > c->tex_pixel_count)
+    return fail(c, PHX_ERR_INVALID, "set_texture_pixels: outside the texture atlas");
+  if (count == 0) return PHX_OK;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipMemcpyAsync(c->tex_pixels.as<float>() + size_t(first) * 3, d_pixels, size_t(count) * 12,
+                            hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
   return PHX_OK;
 }
