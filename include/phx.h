@@ -254,7 +254,7 @@ typedef struct phx_hit_compact {
 typedef struct phx_tuning {
   uint32_t trace_block;          /* work-group size of the traversal kernels: 128 or 256 (256) */
   uint32_t trace_groups_per_cu;  /* persistent traversal work-groups per CU, 1..32 (8) */
-  uint32_t reserved0;            /* must be 0 */
+  uint32_t refill_lanes;         /* idle lanes of a traversal wave that make it take new rays, 1..64 (16) */
   uint32_t queue_chunk;          /* rays a wave takes per queue fetch, 64..4096, multiple of 64 (256) */
   uint32_t paths_in_flight;      /* path slots of the wavefront (2^25) */
   uint32_t sort_by_material;     /* reserved, must be 0 */
@@ -351,7 +351,7 @@ int phx_render_batch(phx_context *ctx, const phx_view *views, uint32_t count);
 int phx_read_batch_result(phx_context *ctx, uint32_t view, float *color_rgb, float *alpha, float *depth,
                           float *normal_xyz, float *albedo_rgb);
 /* Times phx_render_batch of `views` under a set of launch shapes (traversal
- * work-group size and groups per CU, queue chunk, path slots) and keeps the fastest in ctx's tuning.  `best` (optional) receives it,
+ * work-group size and groups per CU, queue chunk, refill threshold, path slots) and keeps the fastest in ctx's tuning.  `best` (optional) receives it,
  * `best_ms` (optional) its time.  Each candidate runs `repeats` times (>= 1). */
 int phx_autotune(phx_context *ctx, const phx_view *views, uint32_t count, uint32_t repeats, phx_tuning *best,
                  double *best_ms);

@@ -113,7 +113,7 @@ class HitCompact(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [
-        ("trace_block", C.c_uint32), ("trace_groups_per_cu", C.c_uint32), ("reserved0", C.c_uint32),
+        ("trace_block", C.c_uint32), ("trace_groups_per_cu", C.c_uint32), ("refill_lanes", C.c_uint32),
         ("queue_chunk", C.c_uint32), ("paths_in_flight", C.c_uint32), ("sort_by_material", C.c_uint32),
         ("use_graph", C.c_uint32), ("time_kernels", C.c_uint32),
     ]

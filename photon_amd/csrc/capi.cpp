@@ -121,7 +121,7 @@ int phx_set_tuning(phx_context *c, const phx_tuning *t) {
   if (t->trace_block != 0 && t->trace_block != 128 && t->trace_block != 256)
     return fail(c, PHX_ERR_UNSUPPORTED, "set_tuning: trace_block must be 128 or 256");
   if (t->trace_groups_per_cu > 32) return fail(c, PHX_ERR_INVALID, "set_tuning: at most 32 work-groups per CU");
-  if (t->reserved0 != 0) return fail(c, PHX_ERR_INVALID, "set_tuning: reserved field must be 0");
+  if (t->refill_lanes > 64) return fail(c, PHX_ERR_INVALID, "set_tuning: refill_lanes must be at most 64");
   if (t->queue_chunk != 0 && (t->queue_chunk % 64 != 0 || t->queue_chunk > 4096))
     return fail(c, PHX_ERR_INVALID, "set_tuning: queue_chunk must be a multiple of 64 up to 4096");
   if (t->paths_in_flight > (1u << 28)) return fail(c, PHX_ERR_INVALID, "set_tuning: more than 2^28 path slots");

@@ -22,6 +22,14 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, bool pred) {
   return base + rank;
 }
 
+// Adds the sum over the wave's lanes of `n` to a 64-bit statistic, one atomic
+// per wave.  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_sum(unsigned long long *counter, uint32_t n) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+  if (lane_id() == 0 && n) atomicAdd(counter, (unsigned long long)n);
+}
+
 // Adds the wave's count of `pred` lanes to a 64-bit statistic, one atomic per wave.
 __device__ __forceinline__ void wave_count(unsigned long long *counter, bool pred) {
   const uint64_t mask = __ballot(pred);

@@ -164,4 +164,138 @@ PHX_DEV bool traverse(const DevScene &s, vec3 o, vec3 d, float tmin, float tmax,
   return h.tri != 0xffffffffu;
 }
 
+// ---- streaming traversal of a ray queue ---------------------------------------
+// The persistent traversal kernels run one node (or one leaf) per loop
+// iteration for every lane that holds a ray.  A lane whose ray is done takes
+// the next ray at once, instead of idling until the wave's slowest ray is
+// done: when at least `refill_min` lanes are idle, the wave hands them rays
+// from its current chunk of the queue, taking a new chunk of `chunk` rays from
+// `counter` (one atomic per wave) when the chunk runs out.  Per ray the result
+// is exactly traverse<ANY>'s.
+constexpr uint32_t kNoRay = 0xffffffffu;
+
+// load(i, o, d, tmin, tmax, nodes, tris) reads ray i and the acceleration
+// structure it is traced against; done(i, hit, h) consumes its result.
+template <bool ANY, class Load, class Done>
+PHX_DEV void traverse_stream(LaneStack &st, uint32_t *counter, uint32_t count, uint32_t chunk, uint32_t refill_min,
+                             Load load, Done done) {
+  const uint32_t lane = __lane_id();
+  uint32_t next = 0, end = 0;  // the wave's current chunk [next, end)
+  bool exhausted = false;
+  uint32_t idx = kNoRay;
+  vec3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 1.f}, inv{0.f, 0.f, 0.f};
+  float tmin = 0.f, best = 0.f;
+  int node = 0;
+  const Node4 *nodes = nullptr;
+  const TriRec *tris = nullptr;
+  TraceHit h;
+  h.tri = h.prim = 0xffffffffu;
+  h.t = h.u = h.v = 0.f;
+  while (true) {
+    const uint64_t idle = __ballot(idx == kNoRay);
+    const uint32_t nidle = uint32_t(__popcll(idle));
+    if (nidle == 64u && exhausted) break;
+    if (!exhausted && (nidle >= refill_min || nidle == 64u)) {
+      const uint32_t avail = end - next;
+      uint32_t nb = kNoRay;
+      if (nidle > avail) {
+        uint32_t b = 0;
+        if (lane == 0) b = atomicAdd(counter, chunk);
+        nb = __shfl(b, 0);
+        if (nb >= count) nb = kNoRay;
+      }
+      if (idx == kNoRay) {
+        const uint32_t rank = uint32_t(__popcll(idle & ((uint64_t(1) << lane) - 1)));
+        uint32_t i = kNoRay;
+        if (rank < avail) i = next + rank;
+        else if (nb != kNoRay) i = nb + (rank - avail);
+        if (i < count) {
+          float tmax;
+          load(i, o, d, tmin, tmax, nodes, tris);
+          h.tri = h.prim = 0xffffffffu;
+          h.t = tmax;
+          h.u = h.v = 0.f;
+          if (!(tmax > tmin)) {
+            done(i, false, h);
+          } else {
+            idx = i;
+            inv = vec3{safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
+            best = tmax;
+            node = 0;
+            st.sp = 0;
+          }
+        }
+      }
+      if (nidle <= avail) {
+        next += nidle;
+      } else if (nb != kNoRay) {
+        next = nb + (nidle - avail);
+        end = nb + chunk;
+      } else {
+        next = end;
+        exhausted = true;
+      }
+      if (next >= count) {
+        next = end;
+        exhausted = true;
+      }
+    }
+    if (idx != kNoRay) {
+      bool finished = false;
+      bool pop = true;
+      if (node >= 0) {
+        const Node4 n = nodes[node];
+        float tn[4];
+        bool hb[4];
+        box4(n, o, inv, tmin, best, tn, hb);
+        float k0 = hb[0] ? tn[0] : INFINITY, k1 = hb[1] ? tn[1] : INFINITY;
+        float k2 = hb[2] ? tn[2] : INFINITY, k3 = hb[3] ? tn[3] : INFINITY;
+        int c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
+        const int cnt = int(hb[0]) + int(hb[1]) + int(hb[2]) + int(hb[3]);
+        if (cnt > 0) {
+          cswap(k0, c0, k1, c1);
+          cswap(k2, c2, k3, c3);
+          cswap(k0, c0, k2, c2);
+          cswap(k1, c1, k3, c3);
+          cswap(k1, c1, k2, c2);
+          if (cnt > 3) st.push(c3);
+          if (cnt > 2) st.push(c2);
+          if (cnt > 1) st.push(c1);
+          node = c0;
+          pop = false;
+        }
+      } else {
+        const uint32_t code = ~uint32_t(node);
+        const uint32_t first = code >> 3, cnt = (code & 7u) + 1u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+          const TriRec r = tris[first + k];
+          float t, u, v;
+          if (!tri_test(r, o, d, tmin, best, t, u, v)) continue;
+          const uint32_t prim = __float_as_uint(r.v0.w);
+          if (t < best || prim > h.prim || h.tri == 0xffffffffu) {
+            best = t;
+            h.t = t;
+            h.u = u;
+            h.v = v;
+            h.tri = first + k;
+            h.prim = prim;
+            if (ANY) {
+              finished = true;
+              break;
+            }
+          }
+        }
+      }
+      if (pop && !finished) {
+        if (st.sp == 0) finished = true;
+        else node = st.pop();
+      }
+      if (finished) {
+        done(idx, h.tri != 0xffffffffu, h);
+        idx = kNoRay;
+      }
+    }
+  }
+}
+
 }  // namespace phx

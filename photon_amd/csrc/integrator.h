@@ -116,6 +116,7 @@ struct LoopShape {
   uint32_t shade_grid;   // work-groups (of 256) of the shade kernel: covers the path slots
   uint32_t regen_grid;   // work-groups (of 256) of the regeneration kernel
   uint32_t chunk;        // work items a wave takes per fetch (multiple of 64)
+  uint32_t refill;       // idle lanes of a traversal wave that trigger a refill (1..64)
 };
 
 // RenderParams -> the shading constants of one frame (render.cpp).

@@ -77,9 +77,24 @@ __global__ void k_commit(StreamQueues q, uint32_t ahead) {
   }
 }
 
-// Closest hit over the ray queue of the current iteration (persistent waves).
+// The acceleration structure of a lane's ray: the kernel-argument scene, or the
+// ray's view's scene-table entry.
+template <class P>
+PHX_DEV void ray_accel(const P &v, uint32_t g, const Node4 *&nodes, const TriRec *&tris) {
+  if constexpr (std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value) {
+    nodes = v.scene.nodes;
+    tris = v.scene.tris;
+  } else {
+    const DevScene &sc = v.scene_of(v.item_at(v.find(g)));
+    nodes = sc.nodes;
+    tris = sc.tris;
+  }
+}
+
+// Closest hit over the ray queue of the current iteration (persistent waves,
+// per-lane refill).
 template <class P, int B>
-__global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chunk) {
+__global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chunk, uint32_t refill) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[kRayCount + ring(it)];
   if (count == 0) return;
@@ -87,24 +102,20 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
   const int buf = it & 1;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
-  const uint32_t lane = lane_id();
-  WorkCursor cur = first_chunk(chunk);
-  while (true) {
-    const uint32_t base = next_batch(cur, &q.counters[kFetchExtend + ring(it)], count, chunk);
-    if (base >= count) break;
-    const uint32_t i = base + lane;
-    const bool valid = i < count;
-    bool live = false;
-    if (valid) {
-      const float4 o = q.o[buf][i], d = q.d[buf][i];
-      live = d.w > o.w;
-      const DevScene sc = trace_scene(v, q.st[buf][i].x);
-      TraceHit h;
-      traverse<false>(sc, xyz(o), xyz(d), o.w, d.w, st, h);
-      q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri));
-    }
-    wave_count(&q.stats[kStatClosest], live);
-  }
+  uint32_t live = 0;
+  traverse_stream<false>(
+      st, &q.counters[kFetchExtend + ring(it)], count, chunk, refill,
+      [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const Node4 *&nodes, const TriRec *&tris) {
+        const float4 a = q.o[buf][i], b = q.d[buf][i];
+        o = xyz(a);
+        d = xyz(b);
+        tmin = a.w;
+        tmax = b.w;
+        live += tmax > tmin ? 1u : 0u;
+        ray_accel(v, q.st[buf][i].x, nodes, tris);
+      },
+      [&](uint32_t i, bool, const TraceHit &h) { q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri)); });
+  wave_sum(&q.stats[kStatClosest], live);
 }
 
 // pt_shade over the ray queue, one thread per path (a persistent loop would
@@ -162,47 +173,59 @@ __global__ __launch_bounds__(256) void k_shade(P v, StreamQueues q) {
   }
 }
 
-// Any hit + NEE accumulation over the shadow queue (trace_occluded,
-// pt_nee_accum_pack; persistent waves).
+// Any hit over the shadow queue (trace_occluded; persistent waves, per-lane
+// refill).
 template <class P, int B>
-__global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chunk) {
+__global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chunk, uint32_t refill) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[kShadowCount + ring(it)];
   if (count == 0) return;
   __shared__ int lds[kShortStack * B];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
-  const uint32_t lane = lane_id();
-  WorkCursor cur = first_chunk(chunk);
-  while (true) {
-    const uint32_t base = next_batch(cur, &q.counters[kFetchShadow + ring(it)], count, chunk);
-    if (base >= count) break;
-    const uint32_t i = base + lane;
-    const bool valid = i < count;
-    bool live = false;
-    if (valid) {
-      const float4 o = q.sh_o[i], d = q.sh_d[i], c = q.sh_c[i];
-      live = d.w > 0.f;
-      const uint32_t g = __float_as_uint(c.w);
-      const DevScene sc = trace_scene(v, g);
-      TraceHit h;
-      const bool occ = traverse<true>(sc, xyz(o), xyz(d), 0.f, d.w, st, h);
-      if (!occ) {
-        vec3 contrib = xyz(c);
-        const RunItem &r = v.item_at(v.find(g));
-        const DevScene &s = v.scene_of(r);
-        if (s.vol_count > 0) {
-          const uint32_t l = g - r.first;
-          Rng rng = make_rng(l % r.sp.npix, r.sample_base + l / r.sp.npix, 0x10000u + __float_as_uint(o.w));
-          contrib = contrib * vol_transmittance(s, xyz(o), xyz(d), 0.f, d.w, rng);
-        }
-        if (finite3(contrib)) {
-          const float4 L = q.sample_L[g];
-          q.sample_L[g] = make_float4(L.x + contrib.x, L.y + contrib.y, L.z + contrib.z, L.w);
-        }
-      }
-    }
-    wave_count(&q.stats[kStatShadow], live);
+  uint32_t live = 0;
+  traverse_stream<true>(
+      st, &q.counters[kFetchShadow + ring(it)], count, chunk, refill,
+      [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const Node4 *&nodes, const TriRec *&tris) {
+        const float4 a = q.sh_o[i], b = q.sh_d[i];
+        o = xyz(a);
+        d = xyz(b);
+        tmin = 0.f;
+        tmax = b.w;
+        live += tmax > 0.f ? 1u : 0u;
+        ray_accel(v, __float_as_uint(q.sh_c[i].w), nodes, tris);
+      },
+      [&](uint32_t i, bool occ, const TraceHit &) {
+        // occluded entries lose their sample id; k_nee_accumulate adds the rest
+        if (occ) q.sh_c[i].w = __uint_as_float(kNoRay);
+      });
+  wave_sum(&q.stats[kStatShadow], live);
+}
+
+// pt_nee_accum_pack (pathtracer.cpp:1095-1121): the unoccluded shadow rays'
+// contributions, times the volume transmittance along them, into their
+// samples' radiance.  One thread per shadow ray; the grid covers the path slots.
+template <class P>
+__global__ __launch_bounds__(256) void k_nee_accumulate(P v, StreamQueues q) {
+  const uint32_t it = q.counters[kIter];
+  const uint32_t count = q.counters[kShadowCount + ring(it)];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float4 c = q.sh_c[i];
+  const uint32_t g = __float_as_uint(c.w);
+  if (g == kNoRay) return;
+  vec3 contrib = xyz(c);
+  const RunItem &r = v.item_at(v.find(g));
+  const DevScene &s = v.scene_of(r);
+  if (s.vol_count > 0) {
+    const float4 o = q.sh_o[i], d = q.sh_d[i];
+    const uint32_t l = g - r.first;
+    Rng rng = make_rng(l % r.sp.npix, r.sample_base + l / r.sp.npix, 0x10000u + __float_as_uint(o.w));
+    contrib = contrib * vol_transmittance(s, xyz(o), xyz(d), 0.f, d.w, rng);
+  }
+  if (finite3(contrib)) {
+    const float4 L = q.sample_L[g];
+    q.sample_L[g] = make_float4(L.x + contrib.x, L.y + contrib.y, L.z + contrib.z, L.w);
   }
 }
 
@@ -347,9 +370,9 @@ hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopSha
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
     if (ls.trace_block == 128)
-      hipLaunchKernelGGL((k_extend<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk);
+      hipLaunchKernelGGL((k_extend<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk, ls.refill);
     else
-      hipLaunchKernelGGL((k_extend<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk);
+      hipLaunchKernelGGL((k_extend<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill);
     return hipGetLastError();
   });
 }
@@ -364,9 +387,12 @@ hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopSha
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
     if (ls.trace_block == 128)
-      hipLaunchKernelGGL((k_shadow<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk);
+      hipLaunchKernelGGL((k_shadow<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk, ls.refill);
     else
-      hipLaunchKernelGGL((k_shadow<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk);
+      hipLaunchKernelGGL((k_shadow<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_nee_accumulate<P>, dim3(ls.shade_grid), dim3(256), 0, st, p, q);
     return hipGetLastError();
   });
 }

@@ -59,6 +59,7 @@ constexpr uint32_t kIterBatch = 8;
 // turns a logic error into an error code instead of a hang.
 constexpr uint32_t kIterationGuard = 1u << 26;
 constexpr uint32_t kDefaultPaths = 1u << 25;
+constexpr uint32_t kDefaultRefill = 16;
 constexpr uint32_t kGradParamsAbi = 5;  // base r, g, b, roughness, metallic
 
 // One view of a render or a batch.
@@ -150,6 +151,7 @@ LoopShape loop_shape(const phx_context *c, uint32_t capacity) {
   ls.shade_grid = std::max<uint32_t>(1, (capacity + 255) / 256);
   ls.regen_grid = std::max<uint32_t>(1, std::min<uint32_t>((capacity + 255) / 256, uint32_t(c->cu_count) * 16u));
   ls.chunk = t.queue_chunk ? t.queue_chunk : kChunk;
+  ls.refill = t.refill_lanes ? t.refill_lanes : kDefaultRefill;
   return ls;
 }
 
@@ -531,6 +533,7 @@ extern "C" int phx_autotune(phx_context *c, const phx_view *views, uint32_t coun
   if (!cur.trace_block) cur.trace_block = 256;
   if (!cur.trace_groups_per_cu) cur.trace_groups_per_cu = 8;
   if (!cur.queue_chunk) cur.queue_chunk = kChunk;
+  if (!cur.refill_lanes) cur.refill_lanes = kDefaultRefill;
   if (!cur.paths_in_flight) cur.paths_in_flight = kDefaultPaths;
   double cur_ms;
   int r = time_it(cur, cur_ms);
@@ -543,6 +546,7 @@ extern "C" int phx_autotune(phx_context *c, const phx_view *views, uint32_t coun
       {&phx_tuning::trace_block, {128, 256}},
       {&phx_tuning::trace_groups_per_cu, {4, 8, 16, 32}},
       {&phx_tuning::queue_chunk, {64, 128, 256, 512, 1024}},
+      {&phx_tuning::refill_lanes, {8, 16, 32, 64}},
       {&phx_tuning::paths_in_flight, {1u << 23, 1u << 24, 1u << 25}},
   };
   for (const auto &k : knobs) {
