@@ -19,6 +19,8 @@ in HBM before the timed region.  Prints one JSON line.
             16 views as separate renders
   adjoint   path-replay adjoint time over primal time on the differentiable
             workload
+  volume    the 256^3 heterogeneous-medium workload: frame time, and 8 orbit
+            views batched against 8 separate renders
 """
 import argparse
 import json
@@ -104,6 +106,62 @@ def batched_form(pt_c1, c1, args):
             "equal_to_plain": same}
 
 
+def volume_form(args):
+    """The volume workload: a 256^3 fBm density grid (threshold 0.4, scale 20)
+    with a 32^3 majorant grid and a 256-entry transfer function in an open box
+    lit by a quad light and the sky environment; delta tracking with NEE ratio
+    tracking, albedo 0.9, 1024x1024, 32 spp, max depth 16.  Plain frame time,
+    then 8 orbit views in one batched launch against 8 plain renders."""
+    import numpy as np
+    import torch
+
+    from photon_amd import abi, scenes
+    from photon_amd.backend import BatchRenderer, HipBackend, PathTracer, make_camera
+
+    t = time.time()
+    s = scenes.fog_box(n=256, macro=32)
+    gen = time.time() - t
+    pt = PathTracer(HipBackend(device=0))
+    pt.set_scene(s)
+
+    def cam_at(ang):
+        eye = np.asarray(s.camera["look_from"], np.float64)
+        at = np.asarray(s.camera["look_at"], np.float64)
+        d = eye - at
+        c, sn = np.cos(ang), np.sin(ang)
+        eye = at + np.array([c * d[0] + sn * d[2], d[1], -sn * d[0] + c * d[2]])
+        return make_camera(eye, at, s.camera["vup"], s.camera["vfov"], 1.0)
+
+    params = abi.default_render_params(1024, 1024, 32, 16)
+    pt.camera, pt.params = cam_at(0.0), params
+    pt.render(read=False)
+    times = []
+    for _ in range(args.volume_reps):
+        pt.render(read=False)
+        times.append(pt.ctx.stats().last_render_ms)
+    ms = min(times)
+    st = pt.ctx.stats()
+    samples = 1024 * 1024 * 32
+    views = [(pt, cam_at(a), params) for a in np.linspace(-0.6, 0.6, 8)]
+    seq = 0.0
+    for p_, cam, prm in views:
+        p_.camera, p_.params = cam, prm
+        p_.render(read=False)
+        seq += p_.ctx.stats().last_render_ms
+    br = BatchRenderer(device=0)
+    br.render(views)
+    torch.cuda.synchronize()
+    br.render(views)
+    bms = br.ctx.stats().last_render_ms
+    print(f"[bench] volume: frame {ms:.1f} ms, 8 views batched {bms:.1f} ms vs {seq:.1f} ms", file=sys.stderr,
+          flush=True)
+    return {"workload": "fog box 256^3 fBm (32^3 majorants), quad light + sky, 1024x1024, 32 spp, max depth 16",
+            "ms": ms, "samples_per_s": samples / (ms * 1e-3),
+            "mrays_per_s": (st.closest_rays + st.shadow_rays) / (ms * 1e-3) / 1e6,
+            "batched_8_views_ms": bms, "sequential_8_views_ms": seq, "batched_speedup": seq / bms,
+            "scene_generation_s": gen}
+
+
 def adjoint_form(args):
     """Path-replay adjoint against its primal on the differentiable workload:
     Cornell box with a textured floor (512^2 RGB albedo, initialised to 0.5)
@@ -162,6 +220,7 @@ def main():
                     help="timed repetitions of the batched 16-view launch and of its 16 plain launches (0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="batched form with the default launch shape")
     ap.add_argument("--adjoint-reps", type=int, default=3, help="timed primal+adjoint passes (0: skip)")
+    ap.add_argument("--volume-reps", type=int, default=2, help="timed frames of the volume workload (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -276,6 +335,7 @@ def main():
 
     batched = batched_form(pt, scene, args) if args.batch_reps > 0 else None
     adjoint = adjoint_form(args) if args.adjoint_reps > 0 else None
+    volume = volume_form(args) if args.volume_reps > 0 else None
 
     line = {
         "metric": "samples/s & Mrays/s at 1024^2 64spp (plain)",
@@ -303,6 +363,7 @@ def main():
         "gpu_baseline": gpu_base,
         "batched": batched,
         "adjoint": adjoint,
+        "volume": volume,
         "bvh_build_s": build_s,
     }
     print(json.dumps(line))

@@ -70,6 +70,21 @@ void env_cdf(const float *px, uint32_t w, uint32_t h, std::vector<float> &marg, 
   }
 }
 
+// The CDF search of env_sample (largest i in [0, n) with cdf[i] <= u) and its
+// guide: the answer at u = b / kEnvGuide for b = 0..kEnvGuide.
+uint32_t cdf_search_host(const float *cdf, uint32_t n, float u) {
+  uint32_t lo = 0, hi = n;
+  while (lo + 1 < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cdf[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+void cdf_guide(const float *cdf, uint32_t n, uint32_t *guide) {
+  for (uint32_t b = 0; b <= kEnvGuide; ++b) guide[b] = cdf_search_host(cdf, n, float(b) / float(kEnvGuide));
+}
+
 bool ray_args_ok(const float *a, const float *b, const float *c, const float *d) { return a && b && c && d; }
 
 }  // namespace
@@ -316,6 +331,12 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
     if ((r = upload(c, c->env_pixels, envpx.data(), envpx.size()))) return r;
     if ((r = upload(c, c->env_marginal, marg.data(), marg.size()))) return r;
     if ((r = upload(c, c->env_cond, cond.data(), cond.size()))) return r;
+    std::vector<uint32_t> gm(kEnvGuide + 1), gc(size_t(d->env_height) * (kEnvGuide + 1));
+    cdf_guide(marg.data(), d->env_height, gm.data());
+    for (uint32_t y = 0; y < d->env_height; ++y)
+      cdf_guide(&cond[size_t(y) * (d->env_width + 1)], d->env_width, &gc[size_t(y) * (kEnvGuide + 1)]);
+    if ((r = upload(c, c->env_guide_m, gm.data(), gm.size()))) return r;
+    if ((r = upload(c, c->env_guide_c, gc.data(), gc.size()))) return r;
     s.has_env = 1;
     s.env_w = d->env_width;
     s.env_h = d->env_height;
@@ -328,6 +349,8 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
   s.env_pixels = c->env_pixels.as<float>();
   s.env_marginal = c->env_marginal.as<float>();
   s.env_cond = c->env_cond.as<float>();
+  s.env_guide_m = c->env_guide_m.as<uint32_t>();
+  s.env_guide_c = c->env_guide_c.as<uint32_t>();
 
   // Texture atlas.
   s.tex_count = 0;

@@ -64,7 +64,8 @@ struct phx_context {
 
   // shading data
   bool have_scene = false;
-  DBuf materials, lights, em_ids, em_cdf, prim_light_pdf, env_pixels, env_marginal, env_cond, tex_pixels, tex_infos,
+  DBuf materials, lights, em_ids, em_cdf, prim_light_pdf, env_pixels, env_marginal, env_cond, env_guide_m, env_guide_c,
+      tex_pixels, tex_infos,
       volumes, vol_values, vol_tf, majorants;
   phx::DevScene dev{};
   uint32_t tex_pixel_count = 0;  // texels of the atlas

@@ -28,6 +28,9 @@ struct alignas(128) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 is one 128-byte line");
 
+// Buckets of the environment CDF search guides (a power of two).
+constexpr uint32_t kEnvGuide = 128;
+
 // Triangles in leaf order: v0, e1 = v1 - v0, e2 = v2 - v0 (the Moller-Trumbore
 // edge vectors, computed with the same float subtraction the reference does)
 // and the primitive id in v0's w.
@@ -54,6 +57,8 @@ struct DevScene {
   const float *env_pixels;
   const float *env_marginal;
   const float *env_cond;
+  const uint32_t *env_guide_m;  // kEnvGuide + 1 search bounds of the marginal CDF
+  const uint32_t *env_guide_c;  // per row: kEnvGuide + 1 bounds of its conditional CDF
   const float *tex_pixels;
   const phx_texture_info *tex_infos;
   const phx_volume *volumes;
@@ -184,26 +189,29 @@ PHX_DEV float env_pdf(const DevScene &s, vec3 dir) {
   const float st = sinf(uv.y * kPi);
   return (rp * cp) * float(s.env_w) * float(s.env_h) / (kTwoPi * kPi * fmaxf(st, 1e-6f));
 }
+// Largest i in [0, n) with cdf[i] <= u (cdf[0] = 0), the search of
+// environment_map.cpp's sampling.  guide[b] is that index for u = b / kEnvGuide,
+// so for u in [b, b + 1) / kEnvGuide the answer lies in [guide[b],
+// guide[b + 1]]: the search starts there and returns the same index in a few
+// steps instead of log2(n).  kEnvGuide is a power of two: u * kEnvGuide and
+// b / kEnvGuide are exact in float.
+PHX_DEV uint32_t cdf_search(const float *cdf, uint32_t n, const uint32_t *guide, float u) {
+  const uint32_t b = min(uint32_t(u * float(kEnvGuide)), kEnvGuide - 1u);
+  uint32_t lo = guide[b], hi = min(guide[b + 1] + 1u, n);
+  while (lo + 1 < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (cdf[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 // Marginal then conditional CDF search, each over a table with a leading 0
 // (environment_map.h:56-115).
 PHX_DEV vec3 env_sample(const DevScene &s, Rng &rng, float &pdf) {
   const float u1 = rng.next(), u2 = rng.next();
-  uint32_t lo = 0, hi = s.env_h;
-  while (lo + 1 < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (s.env_marginal[mid] <= u2) lo = mid;
-    else hi = mid;
-  }
-  const uint32_t row = lo;
+  const uint32_t row = cdf_search(s.env_marginal, s.env_h, s.env_guide_m, u2);
   const float *c = s.env_cond + size_t(row) * (s.env_w + 1);
-  lo = 0;
-  hi = s.env_w;
-  while (lo + 1 < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (c[mid] <= u1) lo = mid;
-    else hi = mid;
-  }
-  const uint32_t col = lo;
+  const uint32_t col = cdf_search(c, s.env_w, s.env_guide_c + size_t(row) * (kEnvGuide + 1), u1);
   const float rp = s.env_marginal[row + 1] - s.env_marginal[row];
   const float cp = c[col + 1] - c[col];
   const float u = (float(col) + rng.next()) / float(s.env_w);

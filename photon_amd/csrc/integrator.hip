@@ -123,7 +123,7 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
 // use); continuing paths and shadow rays go to compacted queues, one atomic
 // per wave.  The grid covers the path slots; threads beyond the queue exit.
 template <class P>
-__global__ __launch_bounds__(256) void k_shade(P v, StreamQueues q) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(P v, StreamQueues q) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[kRayCount + ring(it)];
   if (blockIdx.x * blockDim.x >= count) return;

@@ -60,6 +60,7 @@ constexpr uint32_t kIterBatch = 8;
 constexpr uint32_t kIterationGuard = 1u << 26;
 constexpr uint32_t kDefaultPaths = 1u << 25;
 constexpr uint32_t kDefaultRefill = 16;
+constexpr uint32_t kDefaultChunk = 128;
 constexpr uint32_t kGradParamsAbi = 5;  // base r, g, b, roughness, metallic
 
 // One view of a render or a batch.
@@ -150,7 +151,7 @@ LoopShape loop_shape(const phx_context *c, uint32_t capacity) {
   ls.trace_grid = uint32_t(c->cu_count) * (t.trace_groups_per_cu ? t.trace_groups_per_cu : 8u);
   ls.shade_grid = std::max<uint32_t>(1, (capacity + 255) / 256);
   ls.regen_grid = std::max<uint32_t>(1, std::min<uint32_t>((capacity + 255) / 256, uint32_t(c->cu_count) * 16u));
-  ls.chunk = t.queue_chunk ? t.queue_chunk : kChunk;
+  ls.chunk = t.queue_chunk ? t.queue_chunk : kDefaultChunk;
   ls.refill = t.refill_lanes ? t.refill_lanes : kDefaultRefill;
   return ls;
 }
@@ -532,7 +533,7 @@ extern "C" int phx_autotune(phx_context *c, const phx_view *views, uint32_t coun
   cur.time_kernels = 0;
   if (!cur.trace_block) cur.trace_block = 256;
   if (!cur.trace_groups_per_cu) cur.trace_groups_per_cu = 8;
-  if (!cur.queue_chunk) cur.queue_chunk = kChunk;
+  if (!cur.queue_chunk) cur.queue_chunk = kDefaultChunk;
   if (!cur.refill_lanes) cur.refill_lanes = kDefaultRefill;
   if (!cur.paths_in_flight) cur.paths_in_flight = kDefaultPaths;
   double cur_ms;
