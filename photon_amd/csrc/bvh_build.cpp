@@ -1,5 +1,6 @@
-// Host BVH builder: binned SAH binary tree, collapsed to a 4-wide BVH of
-// 128-byte nodes with triangles in leaf order.
+// Host BVH builder: binned SAH binary tree, collapsed to an 8-wide BVH of
+// 128-byte nodes with 16-bit quantised child boxes and triangles in leaf
+// order.
 //
 // Replaces the reference's Bvh::build_cpu (bvh.cpp:20-125: median split on the
 // widest centroid axis, leaves of <= 4 triangles, 44-byte binary nodes).  The
@@ -179,33 +180,57 @@ struct Builder {
   }
 };
 
-Box empty_box() {
-  Box b;
-  for (int a = 0; a < 3; ++a) b.lo[a] = b.hi[a] = 0.f;
-  return b;
-}
-
-void set_child(Node4 &n, int slot, const Box &b, int code) {
-  float *lo[3] = {&n.lox.x, &n.loy.x, &n.loz.x};
-  float *hi[3] = {&n.hix.x, &n.hiy.x, &n.hiz.x};
+// Quantises the children's boxes into node n's frame: per axis the origin is
+// the node box's low corner and the step a power of two 2^e with
+// org + 65535 * 2^e >= the node's high bound.  Each bound is rounded outward
+// until org + float(q) * 2^e, evaluated in float as the traversal does,
+// contains the child's bound.
+void quantize(BvhNode &n, const Box *kids, int nk) {
+  Box nb;
+  for (int k = 0; k < nk; ++k) nb.grow(kids[k]);
   for (int a = 0; a < 3; ++a) {
-    lo[a][slot] = b.lo[a];
-    hi[a][slot] = b.hi[a];
+    const float org = nb.lo[a];
+    const float ext = nb.hi[a] - org;
+    int e = -100;
+    if (ext > 0.f) e = std::max(-100, int(std::ceil(std::log2(double(ext) / 65535.0))));
+    auto dec = [&](uint32_t q, int ee) {
+      const float s = std::ldexp(1.f, ee);
+      return org + float(q) * s;
+    };
+    while (dec(65535u, e) < nb.hi[a]) ++e;
+    n.org[a] = org;
+    n.ex[a] = int8_t(e);
+    const float s = std::ldexp(1.f, e);
+    for (int k = 0; k < kNodeWidth; ++k) {
+      if (k >= nk) {
+        n.qlo[a][k] = 1;  // empty slots: lo > hi, never reported (k >= nchild)
+        n.qhi[a][k] = 0;
+        continue;
+      }
+      double ql = std::floor(double(kids[k].lo[a] - org) / double(s));
+      double qh = std::ceil(double(kids[k].hi[a] - org) / double(s));
+      uint32_t lo = uint32_t(std::min(std::max(ql, 0.0), 65535.0));
+      uint32_t hi = uint32_t(std::min(std::max(qh, 0.0), 65535.0));
+      while (lo > 0 && dec(lo, e) > kids[k].lo[a]) --lo;
+      while (hi < 65535u && dec(hi, e) < kids[k].hi[a]) ++hi;
+      if (hi < lo) hi = lo;
+      n.qlo[a][k] = uint16_t(lo);
+      n.qhi[a][k] = uint16_t(hi);
+    }
   }
-  (&n.child.x)[slot] = code;
 }
 
 }  // namespace
 
-int build_bvh4(const float *positions, uint32_t vertex_count, const uint32_t *indices, uint32_t triangle_count,
-               const Bvh4Options &opt, Bvh4 &out, std::string &err) {
-  out = Bvh4{};
+int build_bvh(const float *positions, uint32_t vertex_count, const uint32_t *indices, uint32_t triangle_count,
+              const BvhOptions &opt, Bvh &out, std::string &err) {
+  out = Bvh{};
   if (triangle_count == 0) {
     err = "build_accel: the mesh has no triangles";
     return -1;
   }
-  if (opt.max_leaf < 1 || opt.max_leaf > 8) {
-    err = "build_accel: leaf size must be 1..8";
+  if (opt.max_leaf < 1 || opt.max_leaf > 4) {
+    err = "build_accel: leaf size must be 1..4";
     return -1;
   }
   Builder b;
@@ -232,52 +257,44 @@ int build_bvh4(const float *positions, uint32_t vertex_count, const uint32_t *in
   }
   b.build(triangle_count);
 
-  // Triangle records in leaf order (the binary leaves' ranges of `order`).
-  out.tris.resize(triangle_count);
-  for (uint32_t i = 0; i < triangle_count; ++i) {
-    const uint32_t p = b.order[i];
+  // Collapse: each 8-wide node takes its binary node's children and keeps
+  // opening the inner child of largest surface area until it has 8.  Inner
+  // children are numbered consecutively; the leaf children's triangles are
+  // appended in slot order, so a node's leaves are one run of records.
+  auto tri_rec = [&](uint32_t p) {
     const float *v0 = positions + size_t(indices[size_t(p) * 3]) * 3;
     const float *v1 = positions + size_t(indices[size_t(p) * 3 + 1]) * 3;
     const float *v2 = positions + size_t(indices[size_t(p) * 3 + 2]) * 3;
-    TriRec &r = out.tris[i];
+    TriRec r;
     r.v0 = make_float4(v0[0], v0[1], v0[2], 0.f);
     std::memcpy(&r.v0.w, &p, 4);
     r.e1 = make_float4(v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2], 0.f);
     r.e2 = make_float4(v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2], 0.f);
-  }
-
-  // Collapse: each 4-wide node takes its binary node's children and keeps
-  // opening the inner child of largest surface area until it has 4.
+    return r;
+  };
   struct Job {
     int bin;
-    int out;
+    uint32_t out;
     uint32_t depth;
   };
-  auto leaf_code = [&](const BinNode &n) { return int(~((n.first << 3) | (n.count - 1))); };
-  const BinNode &root = b.nodes[0];
-  if (root.left < 0) {
-    // A single leaf: wrap it in one node.
-    Node4 n{};
-    for (int s = 0; s < 4; ++s) set_child(n, s, empty_box(), kEmptyChild);
-    set_child(n, 0, root.box, leaf_code(root));
-    out.nodes.push_back(n);
-    out.depth = 1;
-    out.leaves = 1;
-    return 0;
-  }
-  out.nodes.reserve(b.nodes.size() / 3 + 4);
-  out.nodes.push_back(Node4{});
+  out.tris.reserve(triangle_count);
+  out.nodes.reserve(b.nodes.size() / 6 + 4);
+  out.nodes.push_back(BvhNode{});
   std::vector<Job> jobs{{0, 0, 1}};
   uint32_t leaves = 0, depth = 1;
   while (!jobs.empty()) {
     const Job j = jobs.back();
     jobs.pop_back();
     depth = std::max(depth, j.depth);
-    int kids[4];
+    int kids[kNodeWidth];
     int nk = 0;
-    kids[nk++] = b.nodes[j.bin].left;
-    kids[nk++] = b.nodes[j.bin].right;
-    while (nk < 4) {
+    if (b.nodes[j.bin].left < 0) {
+      kids[nk++] = j.bin;  // the root is a leaf
+    } else {
+      kids[nk++] = b.nodes[j.bin].left;
+      kids[nk++] = b.nodes[j.bin].right;
+    }
+    while (nk < kNodeWidth) {
       int best = -1;
       float best_area = -1.f;
       for (int k = 0; k < nk; ++k) {
@@ -292,20 +309,30 @@ int build_bvh4(const float *positions, uint32_t vertex_count, const uint32_t *in
       kids[best] = b.nodes[open].left;
       kids[nk++] = b.nodes[open].right;
     }
-    Node4 n{};
-    for (int s = 0; s < 4; ++s) set_child(n, s, empty_box(), kEmptyChild);
+    // inner children first, then leaves
+    std::stable_partition(kids, kids + nk, [&](int k) { return b.nodes[k].left >= 0; });
+    BvhNode n{};
+    Box boxes[kNodeWidth];
+    int ninner = 0;
     for (int k = 0; k < nk; ++k) {
-      const BinNode &c = b.nodes[kids[k]];
-      if (c.left < 0) {
-        set_child(n, k, c.box, leaf_code(c));
-        ++leaves;
-      } else {
-        const int idx = int(out.nodes.size());
-        out.nodes.push_back(Node4{});
-        set_child(n, k, c.box, idx);
-        jobs.push_back({kids[k], idx, j.depth + 1});
-      }
+      boxes[k] = b.nodes[kids[k]].box;
+      if (b.nodes[kids[k]].left >= 0) ++ninner;
     }
+    n.nchild = uint8_t(nk);
+    n.ninner = uint8_t(ninner);
+    n.child_base = uint32_t(out.nodes.size());
+    n.tri_base = uint32_t(out.tris.size());
+    for (int k = 0; k < ninner; ++k) {
+      out.nodes.push_back(BvhNode{});
+      jobs.push_back({kids[k], n.child_base + uint32_t(k), j.depth + 1});
+    }
+    for (int k = ninner; k < nk; ++k) {
+      const BinNode &c = b.nodes[kids[k]];
+      n.leaf_cnt = uint16_t(n.leaf_cnt | ((c.count - 1u) << (2 * k)));
+      for (uint32_t i = c.first; i < c.first + c.count; ++i) out.tris.push_back(tri_rec(b.order[i]));
+      ++leaves;
+    }
+    quantize(n, boxes, nk);
     out.nodes[j.out] = n;
   }
   out.depth = depth;

@@ -10,19 +10,19 @@
 
 namespace phx {
 
-struct Bvh4Options {
+struct BvhOptions {
   uint32_t max_leaf = 4;          // triangles per leaf (the reference uses <= 4, bvh.cpp:42)
   int max_binary_depth = 40;      // bound on the binary tree depth before collapsing
 };
 
-struct Bvh4 {
-  std::vector<Node4> nodes;       // nodes[0] is the root
+struct Bvh {
+  std::vector<BvhNode> nodes;     // nodes[0] is the root
   std::vector<TriRec> tris;       // leaf order
-  uint32_t depth = 0;             // levels of 4-wide nodes
+  uint32_t depth = 0;             // levels of 8-wide nodes
   uint32_t leaves = 0;
 };
 
-int build_bvh4(const float *positions, uint32_t vertex_count, const uint32_t *indices, uint32_t triangle_count,
-               const Bvh4Options &opt, Bvh4 &out, std::string &err);
+int build_bvh(const float *positions, uint32_t vertex_count, const uint32_t *indices, uint32_t triangle_count,
+              const BvhOptions &opt, Bvh &out, std::string &err);
 
 }  // namespace phx

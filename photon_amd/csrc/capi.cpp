@@ -169,14 +169,14 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   c->have_accel = false;
   c->have_scene = false;
   c->scene_bytes = 0;
-  Bvh4 bvh;
+  Bvh bvh;
   std::string err;
-  Bvh4Options opt;
-  if (build_bvh4(d->positions, d->vertex_count, d->indices, d->triangle_count, opt, bvh, err) != 0)
+  BvhOptions opt;
+  if (build_bvh(d->positions, d->vertex_count, d->indices, d->triangle_count, opt, bvh, err) != 0)
     return fail(c, PHX_ERR_INVALID, err);
-  // Stack bound: a visit pushes at most 3 entries per level below the root.
-  if (3 * bvh.depth + 1 > uint32_t(kMaxStack))
-    return fail(c, PHX_ERR_OVERFLOW, "build_accel: BVH too deep for the 64-entry traversal stack");
+  // Stack bound: a visit pushes at most kNodeWidth - 1 entries per level.
+  if ((kNodeWidth - 1) * bvh.depth + 1 > uint32_t(kMaxStack))
+    return fail(c, PHX_ERR_OVERFLOW, "build_accel: BVH too deep for the traversal stack");
   c->bvh_depth = bvh.depth;
   int r;
   if ((r = upload(c, c->nodes, bvh.nodes.data(), bvh.nodes.size()))) return r;
@@ -213,7 +213,7 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   std::memcpy(c->bounds_hi, hi, sizeof hi);
   DevScene &s = c->dev;
   s = DevScene{};
-  s.nodes = c->nodes.as<Node4>();
+  s.nodes = c->nodes.as<BvhNode>();
   s.tris = c->tris.as<TriRec>();
   s.indices = c->indices.as<uint32_t>();
   s.positions = c->positions.as<float>();
@@ -223,7 +223,7 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   s.material_ids = c->material_ids.as<uint32_t>();
   s.triangle_count = d->triangle_count;
   c->stats.bvh_nodes = bvh.nodes.size();
-  c->stats.bvh_bytes = bvh.nodes.size() * sizeof(Node4) + bvh.tris.size() * sizeof(TriRec);
+  c->stats.bvh_bytes = bvh.nodes.size() * sizeof(BvhNode) + bvh.tris.size() * sizeof(TriRec);
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->have_accel = true;
   return PHX_OK;

@@ -15,18 +15,32 @@
 
 namespace phx {
 
-// 4-wide BVH node: one 128-byte line.  Child bounds are stored as six float4
-// (lo.x, lo.y, lo.z, hi.x, hi.y, hi.z over the 4 children) so each lane
-// fetches its node with eight 16-byte loads from one cache line.  child[i] >= 0
-// is an inner node, child[i] < 0 a leaf: ~child = (first triangle << 3) |
-// (count - 1).  Unused slots hold kEmptyChild.
-constexpr int kEmptyChild = int(0x80000000u);
-struct alignas(128) Node4 {
-  float4 lox, loy, loz, hix, hiy, hiz;
-  int4 child;
-  int4 pad;
+// 8-wide BVH node in one 128-byte line, child boxes quantised to 16 bits in
+// the node's frame: on axis a, child k spans
+//   [org[a] + float(qlo[a][k]) * 2^ex[a], org[a] + float(qhi[a][k]) * 2^ex[a]]
+// evaluated in float exactly as written (the builder rounds the quantised
+// bounds outward until that evaluation contains the child's box).  Children
+// [0, ninner) are the inner nodes child_base + k; children [ninner, nchild)
+// are leaves whose triangles follow one another from tri_base in slot order,
+// slot k holding ((leaf_cnt >> 2k) & 3) + 1 triangles.
+constexpr int kNodeWidth = 8;
+struct alignas(128) BvhNode {
+  float org[3];
+  int8_t ex[3];
+  uint8_t ninner;
+  uint32_t child_base;
+  uint32_t tri_base;
+  uint16_t leaf_cnt;
+  uint8_t nchild;
+  uint8_t pad0;
+  uint32_t pad1;
+  uint16_t qlo[3][kNodeWidth];  // bytes 32..79: one 16-byte word per axis
+  uint16_t qhi[3][kNodeWidth];  // bytes 80..127
 };
-static_assert(sizeof(Node4) == 128, "Node4 is one 128-byte line");
+static_assert(sizeof(BvhNode) == 128, "BvhNode is one 128-byte line");
+// Stack entry of a leaf: ~((first triangle << 3) | (count - 1)); inner nodes
+// are pushed as their index (>= 0).
+PHX_HD int leaf_code(uint32_t first, uint32_t count) { return int(~((first << 3) | (count - 1u))); }
 
 // Buckets of the environment CDF search guides (a power of two).
 constexpr uint32_t kEnvGuide = 128;
@@ -41,7 +55,7 @@ struct alignas(16) TriRec {
 };
 
 struct DevScene {
-  const Node4 *nodes;
+  const BvhNode *nodes;
   const TriRec *tris;
   const uint32_t *indices;
   const float *positions;

@@ -1,11 +1,11 @@
 // Software BVH traversal for gfx950 (no ray accelerators on CDNA4).
 //
-// One lane traces one ray through the 4-wide BVH.  The traversal stack is a
+// One lane traces one ray through the 8-wide quantised BVH.  The traversal stack is a
 // short per-lane stack in LDS (stack[depth][lane], so a wave's 64 lanes touch
 // 64 consecutive dwords at any depth: no bank conflicts) that spills to a
 // per-thread region of global memory beyond kShortStack entries; the builder
-// guarantees the total never exceeds kMaxStack (the reference's 64-entry
-// stack, mesh_intersector.h:74).
+// guarantees the total never exceeds kMaxStack (the reference's stack has 64
+// entries and silently drops pushes beyond them, mesh_intersector.h:74).
 //
 // Hit semantics are those of the reference's intersect_mesh_bvh
 // (mesh_intersector.h:57-124) and intersect_triangle (triangle_intersect.h:18-52):
@@ -23,7 +23,9 @@
 namespace phx {
 
 constexpr int kShortStack = 16;
-constexpr int kMaxStack = 64;
+// Worst case of an 8-wide tree of depth D: 7 D + 1 entries (the builder
+// rejects trees deeper than that allows).
+constexpr int kMaxStack = 128;
 
 struct TraceHit {
   float t;
@@ -70,29 +72,6 @@ PHX_DEV bool tri_test(const TriRec &r, vec3 o, vec3 d, float tmin, float tmax, f
   return !(t < tmin || t > tmax);
 }
 
-// Slab test of the 4 children.  Entry distances are clamped to [tmin, tmax];
-// the exit is widened by a few ulps so rounding never culls a box the exact
-// test would enter.
-PHX_DEV void box4(const Node4 &n, vec3 o, vec3 inv, float tmin, float tmax, float tn[4], bool hit[4]) {
-  const float lox[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w};
-  const float loy[4] = {n.loy.x, n.loy.y, n.loy.z, n.loy.w};
-  const float loz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w};
-  const float hix[4] = {n.hix.x, n.hix.y, n.hix.z, n.hix.w};
-  const float hiy[4] = {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w};
-  const float hiz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
-  const int ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const float tx0 = (lox[i] - o.x) * inv.x, tx1 = (hix[i] - o.x) * inv.x;
-    const float ty0 = (loy[i] - o.y) * inv.y, ty1 = (hiy[i] - o.y) * inv.y;
-    const float tz0 = (loz[i] - o.z) * inv.z, tz1 = (hiz[i] - o.z) * inv.z;
-    const float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-    const float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-    tn[i] = a;
-    hit[i] = a <= b * 1.0000004f && ch[i] != kEmptyChild;
-  }
-}
-
 PHX_DEV void cswap(float &ka, int &va, float &kb, int &vb) {
   if (kb < ka) {
     const float k = ka;
@@ -102,6 +81,133 @@ PHX_DEV void cswap(float &ka, int &va, float &kb, int &vb) {
     va = vb;
     vb = v;
   }
+}
+
+// Slab test of a node's children with the dequantised boxes.  Entry
+// distances are clamped to [tmin, tmax]; the exit is widened by a few ulps so
+// rounding never culls a box the exact test would enter.  Returns the number
+// of children entered; codes[0..count) are their stack entries, nearest first.
+template <bool SORT>
+PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float tmin, float tmax,
+                       int codes[kNodeWidth], float key[kNodeWidth]) {
+  // the node as eight 16-byte loads of one line, unpacked in registers
+  const uint4 *src = reinterpret_cast<const uint4 *>(nodes + node);
+  uint4 w[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = src[i];
+  const vec3 org{__uint_as_float(w[0].x), __uint_as_float(w[0].y), __uint_as_float(w[0].z)};
+  const int ex = int(int8_t(w[0].w & 0xffu)), ey = int(int8_t((w[0].w >> 8) & 0xffu)),
+            ez = int(int8_t((w[0].w >> 16) & 0xffu));
+  const int ninner = int(w[0].w >> 24);
+  const uint32_t child_base = w[1].x, tri_base = w[1].y;
+  const uint32_t leaf_cnt = w[1].z & 0xffffu, nchild = (w[1].z >> 16) & 0xffu;
+  const float sx = ldexpf(1.f, ex), sy = ldexpf(1.f, ey), sz = ldexpf(1.f, ez);
+  auto q16 = [](const uint4 &v, int k) {
+    const uint32_t d = k < 2 ? v.x : (k < 4 ? v.y : (k < 6 ? v.z : v.w));
+    return float((k & 1) ? (d >> 16) : (d & 0xffffu));
+  };
+  uint32_t leaf_first = tri_base;
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < kNodeWidth; ++k) {
+    const float lox = org.x + q16(w[2], k) * sx, hix = org.x + q16(w[5], k) * sx;
+    const float loy = org.y + q16(w[3], k) * sy, hiy = org.y + q16(w[6], k) * sy;
+    const float loz = org.z + q16(w[4], k) * sz, hiz = org.z + q16(w[7], k) * sz;
+    const float tx0 = (lox - o.x) * inv.x, tx1 = (hix - o.x) * inv.x;
+    const float ty0 = (loy - o.y) * inv.y, ty1 = (hiy - o.y) * inv.y;
+    const float tz0 = (loz - o.z) * inv.z, tz1 = (hiz - o.z) * inv.z;
+    const float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    const float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    const bool valid = k < int(nchild);
+    const bool hit = valid && a <= b * 1.0000004f;
+    int code;
+    if (k < ninner) {
+      code = int(child_base) + k;
+    } else {
+      const uint32_t c = ((leaf_cnt >> (2 * k)) & 3u) + 1u;
+      code = leaf_code(leaf_first, c);
+      if (valid) leaf_first += c;
+    }
+    key[k] = hit ? a : INFINITY;
+    codes[k] = code;
+    cnt += hit ? 1 : 0;
+  }
+  if (!SORT) return cnt;  // any hit: children stay in slot order (descend_unsorted)
+  // Batcher's odd-even merge network on 8 (key, code) pairs.
+  cswap(key[0], codes[0], key[1], codes[1]);
+  cswap(key[2], codes[2], key[3], codes[3]);
+  cswap(key[4], codes[4], key[5], codes[5]);
+  cswap(key[6], codes[6], key[7], codes[7]);
+  cswap(key[0], codes[0], key[2], codes[2]);
+  cswap(key[1], codes[1], key[3], codes[3]);
+  cswap(key[4], codes[4], key[6], codes[6]);
+  cswap(key[5], codes[5], key[7], codes[7]);
+  cswap(key[1], codes[1], key[2], codes[2]);
+  cswap(key[5], codes[5], key[6], codes[6]);
+  cswap(key[0], codes[0], key[4], codes[4]);
+  cswap(key[1], codes[1], key[5], codes[5]);
+  cswap(key[2], codes[2], key[6], codes[6]);
+  cswap(key[3], codes[3], key[7], codes[7]);
+  cswap(key[2], codes[2], key[4], codes[4]);
+  cswap(key[3], codes[3], key[5], codes[5]);
+  cswap(key[1], codes[1], key[2], codes[2]);
+  cswap(key[3], codes[3], key[4], codes[4]);
+  cswap(key[5], codes[5], key[6], codes[6]);
+  return cnt;
+}
+
+// Pushes codes[1..cnt) farthest first; the nearest becomes the next node.
+PHX_DEV int descend(LaneStack &st, const int codes[kNodeWidth], int cnt) {
+#pragma unroll
+  for (int k = kNodeWidth - 1; k >= 1; --k)
+    if (k < cnt) st.push(codes[k]);
+  return codes[0];
+}
+// Children in slot order with key INFINITY for the missed ones: the nearest
+// entered child becomes the next node, the others are pushed in slot order.
+PHX_DEV int descend_unsorted(LaneStack &st, const int codes[kNodeWidth], const float key[kNodeWidth]) {
+  float bk = INFINITY;
+  int bc = 0, bi = -1;
+#pragma unroll
+  for (int k = 0; k < kNodeWidth; ++k) {
+    const bool better = key[k] < bk;
+    bk = better ? key[k] : bk;
+    bc = better ? codes[k] : bc;
+    bi = better ? k : bi;
+  }
+#pragma unroll
+  for (int k = kNodeWidth - 1; k >= 0; --k)
+    if (key[k] != INFINITY && k != bi) st.push(codes[k]);
+  return bc;
+}
+template <bool SORT>
+PHX_DEV int descend_any(LaneStack &st, const int codes[kNodeWidth], const float key[kNodeWidth], int cnt) {
+  return SORT ? descend(st, codes, cnt) : descend_unsorted(st, codes, key);
+}
+
+// The triangles of a leaf entry against the ray: updates the closest hit h
+// (tie rule: equal t goes to the larger primitive id); returns true on a hit
+// when ANY (any hit ends the ray).
+template <bool ANY>
+PHX_DEV bool visit_leaf(const TriRec *tris, int node, vec3 o, vec3 d, float tmin, float &best, TraceHit &h) {
+  const uint32_t code = ~uint32_t(node);
+  const uint32_t first = code >> 3, count = (code & 7u) + 1u;
+  for (uint32_t k = 0; k < count; ++k) {
+    const TriRec r = tris[first + k];
+    float t, u, v;
+    if (!tri_test(r, o, d, tmin, best, t, u, v)) continue;
+    const uint32_t prim = __float_as_uint(r.v0.w);
+    if (t < best || prim > h.prim || h.tri == 0xffffffffu) {
+      best = t;
+      h.t = t;
+      h.u = u;
+      h.v = v;
+      h.tri = first + k;
+      h.prim = prim;
+      if (ANY) return true;
+    }
+  }
+  return false;
 }
 
 // Closest hit (ANY = false) or any hit (ANY = true).  Returns true on a hit.
@@ -118,45 +224,15 @@ PHX_DEV bool traverse(const DevScene &s, vec3 o, vec3 d, float tmin, float tmax,
   st.sp = 0;
   while (true) {
     if (node >= 0) {
-      const Node4 n = s.nodes[node];
-      float tn[4];
-      bool hb[4];
-      box4(n, o, inv, tmin, best, tn, hb);
-      float k0 = hb[0] ? tn[0] : INFINITY, k1 = hb[1] ? tn[1] : INFINITY;
-      float k2 = hb[2] ? tn[2] : INFINITY, k3 = hb[3] ? tn[3] : INFINITY;
-      int c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
-      const int cnt = int(hb[0]) + int(hb[1]) + int(hb[2]) + int(hb[3]);
+      int codes[kNodeWidth];
+      float key[kNodeWidth];
+      const int cnt = visit_node<!ANY>(s.nodes, node, o, inv, tmin, best, codes, key);
       if (cnt > 0) {
-        // sort the 4 (distance, child) pairs, misses last
-        cswap(k0, c0, k1, c1);
-        cswap(k2, c2, k3, c3);
-        cswap(k0, c0, k2, c2);
-        cswap(k1, c1, k3, c3);
-        cswap(k1, c1, k2, c2);
-        if (cnt > 3) st.push(c3);
-        if (cnt > 2) st.push(c2);
-        if (cnt > 1) st.push(c1);
-        node = c0;
+        node = descend_any<!ANY>(st, codes, key, cnt);
         continue;
       }
-    } else {
-      const uint32_t code = ~uint32_t(node);
-      const uint32_t first = code >> 3, count = (code & 7u) + 1u;
-      for (uint32_t k = 0; k < count; ++k) {
-        const TriRec r = s.tris[first + k];
-        float t, u, v;
-        if (!tri_test(r, o, d, tmin, best, t, u, v)) continue;
-        const uint32_t prim = __float_as_uint(r.v0.w);
-        if (t < best || prim > h.prim || h.tri == 0xffffffffu) {
-          best = t;
-          h.t = t;
-          h.u = u;
-          h.v = v;
-          h.tri = first + k;
-          h.prim = prim;
-          if (ANY) return true;
-        }
-      }
+    } else if (visit_leaf<ANY>(s.tris, node, o, d, tmin, best, h)) {
+      return true;
     }
     if (st.sp == 0) break;
     node = st.pop();
@@ -186,7 +262,7 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *counter, uint32_t count, u
   vec3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 1.f}, inv{0.f, 0.f, 0.f};
   float tmin = 0.f, best = 0.f;
   int node = 0;
-  const Node4 *nodes = nullptr;
+  const BvhNode *nodes = nullptr;
   const TriRec *tris = nullptr;
   TraceHit h;
   h.tri = h.prim = 0xffffffffu;
@@ -244,47 +320,15 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *counter, uint32_t count, u
       bool finished = false;
       bool pop = true;
       if (node >= 0) {
-        const Node4 n = nodes[node];
-        float tn[4];
-        bool hb[4];
-        box4(n, o, inv, tmin, best, tn, hb);
-        float k0 = hb[0] ? tn[0] : INFINITY, k1 = hb[1] ? tn[1] : INFINITY;
-        float k2 = hb[2] ? tn[2] : INFINITY, k3 = hb[3] ? tn[3] : INFINITY;
-        int c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
-        const int cnt = int(hb[0]) + int(hb[1]) + int(hb[2]) + int(hb[3]);
+        int codes[kNodeWidth];
+        float key[kNodeWidth];
+        const int cnt = visit_node<!ANY>(nodes, node, o, inv, tmin, best, codes, key);
         if (cnt > 0) {
-          cswap(k0, c0, k1, c1);
-          cswap(k2, c2, k3, c3);
-          cswap(k0, c0, k2, c2);
-          cswap(k1, c1, k3, c3);
-          cswap(k1, c1, k2, c2);
-          if (cnt > 3) st.push(c3);
-          if (cnt > 2) st.push(c2);
-          if (cnt > 1) st.push(c1);
-          node = c0;
+          node = descend_any<!ANY>(st, codes, key, cnt);
           pop = false;
         }
       } else {
-        const uint32_t code = ~uint32_t(node);
-        const uint32_t first = code >> 3, cnt = (code & 7u) + 1u;
-        for (uint32_t k = 0; k < cnt; ++k) {
-          const TriRec r = tris[first + k];
-          float t, u, v;
-          if (!tri_test(r, o, d, tmin, best, t, u, v)) continue;
-          const uint32_t prim = __float_as_uint(r.v0.w);
-          if (t < best || prim > h.prim || h.tri == 0xffffffffu) {
-            best = t;
-            h.t = t;
-            h.u = u;
-            h.v = v;
-            h.tri = first + k;
-            h.prim = prim;
-            if (ANY) {
-              finished = true;
-              break;
-            }
-          }
-        }
+        finished = visit_leaf<ANY>(tris, node, o, d, tmin, best, h);
       }
       if (pop && !finished) {
         if (st.sp == 0) finished = true;

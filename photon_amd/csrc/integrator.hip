@@ -80,7 +80,7 @@ __global__ void k_commit(StreamQueues q, uint32_t ahead) {
 // The acceleration structure of a lane's ray: the kernel-argument scene, or the
 // ray's view's scene-table entry.
 template <class P>
-PHX_DEV void ray_accel(const P &v, uint32_t g, const Node4 *&nodes, const TriRec *&tris) {
+PHX_DEV void ray_accel(const P &v, uint32_t g, const BvhNode *&nodes, const TriRec *&tris) {
   if constexpr (std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value) {
     nodes = v.scene.nodes;
     tris = v.scene.tris;
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
   uint32_t live = 0;
   traverse_stream<false>(
       st, &q.counters[kFetchExtend + ring(it)], count, chunk, refill,
-      [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const Node4 *&nodes, const TriRec *&tris) {
+      [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const BvhNode *&nodes, const TriRec *&tris) {
         const float4 a = q.o[buf][i], b = q.d[buf][i];
         o = xyz(a);
         d = xyz(b);
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chun
   uint32_t live = 0;
   traverse_stream<true>(
       st, &q.counters[kFetchShadow + ring(it)], count, chunk, refill,
-      [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const Node4 *&nodes, const TriRec *&tris) {
+      [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const BvhNode *&nodes, const TriRec *&tris) {
         const float4 a = q.sh_o[i], b = q.sh_d[i];
         o = xyz(a);
         d = xyz(b);

@@ -20,7 +20,7 @@ __global__ __launch_bounds__(256) void k_trace(DevScene s, const float *org, con
   LaneStack st{&lds[threadIdx.x], 256, spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   traverse_stream<MODE == 2>(
       st, fetch, n, kChunk, kRefill,
-      [&](uint32_t i, vec3 &o, vec3 &d, float &t0, float &t1, const Node4 *&nodes, const TriRec *&tris) {
+      [&](uint32_t i, vec3 &o, vec3 &d, float &t0, float &t1, const BvhNode *&nodes, const TriRec *&tris) {
         o = ldv(org, i);
         d = ldv(dir, i);
         t0 = tmin[i];
