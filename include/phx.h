@@ -257,7 +257,7 @@ typedef struct phx_tuning {
   uint32_t refill_lanes;         /* idle lanes of a traversal wave that make it take new rays, 1..64 (16) */
   uint32_t queue_chunk;          /* rays a wave takes per queue fetch, 64..4096, multiple of 64 (128) */
   uint32_t paths_in_flight;      /* path slots of the wavefront (2^25) */
-  uint32_t sort_by_material;     /* reserved, must be 0 */
+  uint32_t sort_by_material;     /* material groups of sorted shading, 0..15 (0: queue order) */
   uint32_t use_graph;            /* 1: phx_render_batch replays its bounce loop as a captured hipGraph */
   uint32_t time_kernels;         /* 1: time each kernel launch with HIP events (phx_stats) */
 } phx_tuning;
@@ -351,7 +351,7 @@ int phx_render_batch(phx_context *ctx, const phx_view *views, uint32_t count);
 int phx_read_batch_result(phx_context *ctx, uint32_t view, float *color_rgb, float *alpha, float *depth,
                           float *normal_xyz, float *albedo_rgb);
 /* Times phx_render_batch of `views` under a set of launch shapes (traversal
- * work-group size and groups per CU, queue chunk, refill threshold, path slots) and keeps the fastest in ctx's tuning.  `best` (optional) receives it,
+ * work-group size and groups per CU, queue chunk, refill threshold, material groups, path slots) and keeps the fastest in ctx's tuning.  `best` (optional) receives it,
  * `best_ms` (optional) its time.  Each candidate runs `repeats` times (>= 1). */
 int phx_autotune(phx_context *ctx, const phx_view *views, uint32_t count, uint32_t repeats, phx_tuning *best,
                  double *best_ms);

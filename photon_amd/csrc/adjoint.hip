@@ -132,7 +132,7 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
       const DevScene &s = v.scene_of(r);
       const PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], stv);
       const float4 hit = q.hit[i];
-      const uint32_t htri = __float_as_uint(hit.w);
+      const uint32_t htri = hit_tri(__float_as_uint(hit.w));
       const uint32_t l = g - r.first;
       const uint32_t pixel = l % r.sp.npix;
       const uint32_t sample = r.sample_base + l / r.sp.npix;

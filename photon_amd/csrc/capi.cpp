@@ -140,7 +140,7 @@ int phx_set_tuning(phx_context *c, const phx_tuning *t) {
   if (t->queue_chunk != 0 && (t->queue_chunk % 64 != 0 || t->queue_chunk > 4096))
     return fail(c, PHX_ERR_INVALID, "set_tuning: queue_chunk must be a multiple of 64 up to 4096");
   if (t->paths_in_flight > (1u << 28)) return fail(c, PHX_ERR_INVALID, "set_tuning: more than 2^28 path slots");
-  if (t->sort_by_material != 0) return fail(c, PHX_ERR_UNSUPPORTED, "set_tuning: sort_by_material is reserved");
+  if (t->sort_by_material > kMaxGroups) return fail(c, PHX_ERR_INVALID, "set_tuning: at most 15 material groups");
   c->tuning = *t;
   return PHX_OK;
 }

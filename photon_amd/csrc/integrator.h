@@ -30,6 +30,12 @@ constexpr uint32_t kIter = kNextSample + 1;
 constexpr uint32_t kTotal = kIter + 1;
 constexpr uint32_t kCapacity = kTotal + 1;
 constexpr uint32_t kCounterWords = kCapacity + 1;
+// Shading by material group: the hit record's top 4 bits carry the hit's
+// group (material id mod the group count) and k_shade orders each work-group's
+// rays by group (misses last) before shading them.
+constexpr uint32_t kMaxGroups = 15;
+constexpr uint32_t kMaxClasses = kMaxGroups + 1;
+__host__ __device__ inline uint32_t hit_tri(uint32_t w) { return w == 0xffffffffu ? w : (w & 0x0fffffffu); }
 __host__ __device__ inline uint32_t ring(uint32_t it) { return it & (kRing - 1); }
 
 constexpr uint32_t kStatClosest = 0;
@@ -117,6 +123,7 @@ struct LoopShape {
   uint32_t regen_grid;   // work-groups (of 256) of the regeneration kernel
   uint32_t chunk;        // work items a wave takes per fetch (multiple of 64)
   uint32_t refill;       // idle lanes of a traversal wave that trigger a refill (1..64)
+  uint32_t groups;       // material groups of sorted shading (0: queue order)
 };
 
 // RenderParams -> the shading constants of one frame (render.cpp).
@@ -137,6 +144,7 @@ hipError_t launch_run_start(const StreamQueues &q, uint32_t total, uint32_t capa
 hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead, const LoopShape &ls,
                         hipStream_t st, bool zero_L = true);
 hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
+// With ls.groups > 0 the rays of each work-group are shaded in material-group order.
 hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 // Sums the item's samples of each pixel into accum (first: overwrite).

@@ -27,8 +27,8 @@ namespace phx {
 
 // Clears the counters of a run and records its size.
 __global__ void k_run_start(StreamQueues q, uint32_t total, uint32_t capacity) {
-  const uint32_t i = threadIdx.x;
-  if (i < kCounterWords) q.counters[i] = i == kTotal ? total : (i == kCapacity ? capacity : 0u);
+  for (uint32_t i = threadIdx.x; i < kCounterWords; i += blockDim.x)
+    q.counters[i] = i == kTotal ? total : (i == kCapacity ? capacity : 0u);
 }
 
 // Fills the free slots of a queue with new camera paths (pt_init_sample).
@@ -91,10 +91,24 @@ PHX_DEV void ray_accel(const P &v, uint32_t g, const BvhNode *&nodes, const TriR
   }
 }
 
+// Material id of primitive `prim` in the scene of the ray with sample id g.
+template <class P>
+PHX_DEV uint32_t ray_material(const P &v, uint32_t g, uint32_t prim) {
+  const uint32_t *mids;
+  if constexpr (std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value) {
+    mids = v.scene.material_ids;
+  } else {
+    mids = v.scene_of(v.item_at(v.find(g))).material_ids;
+  }
+  return mids ? mids[prim] : 0u;
+}
+
 // Closest hit over the ray queue of the current iteration (persistent waves,
-// per-lane refill).
+// per-lane refill).  With groups > 0 the hit record's top 4 bits carry the
+// hit's material group.
 template <class P, int B>
-__global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chunk, uint32_t refill) {
+__global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chunk, uint32_t refill,
+                                              uint32_t groups) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[kRayCount + ring(it)];
   if (count == 0) return;
@@ -114,7 +128,11 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
         live += tmax > tmin ? 1u : 0u;
         ray_accel(v, q.st[buf][i].x, nodes, tris);
       },
-      [&](uint32_t i, bool, const TraceHit &h) { q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.tri)); });
+      [&](uint32_t i, bool hit, const TraceHit &h) {
+        uint32_t w = h.tri;
+        if (hit && groups) w |= (ray_material(v, q.st[buf][i].x, h.prim) % groups) << 28;
+        q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(w));
+      });
   wave_sum(&q.stats[kStatClosest], live);
 }
 
@@ -122,13 +140,42 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
 // keep the shading constants live across iterations and double the register
 // use); continuing paths and shadow rays go to compacted queues, one atomic
 // per wave.  The grid covers the path slots; threads beyond the queue exit.
-template <class P>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(P v, StreamQueues q) {
+template <class P, bool SORTED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(P v, StreamQueues q,
+                                                                                      uint32_t groups) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[kRayCount + ring(it)];
   if (blockIdx.x * blockDim.x >= count) return;
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < count;
+  if (SORTED) {
+    // counting sort of the work-group's 256 rays by material group (misses
+    // last) in LDS: each wave then shades one or two groups, reading rays
+    // from the group's own 4 KB range of the queue
+    __shared__ uint32_t s_cnt[kMaxClasses + 1];
+    __shared__ uint32_t s_perm[256];
+    if (threadIdx.x <= kMaxClasses) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    uint32_t key = kMaxClasses;
+    if (valid) {
+      const uint32_t w = __float_as_uint(q.hit[i].w);
+      key = w == 0xffffffffu ? groups : (w >> 28);
+    }
+    const uint32_t rank = atomicAdd(&s_cnt[key], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t acc = 0;
+      for (uint32_t c = 0; c <= kMaxClasses; ++c) {
+        const uint32_t n = s_cnt[c];
+        s_cnt[c] = acc;
+        acc += n;
+      }
+    }
+    __syncthreads();
+    s_perm[s_cnt[key] + rank] = threadIdx.x;
+    __syncthreads();
+    i = blockIdx.x * blockDim.x + s_perm[threadIdx.x];
+  }
   const int buf = it & 1, nbuf = buf ^ 1;
   ShadeOut out;
   out.cont = false;
@@ -141,7 +188,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const DevScene &s = v.scene_of(r);
     const PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], stv);
     const float4 hit = q.hit[i];
-    const uint32_t htri = __float_as_uint(hit.w);
+    const uint32_t htri = hit_tri(__float_as_uint(hit.w));
     const uint32_t l = g - r.first;
     const uint32_t pixel = l % r.sp.npix;
     const uint32_t s_local = l / r.sp.npix;
@@ -370,16 +417,22 @@ hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopSha
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
     if (ls.trace_block == 128)
-      hipLaunchKernelGGL((k_extend<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk, ls.refill);
+      hipLaunchKernelGGL((k_extend<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk, ls.refill,
+                         ls.groups);
     else
-      hipLaunchKernelGGL((k_extend<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill);
+      hipLaunchKernelGGL((k_extend<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill,
+                         ls.groups);
     return hipGetLastError();
   });
 }
 hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st) {
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
-    hipLaunchKernelGGL(k_shade<P>, dim3(ls.shade_grid), dim3(256), 0, st, p, q);
+    if (ls.groups) {
+      hipLaunchKernelGGL((k_shade<P, true>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, ls.groups);
+    } else {
+      hipLaunchKernelGGL((k_shade<P, false>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, 0u);
+    }
     return hipGetLastError();
   });
 }

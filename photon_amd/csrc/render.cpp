@@ -153,6 +153,7 @@ LoopShape loop_shape(const phx_context *c, uint32_t capacity) {
   ls.regen_grid = std::max<uint32_t>(1, std::min<uint32_t>((capacity + 255) / 256, uint32_t(c->cu_count) * 16u));
   ls.chunk = t.queue_chunk ? t.queue_chunk : kDefaultChunk;
   ls.refill = t.refill_lanes ? t.refill_lanes : kDefaultRefill;
+  ls.groups = std::min<uint32_t>(t.sort_by_material, kMaxGroups);
   return ls;
 }
 
@@ -548,6 +549,7 @@ extern "C" int phx_autotune(phx_context *c, const phx_view *views, uint32_t coun
       {&phx_tuning::trace_groups_per_cu, {4, 8, 16, 32}},
       {&phx_tuning::queue_chunk, {64, 128, 256, 512, 1024}},
       {&phx_tuning::refill_lanes, {8, 16, 32, 64}},
+      {&phx_tuning::sort_by_material, {0, 4, 8, 15}},
       {&phx_tuning::paths_in_flight, {1u << 23, 1u << 24, 1u << 25}},
   };
   for (const auto &k : knobs) {

@@ -99,10 +99,13 @@ def test_launch_shape_does_not_change_results(batch_setup):
     pt, cam, p = views[1]
     pt.camera, pt.params = cam, p
     for kw in (dict(trace_block=128), dict(queue_chunk=64, trace_groups_per_cu=3), dict(paths_in_flight=777),
-               dict(trace_block=256, queue_chunk=1024, trace_groups_per_cu=16, paths_in_flight=0)):
+               dict(refill_lanes=1), dict(refill_lanes=64), dict(sort_by_material=4), dict(sort_by_material=15),
+               dict(trace_block=256, queue_chunk=1024, trace_groups_per_cu=16, paths_in_flight=0, refill_lanes=0,
+                    sort_by_material=0)):
         pt.ctx.set_tuning(**kw)
         _check_equal(pt.render(), plain[1], f"tuning {kw}")
-    pt.ctx.set_tuning(trace_block=0, queue_chunk=0, trace_groups_per_cu=0, paths_in_flight=0)
+    pt.ctx.set_tuning(trace_block=0, queue_chunk=0, trace_groups_per_cu=0, paths_in_flight=0, refill_lanes=0,
+                      sort_by_material=0)
 
 
 def test_autotune_keeps_a_valid_shape(batch_setup):
