@@ -104,7 +104,7 @@ PHX_DEV void grad_route(const float g[kGradParams], const AdjHooks &h, const Dev
 template <class P, int B>
 __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, AdjointArgs a, uint32_t chunk) {
   const uint32_t it = q.counters[kIter];
-  const uint32_t count = q.counters[kRayCount + ring(it)];
+  const uint32_t count = q.counters[ray_count_word(it)];
   if (count == 0) return;
   __shared__ int lds[kShortStack * B];
   __shared__ double glds[kAdjLdsMaterials * kGradParams];
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
       if (h.nee_on || h.bsdf_on) grad_route(gv, h, s, a, glds);
       if (out.next.path_it >= r.sp.max_it) out.cont = false;
     }
-    const uint32_t ci = wave_alloc(&q.counters[kRayCount + ring(it + 1)], out.cont);
+    const uint32_t ci = wave_alloc(&q.counters[ray_count_word(it + 1)], out.cont);
     if (out.cont) {
       const PathState &n = out.next;
       q.o[nbuf][ci] = f4(n.ro, n.tmin);

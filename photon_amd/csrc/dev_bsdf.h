@@ -84,7 +84,8 @@ PHX_DEV bool spec_delta(const phx_material &m) { return m.roughness < 0.001f; }
 struct LobeW {
   float d, s, t, c;
 };
-PHX_DEV LobeW lobe_weights(const phx_material &m, float NoV) {
+// Lobe selection weights; F0 and F90 are spec_f0_f90(m).
+PHX_DEV LobeW lobe_weights(const phx_material &m, float NoV, vec3 F0, vec3 F90) {
   LobeW w;
   const float opaque = 1.f - saturate(m.transmission);
   const float diel = 1.f - saturate(m.metallic);
@@ -92,8 +93,6 @@ PHX_DEV LobeW lobe_weights(const phx_material &m, float NoV) {
   const vec3 base = base_of(m);
   w.d = de > 0.f ? de * fmaxf(luminance(base), 0.05f) : 0.f;
   if (m.sheen > 0.f && diel * opaque > 0.f) w.d += 0.1f * m.sheen;
-  vec3 F0, F90;
-  spec_f0_f90(m, F0, F90);
   const float m5 = schlick_m5(NoV);
   const vec3 F = F0 + (F90 - F0) * m5;
   const float spec = luminance(F);
@@ -105,6 +104,11 @@ PHX_DEV LobeW lobe_weights(const phx_material &m, float NoV) {
     w.c = fmaxf(m.clearcoat * 0.25f * (0.04f + 0.96f * m2 * m2 * mm), 0.02f);
   }
   return w;
+}
+PHX_DEV LobeW lobe_weights(const phx_material &m, float NoV) {
+  vec3 F0, F90;
+  spec_f0_f90(m, F0, F90);
+  return lobe_weights(m, NoV, F0, F90);
 }
 
 PHX_DEV vec3 eval_diffuse(const phx_material &m, float NoL, float NoV) {
@@ -207,6 +211,75 @@ PHX_DEV float bsdf_pdf(const phx_material &m, vec3 wo, vec3 wi, vec3 n, vec3 sn)
   }
   return fmaxf(pdf, 0.f);
 }
+// bsdf_eval and bsdf_pdf of one direction in one pass: the half vector, the
+// GGX distributions and the Fresnel endpoints are computed once.  Every value
+// is rounded exactly as in the two separate functions above.
+PHX_DEV vec3 bsdf_eval_pdf(const phx_material &m, vec3 wo, vec3 wi, vec3 n, vec3 sn, float &pdf) {
+  pdf = 0.f;
+  if (dot(wo, n) <= 0.f || dot(wi, n) <= 0.f) return {0.f, 0.f, 0.f};
+  const float NoL = saturate(dot(sn, wi)), NoV = saturate(dot(sn, wo));
+  if (NoL <= 0.f || NoV <= 0.f) return {0.f, 0.f, 0.f};
+  const float a = fmaxf(m.roughness, 0.001f);
+  const bool delta = spec_delta(m);
+  const vec3 h = normalize(wi + wo);
+  const float NoH = saturate(dot(sn, h)), VoH = saturate(dot(wo, h));
+  const float m5 = schlick_m5(VoH);
+  vec3 F0, F90;
+  spec_f0_f90(m, F0, F90);
+  vec3 f = eval_diffuse(m, NoL, NoV);
+  float D_spec = -1.f;  // the isotropic GGX D of the specular lobe, once known
+  if (!delta) {
+    const vec3 F = F0 + (F90 - F0) * m5;
+    float D, G;
+    if (m.anisotropic > 0.f) {
+      const float aspect = sqrtf(1.f - 0.9f * m.anisotropic);
+      const float ax = fmaxf(a / aspect, 0.001f), ay = fmaxf(a * aspect, 0.001f);
+      vec3 t, b;
+      onb(sn, t, b);
+      D = D_ggx_aniso(NoH, dot(t, h), dot(b, h), ax, ay);
+      G = sqrtf(G2(NoV, NoL, ax) * G2(NoV, NoL, ay));
+    } else {
+      D = D_ggx(NoH, a);
+      G = G2(NoV, NoL, a);
+      D_spec = D;
+    }
+    f = f + F * (D * G / (4.f * fmaxf(NoV, 1e-7f)));
+  }
+  if (m.sheen > 0.f && m.metallic < 1.f) {
+    const float HoL = fabsf(dot(h, wi));
+    const vec3 base = base_of(m);
+    const float lum = luminance(base);
+    const vec3 tint = lum > 0.f ? base * (1.f / lum) : vec3{1.f, 1.f, 1.f};
+    const vec3 cs = lerp(vec3{1.f, 1.f, 1.f}, tint, m.sheen_tint);
+    const float mm = 1.f - HoL, m2 = mm * mm, sm5 = m2 * m2 * mm;
+    const float w = (1.f - m.metallic) * (1.f - m.transmission);
+    f = f + cs * (m.sheen * sm5 * w * NoL);
+  }
+  float D_cc = 0.f;
+  const float a_cc = fmaxf(m.clearcoat_roughness, 0.001f);
+  if (m.clearcoat > 0.f) {
+    const float F0c = dielectric_f0(1.5f);
+    const float F = F0c + (1.f - F0c) * m5;
+    D_cc = D_ggx(NoH, a_cc);
+    const float G = G2(NoV, NoL, a_cc);
+    const float w = m.clearcoat * 0.25f;
+    const float sc = w * D_cc * G / (4.f * fmaxf(NoV, 1e-7f));
+    f = f + vec3{F * sc, F * sc, F * sc};
+  }
+  const LobeW w = lobe_weights(m, NoV, F0, F90);
+  const float sum = w.d + w.s + w.t + w.c;
+  if (sum <= 0.f) return f;
+  const float pd = w.d / sum, ps = w.s / sum, pc = w.c / sum;
+  float p = 0.f;
+  if (pd > 0.f) p += pd * (NoL * kInvPi);
+  if (ps > 0.f && !delta) {
+    const float D = D_spec >= 0.f ? D_spec : D_ggx(NoH, a);
+    p += ps * (D * NoH / (4.f * fmaxf(VoH, 1e-7f)));
+  }
+  if (pc > 0.f) p += pc * (D_cc * NoH / (4.f * fmaxf(VoH, 1e-7f)));
+  pdf = fmaxf(p, 0.f);
+  return f;
+}
 PHX_DEV vec3 ggx_half(vec3 n, float a, Rng &rng) {
   vec3 t, b;
   onb(n, t, b);
@@ -236,15 +309,15 @@ PHX_DEV BsdfSample bsdf_sample(const phx_material &m, vec3 wo, vec3 n, vec3 sn, 
   if (sum <= 0.f) return out;
   const float pd = w.d / sum, ps = w.s / sum, pt = w.t / sum, pc = w.c / sum;
   const float xi = rng.next();
+  // The reflection lobes only choose the direction; f and pdf are evaluated
+  // once below, so a wave whose lanes chose different lobes evaluates once.
+  bool evaluate = false;
+  const BsdfSample none{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false, false};
   if (xi < pd) {
     out.diffuse = true;
     out.wi = cosine_hemisphere(sn, rng);
-    out.f = bsdf_eval(m, wo, out.wi, n, sn);
-    out.pdf = bsdf_pdf(m, wo, out.wi, n, sn);
-    return out;
-  }
-  const BsdfSample none{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false, false};
-  if (xi < pd + ps) {
+    evaluate = true;
+  } else if (xi < pd + ps) {
     if (spec_delta(m)) {
       out.wi = reflect(-wo, sn);
       if (dot(out.wi, n) <= 0.f) out.wi = reflect(-wo, n);
@@ -260,11 +333,8 @@ PHX_DEV BsdfSample bsdf_sample(const phx_material &m, vec3 wo, vec3 n, vec3 sn, 
     const vec3 h = ggx_half(sn, rough, rng);
     out.wi = reflect(-wo, h);
     if (dot(out.wi, n) <= 0.f) return none;
-    out.f = bsdf_eval(m, wo, out.wi, n, sn);
-    out.pdf = bsdf_pdf(m, wo, out.wi, n, sn);
-    return out;
-  }
-  if (xi < pd + ps + pt && m.transmission > 0.f) {
+    evaluate = true;
+  } else if (xi < pd + ps + pt && m.transmission > 0.f) {
     const vec3 nn = front ? n : n * -1.f;
     vec3 snf = front ? sn : sn * -1.f;
     if (dot(snf, wo) <= 0.f) snf = nn;
@@ -303,15 +373,13 @@ PHX_DEV BsdfSample bsdf_sample(const phx_material &m, vec3 wo, vec3 n, vec3 sn, 
     }
     out.f = base_of(m) * lw;
     return out;
-  }
-  if (pc > 0.f) {
+  } else if (pc > 0.f) {
     const vec3 h = ggx_half(sn, fmaxf(m.clearcoat_roughness, 0.001f), rng);
     out.wi = reflect(-wo, h);
     if (dot(out.wi, n) <= 0.f) return none;
-    out.f = bsdf_eval(m, wo, out.wi, n, sn);
-    out.pdf = bsdf_pdf(m, wo, out.wi, n, sn);
-    return out;
+    evaluate = true;
   }
+  if (evaluate) out.f = bsdf_eval_pdf(m, wo, out.wi, n, sn, out.pdf);
   return out;
 }
 

@@ -22,6 +22,25 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, bool pred) {
   return base + rank;
 }
 
+// wave_alloc on the two halves of a 64-bit counter (two queues counted side
+// by side): one atomic per wave claims the slots of both.  A half never
+// carries into the other as long as it stays below 2^32.  Every active lane of
+// the wave must call it.
+__device__ __forceinline__ void wave_alloc_pair(unsigned long long *counter, bool pred_lo, uint32_t &slot_lo,
+                                                bool pred_hi, uint32_t &slot_hi) {
+  const uint64_t ml = __ballot(pred_lo), mh = __ballot(pred_hi);
+  if ((ml | mh) == 0) return;
+  const uint32_t lane = lane_id();
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  const int leader = __ffsll((unsigned long long)(ml | mh)) - 1;
+  unsigned long long base = 0;
+  if (int(lane) == leader)
+    base = atomicAdd(counter, (unsigned long long)__popcll(ml) | ((unsigned long long)__popcll(mh) << 32));
+  const uint32_t lo = __shfl(uint32_t(base), leader), hi = __shfl(uint32_t(base >> 32), leader);
+  slot_lo = lo + uint32_t(__popcll(ml & below));
+  slot_hi = hi + uint32_t(__popcll(mh & below));
+}
+
 // Adds the sum over the wave's lanes of `n` to a 64-bit statistic, one atomic
 // per wave.  Every lane of the wave must call it.
 __device__ __forceinline__ void wave_sum(unsigned long long *counter, uint32_t n) {

@@ -266,10 +266,11 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
     // Next event estimation (:1006-1035).
     const Direct ds = sample_direct(s, p, rng);
     if (ds.valid) {
-      const vec3 f = bsdf_eval(mat, wo, ds.wi, n, sn);
+      float bp;
+      const vec3 f = bsdf_eval_pdf(mat, wo, ds.wi, n, sn, bp);
       if (f.x > 0.f || f.y > 0.f || f.z > 0.f) {
         float wm = 1.f;
-        if (ds.mis) wm = power_heuristic(ds.pdf, bsdf_pdf(mat, wo, ds.wi, n, sn));
+        if (ds.mis) wm = power_heuristic(ds.pdf, bp);
         out.sc = clamp_contribution(tp * f * ds.L * (wm / ds.pdf), int(depth), w.clamp_direct, w.clamp_indirect);
         hooks.nee(mat, sh, wo, ds.wi, n, sn, tp, ds.L * (wm / ds.pdf));
         const vec3 sorg = p + n * s.ray_eps;

@@ -10,22 +10,26 @@ namespace phx {
 
 // Counter words of one streaming run, in rings of kRing entries indexed by
 // ring(it) for wavefront iteration `it` (a global step of the wavefront, not
-// of a path): the ray queue of `it` is counted by kRayCount + ring(it), the
-// shadow queue it fills by kShadowCount + ring(it); the persistent kernels
-// fetch work through kFetch* + ring(it).  kNextSample is the next sample to
-// start, kIter the current iteration, kTotal / kCapacity the run's sample
-// count and path slots.  The kernels read the iteration from kIter (so a
-// captured sequence of iterations can be replayed unchanged); k_run_start
-// clears everything and k_commit clears the entries of it + 1 before they are
-// used, so no host write or memset sits inside the bounce loop.
+// of a path).  Entry it + 1 of the queue pairs counts the ray queue of
+// iteration it + 1 (word ray_count_word(it + 1)) and the shadow queue that
+// iteration it fills (word shadow_count_word(it + 1)): the two words form one
+// 64-bit word, so the shade kernel claims its continuation and shadow slots
+// with one atomic per wave.  The persistent kernels fetch work through
+// kFetch* + ring(it).  kNextSample is the next sample to start, kIter the
+// current iteration, kTotal / kCapacity the run's sample count and path
+// slots.  The kernels read the iteration from kIter (so a captured sequence of
+// iterations can be replayed unchanged); k_run_start clears everything and
+// k_commit clears the entries of it + 1 before they are used, so no host
+// write or memset sits inside the bounce loop.  Each counter group has a
+// 256-byte block of its own, so the atomics of one kernel do not queue behind
+// another group's in the same cache line.
 constexpr uint32_t kMaxIterations = 255;  // per path (RenderParams bound)
 constexpr uint32_t kRing = 16;
-constexpr uint32_t kRayCount = 0;
-constexpr uint32_t kShadowCount = kRayCount + kRing;
-constexpr uint32_t kFetchExtend = kShadowCount + kRing;
-constexpr uint32_t kFetchShadow = kFetchExtend + kRing;
-constexpr uint32_t kRegenTake = kFetchShadow + kRing;
-constexpr uint32_t kNextSample = kRegenTake + kRing;
+constexpr uint32_t kQueuePairs = 0;  // 2 * kRing words
+constexpr uint32_t kFetchExtend = 64;
+constexpr uint32_t kFetchShadow = 128;
+constexpr uint32_t kRegenTake = 192;
+constexpr uint32_t kNextSample = 256;
 constexpr uint32_t kIter = kNextSample + 1;
 constexpr uint32_t kTotal = kIter + 1;
 constexpr uint32_t kCapacity = kTotal + 1;
@@ -37,6 +41,8 @@ constexpr uint32_t kMaxGroups = 15;
 constexpr uint32_t kMaxClasses = kMaxGroups + 1;
 __host__ __device__ inline uint32_t hit_tri(uint32_t w) { return w == 0xffffffffu ? w : (w & 0x0fffffffu); }
 __host__ __device__ inline uint32_t ring(uint32_t it) { return it & (kRing - 1); }
+__host__ __device__ inline uint32_t ray_count_word(uint32_t it) { return kQueuePairs + 2u * ring(it); }
+__host__ __device__ inline uint32_t shadow_count_word(uint32_t it) { return kQueuePairs + 2u * ring(it) + 1u; }
 
 constexpr uint32_t kStatClosest = 0;
 constexpr uint32_t kStatShadow = 1;

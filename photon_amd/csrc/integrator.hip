@@ -36,7 +36,7 @@ __global__ void k_run_start(StreamQueues q, uint32_t total, uint32_t capacity) {
 template <class P>
 __global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahead, uint32_t zero_L) {
   const uint32_t it = q.counters[kIter] + ahead;
-  const uint32_t used = q.counters[kRayCount + ring(it)];
+  const uint32_t used = q.counters[ray_count_word(it)];
   const uint32_t next = q.counters[kNextSample];
   const uint32_t free_slots = q.counters[kCapacity] - used;
   const uint32_t left = q.counters[kTotal] - next;
@@ -65,12 +65,12 @@ __global__ void k_commit(StreamQueues q, uint32_t ahead) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const uint32_t it = q.counters[kIter] + ahead;
     const uint32_t take = q.counters[kRegenTake + ring(it)];
-    q.counters[kRayCount + ring(it)] += take;
+    q.counters[ray_count_word(it)] += take;
     q.counters[kNextSample] += take;
     // entries of iteration it + 1 (last used by it + 1 - kRing)
     const uint32_t n = ring(it + 1);
-    q.counters[kRayCount + n] = 0u;
-    q.counters[kShadowCount + n] = 0u;
+    q.counters[ray_count_word(it + 1)] = 0u;
+    q.counters[shadow_count_word(it + 1)] = 0u;
     q.counters[kFetchExtend + n] = 0u;
     q.counters[kFetchShadow + n] = 0u;
     q.counters[kIter] = it;
@@ -110,7 +110,7 @@ template <class P, int B>
 __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chunk, uint32_t refill,
                                               uint32_t groups) {
   const uint32_t it = q.counters[kIter];
-  const uint32_t count = q.counters[kRayCount + ring(it)];
+  const uint32_t count = q.counters[ray_count_word(it)];
   if (count == 0) return;
   __shared__ int lds[kShortStack * B];
   const int buf = it & 1;
@@ -144,7 +144,7 @@ template <class P, bool SORTED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(P v, StreamQueues q,
                                                                                       uint32_t groups) {
   const uint32_t it = q.counters[kIter];
-  const uint32_t count = q.counters[kRayCount + ring(it)];
+  const uint32_t count = q.counters[ray_count_word(it)];
   if (blockIdx.x * blockDim.x >= count) return;
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < count;
@@ -202,8 +202,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // the reference's loop bound (pathtracer.cpp:682,762)
     if (out.next.path_it >= r.sp.max_it) out.cont = false;
   }
-  wave_count(&q.stats[kStatShade], valid);
-  const uint32_t ci = wave_alloc(&q.counters[kRayCount + ring(it + 1)], out.cont);
+  if (threadIdx.x == 0) atomicAdd(&q.stats[kStatShade], (unsigned long long)min(256u, count - blockIdx.x * 256u));
+  uint32_t ci = 0, si = 0;
+  wave_alloc_pair(reinterpret_cast<unsigned long long *>(&q.counters[ray_count_word(it + 1)]), out.cont, ci, out.shadow,
+                  si);
   if (out.cont) {
     const PathState &n = out.next;
     q.o[nbuf][ci] = f4(n.ro, n.tmin);
@@ -211,7 +213,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     q.tp[nbuf][ci] = f4(n.tp, n.prev_pdf);
     q.st[nbuf][ci] = make_uint4(g, pack_flags(n), n.kinds, __float_as_uint(n.path_dist));
   }
-  const uint32_t si = wave_alloc(&q.counters[kShadowCount + ring(it)], out.shadow);
   if (out.shadow) {
     // the shading path's iteration keys the transmittance stream
     q.sh_o[si] = f4(out.so, __uint_as_float(out.next.path_it - 1u));
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 template <class P, int B>
 __global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chunk, uint32_t refill) {
   const uint32_t it = q.counters[kIter];
-  const uint32_t count = q.counters[kShadowCount + ring(it)];
+  const uint32_t count = q.counters[shadow_count_word(it + 1)];
   if (count == 0) return;
   __shared__ int lds[kShortStack * B];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chun
 template <class P>
 __global__ __launch_bounds__(256) void k_nee_accumulate(P v, StreamQueues q) {
   const uint32_t it = q.counters[kIter];
-  const uint32_t count = q.counters[kShadowCount + ring(it)];
+  const uint32_t count = q.counters[shadow_count_word(it + 1)];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const float4 c = q.sh_c[i];

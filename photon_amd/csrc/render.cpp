@@ -369,7 +369,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
         // the iteration counter, the samples started, then that iteration's queue
         HIP_TRY(c, hipMemcpyAsync(&c->pinned[0], q.counters + kIter, 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(c, hipMemcpyAsync(&c->pinned[1], q.counters + kNextSample, 4, hipMemcpyDeviceToHost, st));
-        HIP_TRY(c, hipMemcpyAsync(&c->pinned[2], q.counters + kRayCount + ring(it), 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(c, hipMemcpyAsync(&c->pinned[2], q.counters + ray_count_word(it), 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(c, hipStreamSynchronize(st));
         if (c->pinned[0] != it)
           return fail(c, PHX_ERR_OVERFLOW, "render: wavefront iteration counter out of step");
