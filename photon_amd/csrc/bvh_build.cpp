@@ -193,11 +193,12 @@ void quantize(BvhNode &n, const Box *kids, int nk) {
     const float ext = nb.hi[a] - org;
     int e = -100;
     if (ext > 0.f) e = std::max(-100, int(std::ceil(std::log2(double(ext) / 65535.0))));
-    auto dec = [&](uint32_t q, int ee) {
-      const float s = std::ldexp(1.f, ee);
-      return org + float(q) * s;
-    };
-    while (dec(65535u, e) < nb.hi[a]) ++e;
+    // The exact value of a quantised plane, org + q * 2^e (exact in double).
+    // Planes are rounded outward against it, which makes the float decode
+    // (its rounding) and the traversal's fused slab test (which evaluates it
+    // unrounded) conservative as well.
+    auto dec = [&](uint32_t q, int ee) { return double(org) + double(q) * std::ldexp(1.0, ee); };
+    while (dec(65535u, e) < double(nb.hi[a])) ++e;
     n.org[a] = org;
     n.ex[a] = int8_t(e);
     const float s = std::ldexp(1.f, e);
@@ -211,8 +212,8 @@ void quantize(BvhNode &n, const Box *kids, int nk) {
       double qh = std::ceil(double(kids[k].hi[a] - org) / double(s));
       uint32_t lo = uint32_t(std::min(std::max(ql, 0.0), 65535.0));
       uint32_t hi = uint32_t(std::min(std::max(qh, 0.0), 65535.0));
-      while (lo > 0 && dec(lo, e) > kids[k].lo[a]) --lo;
-      while (hi < 65535u && dec(hi, e) < kids[k].hi[a]) ++hi;
+      while (lo > 0 && dec(lo, e) > double(kids[k].lo[a])) --lo;
+      while (hi < 65535u && dec(hi, e) < double(kids[k].hi[a])) ++hi;
       if (hi < lo) hi = lo;
       n.qlo[a][k] = uint16_t(lo);
       n.qhi[a][k] = uint16_t(hi);

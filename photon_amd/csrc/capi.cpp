@@ -301,11 +301,29 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
       cdf[n - 1] = 1.f;
     }
   }
+  // Search bounds of the emissive CDF per xi bucket: the answer of the
+  // search at each bucket edge b / kEmGuide (exact in float).
+  std::vector<uint32_t> em_guide;
+  if (em_count) {
+    em_guide.resize(phx::kEmGuide + 1);
+    for (uint32_t b = 0; b <= phx::kEmGuide; ++b) {
+      const float u = float(b) / float(phx::kEmGuide);
+      uint32_t lo = 0, hi = em_count - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid + 1;
+        else hi = mid;
+      }
+      em_guide[b] = lo;
+    }
+  }
 
   if ((r = upload(c, c->materials, mats.data(), mats.size()))) return r;
   if ((r = upload(c, c->lights, nee.data(), nee.size()))) return r;
   if ((r = upload(c, c->em_ids, em_ids.data(), em_count ? em_ids.size() : 0))) return r;
   if ((r = upload(c, c->em_cdf, cdf.data(), cdf.size()))) return r;
+  if ((r = upload(c, c->em_guide, em_guide.data(), em_guide.size()))) return r;
+  HIP_TRY(c, c->em_recs.reserve(size_t(em_count) * 4 * sizeof(float4)));
   if ((r = upload(c, c->prim_light_pdf, prim_pdf.data(), prim_pdf.size()))) return r;
 
   DevScene &s = c->dev;
@@ -315,6 +333,8 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
   s.light_count = uint32_t(nee.size());
   s.em_ids = c->em_ids.as<uint32_t>();
   s.em_cdf = c->em_cdf.as<float>();
+  s.em_guide = c->em_guide.as<uint32_t>();
+  s.em_recs = c->em_recs.as<float4>();
   s.prim_light_pdf = c->prim_light_pdf.as<float>();
   s.em_count = em_count;
 
@@ -403,6 +423,7 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
     eps = e < 1e-6f ? 1e-6f : (e > 1e-2f ? 1e-2f : e);
   }
   s.ray_eps = eps;
+  HIP_TRY(c, launch_em_records(s, c->em_recs.as<float4>(), c->stream));
   c->stats.scene_bytes = c->scene_bytes;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->have_scene = true;

@@ -64,7 +64,7 @@ struct phx_context {
 
   // shading data
   bool have_scene = false;
-  DBuf materials, lights, em_ids, em_cdf, prim_light_pdf, env_pixels, env_marginal, env_cond, env_guide_m, env_guide_c,
+  DBuf materials, lights, em_ids, em_cdf, em_guide, em_recs, prim_light_pdf, env_pixels, env_marginal, env_cond, env_guide_m, env_guide_c,
       tex_pixels, tex_infos,
       volumes, vol_values, vol_tf, majorants;
   phx::DevScene dev{};

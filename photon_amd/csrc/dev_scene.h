@@ -44,6 +44,8 @@ PHX_HD int leaf_code(uint32_t first, uint32_t count) { return int(~((first << 3)
 
 // Buckets of the environment CDF search guides (a power of two).
 constexpr uint32_t kEnvGuide = 128;
+// Buckets of the emissive-triangle CDF search guide (a power of two).
+constexpr uint32_t kEmGuide = 1024;
 
 // Triangles in leaf order: v0, e1 = v1 - v0, e2 = v2 - v0 (the Moller-Trumbore
 // edge vectors, computed with the same float subtraction the reference does)
@@ -67,6 +69,8 @@ struct DevScene {
   const phx_light *lights;
   const uint32_t *em_ids;
   const float *em_cdf;
+  const uint32_t *em_guide;  // kEmGuide + 1 search bounds of em_cdf
+  const float4 *em_recs;     // per em_cdf entry: the emitter record of em_record()
   const float *prim_light_pdf;
   const float *env_pixels;
   const float *env_marginal;
@@ -327,6 +331,27 @@ PHX_DEV bool emits_on_side(const phx_material &m, bool front) {
   return front ? !back_only : back_only;
 }
 
+// The emitter record of emissive-CDF entry i (four float4): the triangle's
+// vertices with its area pdf (pathtracer.cpp:411-503) and the emitter's
+// radiance, emission times alpha of its material resolved at uv (0, 0), with
+// bit 0 / bit 1 of r3.y set when it emits toward its front / back side.
+PHX_DEV void em_record(const DevScene &s, uint32_t i, float4 *out) {
+  const uint32_t prim = s.em_ids[i];
+  const float pa = s.prim_light_pdf[prim];
+  const vec3 v0 = ld3(s.positions + size_t(s.indices[prim * 3]) * 3);
+  const vec3 v1 = ld3(s.positions + size_t(s.indices[prim * 3 + 1]) * 3);
+  const vec3 v2 = ld3(s.positions + size_t(s.indices[prim * 3 + 2]) * 3);
+  float alpha = 1.f;
+  const uint32_t mid = s.material_ids ? s.material_ids[prim] : 0u;
+  const phx_material em = resolve_material(s, mid, vec2{0.f, 0.f}, false, vec3{0.f, 0.f, 0.f}, alpha);
+  const vec3 L = material_emission(em) * alpha;
+  const uint32_t sides = (emits_on_side(em, true) ? 1u : 0u) | (emits_on_side(em, false) ? 2u : 0u);
+  out[0] = make_float4(v0.x, v0.y, v0.z, pa);
+  out[1] = make_float4(v1.x, v1.y, v1.z, L.x);
+  out[2] = make_float4(v2.x, v2.y, v2.z, L.y);
+  out[3] = make_float4(L.z, __uint_as_float(sides), 0.f, 0.f);
+}
+
 // ---- light strategy pick (pathtracer.cpp:270-367) ---------------------------
 struct Direct {
   vec3 wi, L;
@@ -351,22 +376,25 @@ PHX_DEV Direct sample_direct(const DevScene &s, vec3 p, Rng &rng) {
     return ds;
   }
   if (s.em_count > 0 && pick == s.light_count) {
+    // The search of pathtracer.cpp:287-295 (first entry above xi, at most
+    // em_count - 1), started from the guide's bounds for xi's bucket; then one
+    // record holds the triangle, its pdf and the emitter's radiance.
     const float xi = rng.next();
-    uint32_t lo = 0, hi = s.em_count - 1;
+    const uint32_t bucket = min(uint32_t(xi * float(kEmGuide)), kEmGuide - 1u);
+    uint32_t lo = s.em_guide[bucket], hi = s.em_guide[bucket + 1];
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
       if (s.em_cdf[mid] <= xi) lo = mid + 1;
       else hi = mid;
     }
-    const uint32_t prim = s.em_ids[lo];
-    const float pa = s.prim_light_pdf[prim];
+    const float4 *rec = s.em_recs + size_t(lo) * 4;
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    const float pa = r0.w;
     if (!(pa > 0.f)) return ds;
-    const vec3 v0 = ld3(s.positions + size_t(s.indices[prim * 3]) * 3);
-    const vec3 v1 = ld3(s.positions + size_t(s.indices[prim * 3 + 1]) * 3);
-    const vec3 v2 = ld3(s.positions + size_t(s.indices[prim * 3 + 2]) * 3);
+    const vec3 v0{r0.x, r0.y, r0.z}, v1{r1.x, r1.y, r1.z}, v2{r2.x, r2.y, r2.z};
     const float su = sqrtf(rng.next());
-    const float r2 = rng.next();
-    const float b0 = 1.f - su, b1 = r2 * su, b2 = 1.f - b0 - b1;
+    const float r2u = rng.next();
+    const float b0 = 1.f - su, b1 = r2u * su, b2 = 1.f - b0 - b1;
     const vec3 lp = v0 * b0 + v1 * b1 + v2 * b2;
     const vec3 to = lp - p;
     const float d2 = dot(to, to);
@@ -376,12 +404,9 @@ PHX_DEV Direct sample_direct(const DevScene &s, vec3 p, Rng &rng) {
     const vec3 ln = normalize(cross(v1 - v0, v2 - v0));
     const float cl = -dot(wi, ln);
     if (fabsf(cl) < 1e-6f) return ds;
-    float alpha = 1.f;
-    const uint32_t mid = s.material_ids ? s.material_ids[prim] : 0u;
-    const phx_material em = resolve_material(s, mid, vec2{0.f, 0.f}, false, vec3{0.f, 0.f, 0.f}, alpha);
-    if (!emits_on_side(em, cl > 0.f)) return ds;
+    if (!(__float_as_uint(r3.y) & (cl > 0.f ? 1u : 2u))) return ds;
     ds.wi = wi;
-    ds.L = material_emission(em) * alpha;
+    ds.L = vec3{r1.w, r2.w, r3.x};
     ds.pdf = pa * d2 / fabsf(cl) * pick_pdf;
     ds.dist = dist;
     ds.mis = true;

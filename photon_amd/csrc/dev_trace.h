@@ -83,10 +83,14 @@ PHX_DEV void cswap(float &ka, int &va, float &kb, int &vb) {
   }
 }
 
-// Slab test of a node's children with the dequantised boxes.  Entry
-// distances are clamped to [tmin, tmax]; the exit is widened by a few ulps so
-// rounding never culls a box the exact test would enter.  Returns the number
-// of children entered; codes[0..count) are their stack entries, nearest first.
+// Slab test of a node's children against their quantised boxes.  A plane at
+// org + q * 2^e is met at t = (org + q * 2^e - o) * inv = q * A + B with
+// A = 2^e * inv (exact) and B = (org - o) * inv, one fused multiply-add per
+// plane; the near and far planes of each axis are picked once per node from
+// the sign of the direction.  Entry distances are clamped to [tmin, tmax]; the
+// exit is widened by a few ulps so rounding never culls a box the exact test
+// would enter.  Returns the number of children entered; codes[0..count) are
+// their stack entries, nearest first.
 template <bool SORT>
 PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float tmin, float tmax,
                        int codes[kNodeWidth], float key[kNodeWidth]) {
@@ -101,7 +105,11 @@ PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float t
   const int ninner = int(w[0].w >> 24);
   const uint32_t child_base = w[1].x, tri_base = w[1].y;
   const uint32_t leaf_cnt = w[1].z & 0xffffu, nchild = (w[1].z >> 16) & 0xffu;
-  const float sx = ldexpf(1.f, ex), sy = ldexpf(1.f, ey), sz = ldexpf(1.f, ez);
+  const float Ax = ldexpf(inv.x, ex), Ay = ldexpf(inv.y, ey), Az = ldexpf(inv.z, ez);
+  const float Bx = (org.x - o.x) * inv.x, By = (org.y - o.y) * inv.y, Bz = (org.z - o.z) * inv.z;
+  const uint4 nxw = inv.x >= 0.f ? w[2] : w[5], fxw = inv.x >= 0.f ? w[5] : w[2];
+  const uint4 nyw = inv.y >= 0.f ? w[3] : w[6], fyw = inv.y >= 0.f ? w[6] : w[3];
+  const uint4 nzw = inv.z >= 0.f ? w[4] : w[7], fzw = inv.z >= 0.f ? w[7] : w[4];
   auto q16 = [](const uint4 &v, int k) {
     const uint32_t d = k < 2 ? v.x : (k < 4 ? v.y : (k < 6 ? v.z : v.w));
     return float((k & 1) ? (d >> 16) : (d & 0xffffu));
@@ -110,14 +118,10 @@ PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float t
   int cnt = 0;
 #pragma unroll
   for (int k = 0; k < kNodeWidth; ++k) {
-    const float lox = org.x + q16(w[2], k) * sx, hix = org.x + q16(w[5], k) * sx;
-    const float loy = org.y + q16(w[3], k) * sy, hiy = org.y + q16(w[6], k) * sy;
-    const float loz = org.z + q16(w[4], k) * sz, hiz = org.z + q16(w[7], k) * sz;
-    const float tx0 = (lox - o.x) * inv.x, tx1 = (hix - o.x) * inv.x;
-    const float ty0 = (loy - o.y) * inv.y, ty1 = (hiy - o.y) * inv.y;
-    const float tz0 = (loz - o.z) * inv.z, tz1 = (hiz - o.z) * inv.z;
-    const float a = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-    const float b = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    const float a = fmaxf(fmaxf(fmaf(q16(nxw, k), Ax, Bx), fmaf(q16(nyw, k), Ay, By)),
+                          fmaxf(fmaf(q16(nzw, k), Az, Bz), tmin));
+    const float b = fminf(fminf(fmaf(q16(fxw, k), Ax, Bx), fmaf(q16(fyw, k), Ay, By)),
+                          fminf(fmaf(q16(fzw, k), Az, Bz), tmax));
     const bool valid = k < int(nchild);
     const bool hit = valid && a <= b * 1.0000004f;
     int code;

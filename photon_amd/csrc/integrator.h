@@ -126,7 +126,7 @@ struct LoopShape {
   uint32_t trace_block;  // 128 or 256
   uint32_t trace_grid;   // work-groups of the persistent traversal kernels
   uint32_t shade_grid;   // work-groups (of 256) of the shade kernel: covers the path slots
-  uint32_t regen_grid;   // work-groups (of 256) of the regeneration kernel
+  uint32_t regen_grid;   // work-groups (of 256) of the regeneration and NEE-accumulation kernels
   uint32_t chunk;        // work items a wave takes per fetch (multiple of 64)
   uint32_t refill;       // idle lanes of a traversal wave that trigger a refill (1..64)
   uint32_t groups;       // material groups of sorted shading (0: queue order)
@@ -143,6 +143,8 @@ struct ViewArgs {
   ItemTable<false> many;
 };
 
+// Fills the emitter records of s's emissive CDF (s.em_count entries of four float4).
+hipError_t launch_em_records(const DevScene &s, float4 *out, hipStream_t st);
 hipError_t launch_run_start(const StreamQueues &q, uint32_t total, uint32_t capacity, hipStream_t st);
 // ahead = 0: fill the queue of the current iteration (start of a run);
 // ahead = 1: fill the next iteration's queue and advance the iteration.

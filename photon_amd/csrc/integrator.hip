@@ -59,6 +59,18 @@ __global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahe
   }
 }
 
+// The emitter records of the emissive CDF (em_record), rebuilt whenever the
+// scene's materials or textures change.
+__global__ __launch_bounds__(256) void k_em_records(DevScene s, float4 *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < s.em_count) em_record(s, i, out + size_t(i) * 4);
+}
+hipError_t launch_em_records(const DevScene &s, float4 *out, hipStream_t st) {
+  if (s.em_count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_em_records, dim3((s.em_count + 255) / 256), dim3(256), 0, st, s, out);
+  return hipGetLastError();
+}
+
 // Commits the regenerated paths, clears the ring entries of the following
 // iteration and (ahead = 1) advances the iteration.
 __global__ void k_commit(StreamQueues q, uint32_t ahead) {
@@ -136,17 +148,10 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
   wave_sum(&q.stats[kStatClosest], live);
 }
 
-// pt_shade over the ray queue, one thread per path (a persistent loop would
-// keep the shading constants live across iterations and double the register
-// use); continuing paths and shadow rays go to compacted queues, one atomic
-// per wave.  The grid covers the path slots; threads beyond the queue exit.
 template <class P, bool SORTED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(P v, StreamQueues q,
-                                                                                      uint32_t groups) {
-  const uint32_t it = q.counters[kIter];
-  const uint32_t count = q.counters[ray_count_word(it)];
-  if (blockIdx.x * blockDim.x >= count) return;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, uint32_t groups, uint32_t it,
+                                           uint32_t count, uint32_t tile) {
+  uint32_t i = tile + threadIdx.x;
   const bool valid = i < count;
   if (SORTED) {
     // counting sort of the work-group's 256 rays by material group (misses
@@ -174,7 +179,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __syncthreads();
     s_perm[s_cnt[key] + rank] = threadIdx.x;
     __syncthreads();
-    i = blockIdx.x * blockDim.x + s_perm[threadIdx.x];
+    i = tile + s_perm[threadIdx.x];
   }
   const int buf = it & 1, nbuf = buf ^ 1;
   ShadeOut out;
@@ -202,7 +207,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // the reference's loop bound (pathtracer.cpp:682,762)
     if (out.next.path_it >= r.sp.max_it) out.cont = false;
   }
-  if (threadIdx.x == 0) atomicAdd(&q.stats[kStatShade], (unsigned long long)min(256u, count - blockIdx.x * 256u));
+  if (threadIdx.x == 0) atomicAdd(&q.stats[kStatShade], (unsigned long long)min(256u, count - tile));
   uint32_t ci = 0, si = 0;
   wave_alloc_pair(reinterpret_cast<unsigned long long *>(&q.counters[ray_count_word(it + 1)]), out.cont, ci, out.shadow,
                   si);
@@ -219,6 +224,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     q.sh_d[si] = f4(out.sd, out.stmax);
     q.sh_c[si] = f4(out.sc, __uint_as_float(g));
   }
+}
+
+// pt_shade over the ray queue, one thread per path; continuing paths and
+// shadow rays go to compacted queues, one atomic per wave.  The grid covers the
+// path slots and work-groups beyond the queue exit at once: a grid-stride loop
+// over the queue measured slower, since the loop raises the spill traffic.
+template <class P, bool SORTED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(P v, StreamQueues q,
+                                                                                      uint32_t groups) {
+  const uint32_t it = q.counters[kIter];
+  const uint32_t count = q.counters[ray_count_word(it)];
+  const uint32_t tile = blockIdx.x * 256u;
+  if (tile >= count) return;
+  shade_tile<P, SORTED>(v, q, groups, it, count, tile);
 }
 
 // Any hit over the shadow queue (trace_occluded; persistent waves, per-lane
@@ -250,15 +269,8 @@ __global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chun
   wave_sum(&q.stats[kStatShadow], live);
 }
 
-// pt_nee_accum_pack (pathtracer.cpp:1095-1121): the unoccluded shadow rays'
-// contributions, times the volume transmittance along them, into their
-// samples' radiance.  One thread per shadow ray; the grid covers the path slots.
 template <class P>
-__global__ __launch_bounds__(256) void k_nee_accumulate(P v, StreamQueues q) {
-  const uint32_t it = q.counters[kIter];
-  const uint32_t count = q.counters[shadow_count_word(it + 1)];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+__device__ __forceinline__ void nee_accumulate_one(const P &v, const StreamQueues &q, uint32_t i) {
   const float4 c = q.sh_c[i];
   const uint32_t g = __float_as_uint(c.w);
   if (g == kNoRay) return;
@@ -277,6 +289,16 @@ __global__ __launch_bounds__(256) void k_nee_accumulate(P v, StreamQueues q) {
   }
 }
 
+// pt_nee_accum_pack (pathtracer.cpp:1095-1121): the unoccluded shadow rays'
+// contributions, times the volume transmittance along them, into their
+// samples' radiance.  One thread per shadow ray, over the queue in strides of
+// the grid.
+template <class P>
+__global__ __launch_bounds__(256) void k_nee_accumulate(P v, StreamQueues q) {
+  const uint32_t it = q.counters[kIter];
+  const uint32_t count = q.counters[shadow_count_word(it + 1)];
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) nee_accumulate_one(v, q, i);
+}
 // Sums an item's samples of each pixel into the frame, in sample order.
 __global__ __launch_bounds__(256) void k_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum,
                                                 uint32_t first) {
@@ -446,7 +468,7 @@ hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopSha
       hipLaunchKernelGGL((k_shadow<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_nee_accumulate<P>, dim3(ls.shade_grid), dim3(256), 0, st, p, q);
+    hipLaunchKernelGGL(k_nee_accumulate<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q);
     return hipGetLastError();
   });
 }

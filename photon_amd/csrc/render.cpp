@@ -614,6 +614,7 @@ extern "C" int phx_set_material_params(phx_context *c, const float *d_params, ui
   if (count > c->dev.material_count) return fail(c, PHX_ERR_INVALID, "set_material_params: more than the scene's materials");
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, launch_set_material_params(c->materials.as<phx_material>(), d_params, count, c->stream));
+  HIP_TRY(c, launch_em_records(c->dev, c->em_recs.as<float4>(), c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return PHX_OK;
 }
@@ -627,6 +628,7 @@ extern "C" int phx_set_texture_pixels(phx_context *c, const float *d_pixels, uin
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipMemcpyAsync(c->tex_pixels.as<float>() + size_t(first) * 3, d_pixels, size_t(count) * 12,
                             hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(c, launch_em_records(c->dev, c->em_recs.as<float4>(), c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return PHX_OK;
 }
