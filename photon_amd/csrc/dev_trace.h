@@ -73,14 +73,14 @@ PHX_DEV bool tri_test(const TriRec &r, vec3 o, vec3 d, float tmin, float tmax, f
 }
 
 PHX_DEV void cswap(float &ka, int &va, float &kb, int &vb) {
-  if (kb < ka) {
-    const float k = ka;
-    ka = kb;
-    kb = k;
-    const int v = va;
-    va = vb;
-    vb = v;
-  }
+  // selects, not a branch: one compare and four v_cndmask per exchange
+  const bool s = kb < ka;
+  const float k0 = s ? kb : ka, k1 = s ? ka : kb;
+  const int v0 = s ? vb : va, v1 = s ? va : vb;
+  ka = k0;
+  kb = k1;
+  va = v0;
+  vb = v1;
 }
 
 // Slab test of a node's children against their quantised boxes.  A plane at
