@@ -189,28 +189,33 @@ PHX_DEV int descend_any(LaneStack &st, const int codes[kNodeWidth], const float 
   return SORT ? descend(st, codes, cnt) : descend_unsorted(st, codes, key);
 }
 
-// The triangles of a leaf entry against the ray: updates the closest hit h
-// (tie rule: equal t goes to the larger primitive id); returns true on a hit
-// when ANY (any hit ends the ray).
+// Triangle `k` of the leaf order against the ray: updates the closest hit h
+// (tie rule: equal t goes to the larger primitive id); returns true when it
+// becomes the hit.
+PHX_DEV bool visit_tri(const TriRec *tris, uint32_t k, vec3 o, vec3 d, float tmin, float &best, TraceHit &h) {
+  const TriRec r = tris[k];
+  float t, u, v;
+  if (!tri_test(r, o, d, tmin, best, t, u, v)) return false;
+  const uint32_t prim = __float_as_uint(r.v0.w);
+  if (t < best || prim > h.prim || h.tri == 0xffffffffu) {
+    best = t;
+    h.t = t;
+    h.u = u;
+    h.v = v;
+    h.tri = k;
+    h.prim = prim;
+    return true;
+  }
+  return false;
+}
+// The triangles of a leaf entry against the ray; returns true on a hit when
+// ANY (any hit ends the ray).
 template <bool ANY>
 PHX_DEV bool visit_leaf(const TriRec *tris, int node, vec3 o, vec3 d, float tmin, float &best, TraceHit &h) {
   const uint32_t code = ~uint32_t(node);
   const uint32_t first = code >> 3, count = (code & 7u) + 1u;
-  for (uint32_t k = 0; k < count; ++k) {
-    const TriRec r = tris[first + k];
-    float t, u, v;
-    if (!tri_test(r, o, d, tmin, best, t, u, v)) continue;
-    const uint32_t prim = __float_as_uint(r.v0.w);
-    if (t < best || prim > h.prim || h.tri == 0xffffffffu) {
-      best = t;
-      h.t = t;
-      h.u = u;
-      h.v = v;
-      h.tri = first + k;
-      h.prim = prim;
-      if (ANY) return true;
-    }
-  }
+  for (uint32_t k = 0; k < count; ++k)
+    if (visit_tri(tris, first + k, o, d, tmin, best, h) && ANY) return true;
   return false;
 }
 
@@ -245,8 +250,8 @@ PHX_DEV bool traverse(const DevScene &s, vec3 o, vec3 d, float tmin, float tmax,
 }
 
 // ---- streaming traversal of a ray queue ---------------------------------------
-// The persistent traversal kernels run one node (or one leaf) per loop
-// iteration for every lane that holds a ray.  A lane whose ray is done takes
+// The persistent traversal kernels run one node (or one leaf triangle) per
+// loop iteration for every lane that holds a ray.  A lane whose ray is done takes
 // the next ray at once, instead of idling until the wave's slowest ray is
 // done: when at least `refill_min` lanes are idle, the wave hands them rays
 // from its current chunk of the queue, taking a new chunk of `chunk` rays from
@@ -332,7 +337,16 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *counter, uint32_t count, u
           pop = false;
         }
       } else {
-        finished = visit_leaf<ANY>(tris, node, o, d, tmin, best, h);
+        // one triangle per iteration: the leaf entry shrinks in place until
+        // its last triangle, so a wave whose lanes hold leaves of different
+        // sizes pays one triangle test per iteration, not the largest leaf
+        const uint32_t code = ~uint32_t(node);
+        const uint32_t first = code >> 3, count = (code & 7u) + 1u;
+        finished = visit_tri(tris, first, o, d, tmin, best, h) && ANY;
+        if (!finished && count > 1) {
+          node = leaf_code(first + 1u, count - 1u);
+          pop = false;
+        }
       }
       if (pop && !finished) {
         if (st.sp == 0) finished = true;
