@@ -41,6 +41,45 @@ __device__ __forceinline__ void wave_alloc_pair(unsigned long long *counter, boo
   slot_hi = hi + uint32_t(__popcll(mh & below));
 }
 
+// wave_alloc_pair for a whole work-group of kWaves waves: one atomic per
+// work-group.  All waves of the GPU append to the same counter, and
+// device-scope atomics on one address are serialised: at one atomic per wave
+// the shade kernel spent most of its time queued on it.  Every thread of the
+// work-group must call this (it synchronises the work-group).
+template <int kWaves>
+__device__ __forceinline__ void block_alloc_pair(unsigned long long *counter, bool pred_lo, uint32_t &slot_lo,
+                                                 bool pred_hi, uint32_t &slot_hi) {
+  __shared__ uint32_t s_n[2][kWaves];
+  __shared__ unsigned long long s_base;
+  const uint64_t ml = __ballot(pred_lo), mh = __ballot(pred_hi);
+  const uint32_t wave = threadIdx.x >> 6, lane = lane_id();
+  if (lane == 0) {
+    s_n[0][wave] = uint32_t(__popcll(ml));
+    s_n[1][wave] = uint32_t(__popcll(mh));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t nl = 0, nh = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      nl += s_n[0][w];
+      nh += s_n[1][w];
+    }
+    s_base = (nl | nh) ? atomicAdd(counter, (unsigned long long)nl | ((unsigned long long)nh << 32)) : 0ull;
+  }
+  __syncthreads();
+  const unsigned long long base = s_base;
+  uint32_t ol = 0, oh = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    ol += uint32_t(w) < wave ? s_n[0][w] : 0u;
+    oh += uint32_t(w) < wave ? s_n[1][w] : 0u;
+  }
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  slot_lo = uint32_t(base) + ol + uint32_t(__popcll(ml & below));
+  slot_hi = uint32_t(base >> 32) + oh + uint32_t(__popcll(mh & below));
+}
+
 // Adds the sum over the wave's lanes of `n` to a 64-bit statistic, one atomic
 // per wave.  Every lane of the wave must call it.
 __device__ __forceinline__ void wave_sum(unsigned long long *counter, uint32_t n) {

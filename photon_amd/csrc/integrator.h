@@ -14,7 +14,7 @@ namespace phx {
 // iteration it + 1 (word ray_count_word(it + 1)) and the shadow queue that
 // iteration it fills (word shadow_count_word(it + 1)): the two words form one
 // 64-bit word, so the shade kernel claims its continuation and shadow slots
-// with one atomic per wave.  The persistent kernels fetch work through
+// with one atomic per work-group.  The persistent kernels fetch work through
 // kFetch* + ring(it).  kNextSample is the next sample to start, kIter the
 // current iteration, kTotal / kCapacity the run's sample count and path
 // slots.  The kernels read the iteration from kIter (so a captured sequence of

@@ -9,7 +9,7 @@
 //              slots of the next iteration's queue
 //   k_extend   closest hit of the ray queue (trace_closest)
 //   k_shade    pt_shade (:765-1090) via shade_path(); continuing paths and
-//              shadow rays go to compacted queues (one atomic per wave)
+//              shadow rays go to compacted queues (one atomic per work-group)
 //   k_shadow   any hit + NEE accumulation (trace_occluded, pt_nee_accum_pack)
 //   k_reduce   per-pixel sums over the samples, in sample order
 //   k_resolve  mean, alpha and the AOVs (pt_resolve, :1143-1155)
@@ -209,8 +209,8 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
   }
   if (threadIdx.x == 0) atomicAdd(&q.stats[kStatShade], (unsigned long long)min(256u, count - tile));
   uint32_t ci = 0, si = 0;
-  wave_alloc_pair(reinterpret_cast<unsigned long long *>(&q.counters[ray_count_word(it + 1)]), out.cont, ci, out.shadow,
-                  si);
+  block_alloc_pair<4>(reinterpret_cast<unsigned long long *>(&q.counters[ray_count_word(it + 1)]), out.cont, ci,
+                      out.shadow, si);
   if (out.cont) {
     const PathState &n = out.next;
     q.o[nbuf][ci] = f4(n.ro, n.tmin);
@@ -227,7 +227,7 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
 }
 
 // pt_shade over the ray queue, one thread per path; continuing paths and
-// shadow rays go to compacted queues, one atomic per wave.  The grid covers the
+// shadow rays go to compacted queues, one atomic per work-group.  The grid covers the
 // path slots and work-groups beyond the queue exit at once: a grid-stride loop
 // over the queue measured slower, since the loop raises the spill traffic.
 template <class P, bool SORTED>
