@@ -117,7 +117,7 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
   WorkCursor cur = first_chunk(chunk);
   while (true) {
     // the replay runs no shadow kernel: its fetch counter serves this loop
-    const uint32_t base = next_batch(cur, &q.counters[kFetchShadow + ring(it)], count, chunk);
+    const uint32_t base = next_batch(cur, &q.counters[fetch_set(kFetchShadow, it)], count, chunk);
     if (base >= count) break;
     const uint32_t i = base + lane;
     const bool valid = i < count;

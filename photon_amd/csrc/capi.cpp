@@ -438,7 +438,7 @@ int phx_trace_closest_compact(phx_context *c, const float *o, const float *d, co
   int r = ensure_trace_scratch(c);
   if (r) return r;
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
+  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), kFetchSetWords, st));
   HIP_TRY(c, launch_trace_compact(c->dev, o, d, tmin, tmax, n, out, c->fetch.as<uint32_t>(), c->spill.as<int>(),
                                   c->trace_grid, st));
   return PHX_OK;
@@ -451,7 +451,7 @@ int phx_trace_closest(phx_context *c, const float *o, const float *d, const floa
   int r = ensure_trace_scratch(c);
   if (r) return r;
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
+  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), kFetchSetWords, st));
   HIP_TRY(c, launch_trace_full(c->dev, o, d, tmin, tmax, n, out, c->fetch.as<uint32_t>(), c->spill.as<int>(),
                                c->trace_grid, st));
   return PHX_OK;
@@ -464,7 +464,7 @@ int phx_trace_occluded(phx_context *c, const float *o, const float *d, const flo
   int r = ensure_trace_scratch(c);
   if (r) return r;
   hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
-  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
+  HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), kFetchSetWords, st));
   HIP_TRY(c, launch_trace_occluded(c->dev, o, d, tmin, tmax, n, out, c->fetch.as<uint32_t>(), c->spill.as<int>(),
                                    c->trace_grid, st));
   return PHX_OK;

@@ -148,7 +148,7 @@ inline int ensure_trace_scratch(phx_context *c, size_t threads = 0) {
     HIP_TRY(c, c->spill.reserve(threads * (phx::kMaxStack - phx::kShortStack) * sizeof(int)));
     c->spill_threads = threads;
   }
-  HIP_TRY(c, c->fetch.reserve(64));
+  HIP_TRY(c, c->fetch.reserve(phx::kFetchSetWords * 4));
   return PHX_OK;
 }
 

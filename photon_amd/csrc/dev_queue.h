@@ -103,6 +103,43 @@ __device__ __forceinline__ void wave_count(unsigned long long *counter, bool pre
 // L2: a coherent pre-read of the counter would bypass L2 and serialize at
 // memory.)
 constexpr uint32_t kChunk = 256;  // default chunk
+
+// Striped work fetch of the persistent traversal kernels.  The queue is cut
+// into kStripes stripes of equal length (a multiple of 64); each stripe has
+// its own chunk counter in a 128-byte line of its own (a fetch set of
+// kFetchSetWords words), so the waves of the GPU spread their chunk atomics
+// over kStripes addresses instead of queueing on one.  A wave starts in stripe
+// (its index mod kStripes) and moves on to the next stripe when its stripe is
+// used up.
+constexpr uint32_t kStripes = 16;
+constexpr uint32_t kStripeStride = 32;  // words between the counters of a set
+constexpr uint32_t kFetchSetWords = kStripes * kStripeStride;
+__device__ __forceinline__ uint32_t stripe_len(uint32_t count) {
+  return ((count + kStripes - 1u) / kStripes + 63u) & ~63u;
+}
+// The wave's next chunk [lo, hi) of at most `chunk` items, from its current
+// stripe or the ones after it; lo = 0xffffffff when every stripe is used up.
+// Wave-uniform: every lane of the wave must call it.
+__device__ __forceinline__ void fetch_chunk(uint32_t *set, uint32_t count, uint32_t slen, uint32_t chunk,
+                                            uint32_t &stripe, uint32_t &lo, uint32_t &hi) {
+  lo = 0xffffffffu;
+  hi = 0u;
+  for (uint32_t t = 0; t < kStripes; ++t) {
+    const uint32_t s_lo = stripe * slen;
+    const uint32_t s_hi = min(s_lo + slen, count);
+    if (s_lo < s_hi) {
+      uint32_t b = 0;
+      if (lane_id() == 0) b = atomicAdd(&set[stripe * kStripeStride], chunk);
+      b = __shfl(b, 0);
+      if (b < s_hi - s_lo) {
+        lo = s_lo + b;
+        hi = min(lo + chunk, s_hi);
+        return;
+      }
+    }
+    stripe = stripe + 1u == kStripes ? 0u : stripe + 1u;
+  }
+}
 struct WorkCursor {
   uint32_t base;  // first item of the current chunk
   uint32_t sub;   // next 64-item sub-batch within the chunk

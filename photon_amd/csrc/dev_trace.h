@@ -18,6 +18,7 @@
 // tie rule is "last visited", which depends on its tree).
 #pragma once
 
+#include "dev_queue.h"
 #include "dev_scene.h"
 
 namespace phx {
@@ -254,18 +255,21 @@ PHX_DEV bool traverse(const DevScene &s, vec3 o, vec3 d, float tmin, float tmax,
 // loop iteration for every lane that holds a ray.  A lane whose ray is done takes
 // the next ray at once, instead of idling until the wave's slowest ray is
 // done: when at least `refill_min` lanes are idle, the wave hands them rays
-// from its current chunk of the queue, taking a new chunk of `chunk` rays from
-// `counter` (one atomic per wave) when the chunk runs out.  Per ray the result
-// is exactly traverse<ANY>'s.
+// from its current chunk of the queue, taking a new chunk of up to `chunk`
+// rays from the fetch set `set` (fetch_chunk: one atomic per wave and chunk,
+// spread over the queue's stripes) when the chunk runs out.  Per ray the
+// result is exactly traverse<ANY>'s.
 constexpr uint32_t kNoRay = 0xffffffffu;
 
 // load(i, o, d, tmin, tmax, nodes, tris) reads ray i and the acceleration
 // structure it is traced against; done(i, hit, h) consumes its result.
 template <bool ANY, class Load, class Done>
-PHX_DEV void traverse_stream(LaneStack &st, uint32_t *counter, uint32_t count, uint32_t chunk, uint32_t refill_min,
+PHX_DEV void traverse_stream(LaneStack &st, uint32_t *set, uint32_t count, uint32_t chunk, uint32_t refill_min,
                              Load load, Done done) {
   const uint32_t lane = __lane_id();
   uint32_t next = 0, end = 0;  // the wave's current chunk [next, end)
+  const uint32_t slen = stripe_len(count);
+  uint32_t stripe = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kStripes;
   bool exhausted = false;
   uint32_t idx = kNoRay;
   vec3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 1.f}, inv{0.f, 0.f, 0.f};
@@ -282,19 +286,14 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *counter, uint32_t count, u
     if (nidle == 64u && exhausted) break;
     if (!exhausted && (nidle >= refill_min || nidle == 64u)) {
       const uint32_t avail = end - next;
-      uint32_t nb = kNoRay;
-      if (nidle > avail) {
-        uint32_t b = 0;
-        if (lane == 0) b = atomicAdd(counter, chunk);
-        nb = __shfl(b, 0);
-        if (nb >= count) nb = kNoRay;
-      }
+      uint32_t nb = kNoRay, nbe = 0;  // a new chunk [nb, nbe)
+      if (nidle > avail) fetch_chunk(set, count, slen, chunk, stripe, nb, nbe);
       if (idx == kNoRay) {
         const uint32_t rank = uint32_t(__popcll(idle & ((uint64_t(1) << lane) - 1)));
         uint32_t i = kNoRay;
         if (rank < avail) i = next + rank;
-        else if (nb != kNoRay) i = nb + (rank - avail);
-        if (i < count) {
+        else if (nb != kNoRay && rank - avail < nbe - nb) i = nb + (rank - avail);
+        if (i != kNoRay) {
           float tmax;
           load(i, o, d, tmin, tmax, nodes, tris);
           h.tri = h.prim = 0xffffffffu;
@@ -314,13 +313,9 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *counter, uint32_t count, u
       if (nidle <= avail) {
         next += nidle;
       } else if (nb != kNoRay) {
-        next = nb + (nidle - avail);
-        end = nb + chunk;
+        next = min(nb + (nidle - avail), nbe);
+        end = nbe;
       } else {
-        next = end;
-        exhausted = true;
-      }
-      if (next >= count) {
         next = end;
         exhausted = true;
       }

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dev_queue.h"
 #include "dev_shade.h"
 
 namespace phx {
@@ -14,8 +15,9 @@ namespace phx {
 // iteration it + 1 (word ray_count_word(it + 1)) and the shadow queue that
 // iteration it fills (word shadow_count_word(it + 1)): the two words form one
 // 64-bit word, so the shade kernel claims its continuation and shadow slots
-// with one atomic per work-group.  The persistent kernels fetch work through
-// kFetch* + ring(it).  kNextSample is the next sample to start, kIter the
+// with one atomic per work-group.  The persistent traversal kernels fetch
+// work through the fetch sets fetch_set(kFetch*, it) (kFetchSetWords words:
+// one chunk counter per stripe of the queue, each in a 128-byte line).  kNextSample is the next sample to start, kIter the
 // current iteration, kTotal / kCapacity the run's sample count and path
 // slots.  The kernels read the iteration from kIter (so a captured sequence of
 // iterations can be replayed unchanged); k_run_start clears everything and
@@ -26,14 +28,14 @@ namespace phx {
 constexpr uint32_t kMaxIterations = 255;  // per path (RenderParams bound)
 constexpr uint32_t kRing = 16;
 constexpr uint32_t kQueuePairs = 0;  // 2 * kRing words
-constexpr uint32_t kFetchExtend = 64;
-constexpr uint32_t kFetchShadow = 128;
-constexpr uint32_t kRegenTake = 192;
-constexpr uint32_t kNextSample = 256;
+constexpr uint32_t kRegenTake = 64;
+constexpr uint32_t kNextSample = 128;
 constexpr uint32_t kIter = kNextSample + 1;
 constexpr uint32_t kTotal = kIter + 1;
 constexpr uint32_t kCapacity = kTotal + 1;
-constexpr uint32_t kCounterWords = kCapacity + 1;
+constexpr uint32_t kFetchExtend = 1024;                           // kRing fetch sets
+constexpr uint32_t kFetchShadow = kFetchExtend + kRing * kFetchSetWords;  // kRing fetch sets
+constexpr uint32_t kCounterWords = kFetchShadow + kRing * kFetchSetWords;
 // Shading by material group: the hit record's top 4 bits carry the hit's
 // group (material id mod the group count) and k_shade orders each work-group's
 // rays by group (misses last) before shading them.
@@ -42,6 +44,7 @@ constexpr uint32_t kMaxClasses = kMaxGroups + 1;
 __host__ __device__ inline uint32_t hit_tri(uint32_t w) { return w == 0xffffffffu ? w : (w & 0x0fffffffu); }
 __host__ __device__ inline uint32_t ring(uint32_t it) { return it & (kRing - 1); }
 __host__ __device__ inline uint32_t ray_count_word(uint32_t it) { return kQueuePairs + 2u * ring(it); }
+__host__ __device__ inline uint32_t fetch_set(uint32_t base, uint32_t it) { return base + ring(it) * kFetchSetWords; }
 __host__ __device__ inline uint32_t shadow_count_word(uint32_t it) { return kQueuePairs + 2u * ring(it) + 1u; }
 
 constexpr uint32_t kStatClosest = 0;

@@ -80,11 +80,12 @@ __global__ void k_commit(StreamQueues q, uint32_t ahead) {
     q.counters[ray_count_word(it)] += take;
     q.counters[kNextSample] += take;
     // entries of iteration it + 1 (last used by it + 1 - kRing)
-    const uint32_t n = ring(it + 1);
     q.counters[ray_count_word(it + 1)] = 0u;
     q.counters[shadow_count_word(it + 1)] = 0u;
-    q.counters[kFetchExtend + n] = 0u;
-    q.counters[kFetchShadow + n] = 0u;
+    for (uint32_t k = 0; k < kStripes; ++k) {
+      q.counters[fetch_set(kFetchExtend, it + 1) + k * kStripeStride] = 0u;
+      q.counters[fetch_set(kFetchShadow, it + 1) + k * kStripeStride] = 0u;
+    }
     q.counters[kIter] = it;
   }
 }
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
   LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   uint32_t live = 0;
   traverse_stream<false>(
-      st, &q.counters[kFetchExtend + ring(it)], count, chunk, refill,
+      st, &q.counters[fetch_set(kFetchExtend, it)], count, chunk, refill,
       [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const BvhNode *&nodes, const TriRec *&tris) {
         const float4 a = q.o[buf][i], b = q.d[buf][i];
         o = xyz(a);
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chun
   LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   uint32_t live = 0;
   traverse_stream<true>(
-      st, &q.counters[kFetchShadow + ring(it)], count, chunk, refill,
+      st, &q.counters[fetch_set(kFetchShadow, it)], count, chunk, refill,
       [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const BvhNode *&nodes, const TriRec *&tris) {
         const float4 a = q.sh_o[i], b = q.sh_d[i];
         o = xyz(a);
