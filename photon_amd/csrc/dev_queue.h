@@ -80,6 +80,24 @@ __device__ __forceinline__ void block_alloc_pair(unsigned long long *counter, bo
   slot_hi = uint32_t(base >> 32) + oh + uint32_t(__popcll(mh & below));
 }
 
+// Adds the sum of `n` over the work-group's threads to a 64-bit statistic,
+// one atomic per work-group.  Every thread of the work-group must call it (it
+// synchronises the work-group).
+template <int kWaves>
+__device__ __forceinline__ void block_sum(unsigned long long *counter, uint32_t n) {
+  __shared__ uint32_t s_sum[kWaves];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+  if (lane_id() == 0) s_sum[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += s_sum[w];
+    if (t) atomicAdd(counter, (unsigned long long)t);
+  }
+}
+
 // Adds the sum over the wave's lanes of `n` to a 64-bit statistic, one atomic
 // per wave.  Every lane of the wave must call it.
 __device__ __forceinline__ void wave_sum(unsigned long long *counter, uint32_t n) {

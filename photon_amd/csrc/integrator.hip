@@ -72,9 +72,13 @@ hipError_t launch_em_records(const DevScene &s, float4 *out, hipStream_t st) {
 }
 
 // Commits the regenerated paths, clears the ring entries of the following
-// iteration and (ahead = 1) advances the iteration.
-__global__ void k_commit(StreamQueues q, uint32_t ahead) {
+// iteration and (ahead = 1) advances the iteration.  With count_shade, the
+// shading events of the iteration that just ran (its ray queue) go into the
+// statistics here, by one thread, rather than by an atomic per work-group of
+// the shade kernel on one shared word.
+__global__ void k_commit(StreamQueues q, uint32_t ahead, uint32_t count_shade) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (ahead && count_shade) q.stats[kStatShade] += q.counters[ray_count_word(q.counters[kIter])];
     const uint32_t it = q.counters[kIter] + ahead;
     const uint32_t take = q.counters[kRegenTake + ring(it)];
     q.counters[ray_count_word(it)] += take;
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
         if (hit && groups) w |= (ray_material(v, q.st[buf][i].x, h.prim) % groups) << 28;
         q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(w));
       });
-  wave_sum(&q.stats[kStatClosest], live);
+  block_sum<B / 64>(&q.stats[kStatClosest], live);
 }
 
 template <class P, bool SORTED>
@@ -208,7 +212,6 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
     // the reference's loop bound (pathtracer.cpp:682,762)
     if (out.next.path_it >= r.sp.max_it) out.cont = false;
   }
-  if (threadIdx.x == 0) atomicAdd(&q.stats[kStatShade], (unsigned long long)min(256u, count - tile));
   uint32_t ci = 0, si = 0;
   block_alloc_pair<4>(reinterpret_cast<unsigned long long *>(&q.counters[ray_count_word(it + 1)]), out.cont, ci,
                       out.shadow, si);
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chun
         // occluded entries lose their sample id; k_nee_accumulate adds the rest
         if (occ) q.sh_c[i].w = __uint_as_float(kNoRay);
       });
-  wave_sum(&q.stats[kStatShadow], live);
+  block_sum<B / 64>(&q.stats[kStatShadow], live);
 }
 
 template <class P>
@@ -433,7 +436,8 @@ hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead
     hipLaunchKernelGGL(k_regen<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q, ahead, zero_L ? 1u : 0u);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, q, ahead);
+    // the forward pass (zero_L) counts its shading events, the adjoint replay does not
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, q, ahead, zero_L ? 1u : 0u);
     return hipGetLastError();
   });
 }
