@@ -340,7 +340,8 @@ int phx_result_device_color(phx_context *ctx, const float **d_color);
  * summation order).  `scene` names the context holding the view's scene
  * (phx_set_scene); it must live on ctx's device and stay alive during the
  * call; it may be ctx itself.  With phx_tuning.use_graph = 1 the bounce loop
- * is captured once as a hipGraph and replayed. */
+ * is captured once as a hipGraph and replayed.  At most 32768 views per
+ * call. */
 typedef struct phx_view {
   phx_context *scene;
   phx_camera camera;

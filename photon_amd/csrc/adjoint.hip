@@ -128,7 +128,7 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
     if (valid) {
       const uint4 stv = q.st[buf][i];
       g = stv.x;
-      const RunItem &r = v.item_at(v.find(g));
+      const RunItem &r = v.item_at(stv.y >> kItemShift);
       const DevScene &s = v.scene_of(r);
       const PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], stv);
       const float4 hit = q.hit[i];
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
 #pragma unroll
       for (int k = 0; k < kGradParams; ++k) gv[k] = 0.f;
       if (out.shadow) {
-        const DevScene sc = trace_scene(v, g);
+        const DevScene sc = trace_scene(v, stv.y >> kItemShift);
         TraceHit th;
         if (!traverse<true>(sc, out.so, out.sd, 0.f, out.stmax, st, th)) {
           vec3 contrib = out.sc;

@@ -37,6 +37,7 @@ struct PathState {
   uint32_t kinds;
   bool primary;
   float path_dist;
+  uint32_t item;  // the path's item (view, spp range) in the run's item table
 };
 
 struct ShadeOut {

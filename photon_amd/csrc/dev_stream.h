@@ -26,12 +26,13 @@ PHX_DEV PathState unpack(float4 o, float4 d, float4 tp, uint4 st) {
   p.depth = st.y & 0xffu;
   p.path_it = (st.y >> 8) & 0xffu;
   p.primary = (st.y & kPrimaryBit) != 0u;
+  p.item = st.y >> kItemShift;
   p.kinds = st.z;
   p.path_dist = __uint_as_float(st.w);
   return p;
 }
 PHX_DEV uint32_t pack_flags(const PathState &p) {
-  return (p.depth & 0xffu) | ((p.path_it & 0xffu) << 8) | (p.primary ? kPrimaryBit : 0u);
+  return (p.depth & 0xffu) | ((p.path_it & 0xffu) << 8) | (p.primary ? kPrimaryBit : 0u) | (p.item << kItemShift);
 }
 
 // Camera ray of sample s of a pixel (pathtracer.cpp:727-759).
@@ -52,16 +53,18 @@ PHX_DEV void camera_path(const ShadeParams &w, uint32_t pixel, uint32_t sample, 
   p.kinds = 0;
   p.primary = true;
   p.path_dist = 0.f;
+  p.item = 0;
 }
 
 // Traversal scene of a lane: the uniform kernel-argument scene, or the two
-// pointers traversal reads, copied out of the lane's scene-table entry.
+// pointers traversal reads, copied out of the scene-table entry of the lane's
+// item.
 template <class P>
-PHX_DEV DevScene trace_scene(const P &v, uint32_t g) {
+PHX_DEV DevScene trace_scene(const P &v, uint32_t item) {
   if constexpr (std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value) {
     return v.scene;
   } else {
-    const DevScene &sc = v.scene_of(v.item_at(v.find(g)));
+    const DevScene &sc = v.scene_of(v.item_at(item));
     DevScene ls{};
     ls.nodes = sc.nodes;
     ls.tris = sc.tris;

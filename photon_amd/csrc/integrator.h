@@ -65,6 +65,12 @@ struct RunItem {
   uint32_t view;         // view index (diagnostics)
 };
 
+// A path's item index rides in the flags word of its queue entry (bits
+// kItemShift and up) and in its shadow ray's origin w (bits 8 and up), so the
+// kernels index the item table directly instead of searching it by sample id.
+constexpr uint32_t kItemShift = 17;
+constexpr uint32_t kMaxRunItems = 1u << (32 - kItemShift);
+
 // View policies.  OneItem: a single view and scene passed as kernel
 // arguments (uniform, scalar loads).  ItemTable: many views in device tables,
 // looked up per lane from the sample id; kOneScene keeps the scene a kernel

@@ -46,10 +46,12 @@ __global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahe
   const int buf = it & 1;
   for (uint32_t i = tid; i < take; i += gridDim.x * blockDim.x) {
     const uint32_t g = next + i;
-    const RunItem &r = v.item_at(v.find(g));
+    const uint32_t item = v.find(g);
+    const RunItem &r = v.item_at(item);
     const uint32_t l = g - r.first;
     PathState p;
     camera_path(r.sp, l % r.sp.npix, r.sample_base + l / r.sp.npix, p);
+    p.item = item;
     const uint32_t slot = used + i;
     q.o[buf][slot] = f4(p.ro, p.tmin);
     q.d[buf][slot] = f4(p.rd, p.tmax);
@@ -95,27 +97,27 @@ __global__ void k_commit(StreamQueues q, uint32_t ahead, uint32_t count_shade) {
 }
 
 // The acceleration structure of a lane's ray: the kernel-argument scene, or the
-// ray's view's scene-table entry.
+// scene-table entry of the ray's item.
 template <class P>
-PHX_DEV void ray_accel(const P &v, uint32_t g, const BvhNode *&nodes, const TriRec *&tris) {
+PHX_DEV void ray_accel(const P &v, uint32_t item, const BvhNode *&nodes, const TriRec *&tris) {
   if constexpr (std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value) {
     nodes = v.scene.nodes;
     tris = v.scene.tris;
   } else {
-    const DevScene &sc = v.scene_of(v.item_at(v.find(g)));
+    const DevScene &sc = v.scene_of(v.item_at(item));
     nodes = sc.nodes;
     tris = sc.tris;
   }
 }
 
-// Material id of primitive `prim` in the scene of the ray with sample id g.
+// Material id of primitive `prim` in the scene of the ray's item.
 template <class P>
-PHX_DEV uint32_t ray_material(const P &v, uint32_t g, uint32_t prim) {
+PHX_DEV uint32_t ray_material(const P &v, uint32_t item, uint32_t prim) {
   const uint32_t *mids;
   if constexpr (std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value) {
     mids = v.scene.material_ids;
   } else {
-    mids = v.scene_of(v.item_at(v.find(g))).material_ids;
+    mids = v.scene_of(v.item_at(item)).material_ids;
   }
   return mids ? mids[prim] : 0u;
 }
@@ -143,11 +145,11 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
         tmin = a.w;
         tmax = b.w;
         live += tmax > tmin ? 1u : 0u;
-        ray_accel(v, q.st[buf][i].x, nodes, tris);
+        ray_accel(v, q.st[buf][i].y >> kItemShift, nodes, tris);
       },
       [&](uint32_t i, bool hit, const TraceHit &h) {
         uint32_t w = h.tri;
-        if (hit && groups) w |= (ray_material(v, q.st[buf][i].x, h.prim) % groups) << 28;
+        if (hit && groups) w |= (ray_material(v, q.st[buf][i].y >> kItemShift, h.prim) % groups) << 28;
         q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(w));
       });
   block_sum<B / 64>(&q.stats[kStatClosest], live);
@@ -191,10 +193,14 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
   out.cont = false;
   out.shadow = false;
   uint32_t g = 0;
-  if (valid) {
-    const uint4 stv = q.st[buf][i];
-    g = stv.x;
-    const RunItem &r = v.item_at(v.find(g));
+  uint4 stv = make_uint4(0u, 0u, 0u, 0u);
+  if (valid) stv = q.st[buf][i];
+  g = stv.x;
+  // One shading event.  (Looping over the wave's distinct items to make the
+  // item wave-uniform, so the view and scene would load through scalar loads,
+  // measured slower: the loop raised the spill traffic.)
+  auto shade_one = [&](uint32_t item) {
+    const RunItem &r = v.item_at(item);
     const DevScene &s = v.scene_of(r);
     const PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], stv);
     const float4 hit = q.hit[i];
@@ -211,7 +217,8 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
     if (out.primary_event) atomicMin(&q.aov_s[r.pix_base + pixel], r.s0 + s_local);
     // the reference's loop bound (pathtracer.cpp:682,762)
     if (out.next.path_it >= r.sp.max_it) out.cont = false;
-  }
+  };
+  if (valid) shade_one(std::is_same<P, OneItem>::value ? 0u : stv.y >> kItemShift);
   uint32_t ci = 0, si = 0;
   block_alloc_pair<4>(reinterpret_cast<unsigned long long *>(&q.counters[ray_count_word(it + 1)]), out.cont, ci,
                       out.shadow, si);
@@ -224,7 +231,7 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
   }
   if (out.shadow) {
     // the shading path's iteration keys the transmittance stream
-    q.sh_o[si] = f4(out.so, __uint_as_float(out.next.path_it - 1u));
+    q.sh_o[si] = f4(out.so, __uint_as_float((out.next.path_it - 1u) | (out.next.item << 8)));
     q.sh_d[si] = f4(out.sd, out.stmax);
     q.sh_c[si] = f4(out.sc, __uint_as_float(g));
   }
@@ -264,7 +271,7 @@ __global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chun
         tmin = 0.f;
         tmax = b.w;
         live += tmax > 0.f ? 1u : 0u;
-        ray_accel(v, __float_as_uint(q.sh_c[i].w), nodes, tris);
+        ray_accel(v, __float_as_uint(a.w) >> 8, nodes, tris);
       },
       [&](uint32_t i, bool occ, const TraceHit &) {
         // occluded entries lose their sample id; k_nee_accumulate adds the rest
@@ -279,12 +286,13 @@ __device__ __forceinline__ void nee_accumulate_one(const P &v, const StreamQueue
   const uint32_t g = __float_as_uint(c.w);
   if (g == kNoRay) return;
   vec3 contrib = xyz(c);
-  const RunItem &r = v.item_at(v.find(g));
+  const uint32_t key = __float_as_uint(reinterpret_cast<const float *>(&q.sh_o[i])[3]);  // path_it | item << 8
+  const RunItem &r = v.item_at(key >> 8);
   const DevScene &s = v.scene_of(r);
   if (s.vol_count > 0) {
     const float4 o = q.sh_o[i], d = q.sh_d[i];
     const uint32_t l = g - r.first;
-    Rng rng = make_rng(l % r.sp.npix, r.sample_base + l / r.sp.npix, 0x10000u + __float_as_uint(o.w));
+    Rng rng = make_rng(l % r.sp.npix, r.sample_base + l / r.sp.npix, 0x10000u + (key & 0xffu));
     contrib = contrib * vol_transmittance(s, xyz(o), xyz(d), 0.f, d.w, rng);
   }
   if (finite3(contrib)) {

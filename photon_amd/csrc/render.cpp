@@ -462,6 +462,8 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
 extern "C" int phx_render_batch(phx_context *c, const phx_view *views, uint32_t count) {
   if (!c || (count && !views)) return PHX_ERR_INVALID;
   if (count == 0) return fail(c, PHX_ERR_INVALID, "render_batch: no views");
+  // a view adds at most one item to a run, and items are numbered in 15 bits
+  if (count > kMaxRunItems) return fail(c, PHX_ERR_INVALID, "render_batch: more than 32768 views");
   int r;
   std::vector<const phx_context *> scenes;
   std::vector<ViewJob> jobs;
