@@ -608,9 +608,22 @@ PHX_DEV float vol_track(const DevScene &s, const phx_volume &v, vec3 org, vec3 d
     }
     t = t_exit;
     if (t_exit >= t1 || lookups >= 8192) break;
-    cell[axis] += step[axis];
-    if (cell[axis] < 0 || cell[axis] >= int(v.macro_dims[axis])) break;
-    t_next[axis] += t_delta[axis];
+    // the step along `axis`, with selects: an index computed at run time into
+    // these arrays (or into v.macro_dims) would put them in scratch memory
+    const bool a0 = axis == 0, a1 = axis == 1;
+    const int ca = (a0 ? cell[0] : (a1 ? cell[1] : cell[2])) + (a0 ? step[0] : (a1 ? step[1] : step[2]));
+    const int ma = int(a0 ? v.macro_dims[0] : (a1 ? v.macro_dims[1] : v.macro_dims[2]));
+    if (ca < 0 || ca >= ma) break;
+    if (a0) {
+      cell[0] = ca;
+      t_next[0] += t_delta[0];
+    } else if (a1) {
+      cell[1] = ca;
+      t_next[1] += t_delta[1];
+    } else {
+      cell[2] = ca;
+      t_next[2] += t_delta[2];
+    }
   }
   return -1.f;
 }
