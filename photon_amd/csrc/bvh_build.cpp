@@ -8,6 +8,7 @@
 // tests every triangle whose leaf box the ray reaches, with the same
 // Moller-Trumbore arithmetic.
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -48,7 +49,6 @@ struct BinNode {
 };
 
 constexpr int kBins = 16;
-constexpr float kTraversalCost = 1.0f;
 constexpr float kTriangleCost = 1.0f;
 
 struct Builder {
@@ -60,6 +60,7 @@ struct Builder {
   std::vector<BinNode> nodes;
   int max_depth;
   uint32_t max_leaf;
+  float traversal_cost;
 
   int make_node(uint32_t first, uint32_t count, const Box &box) {
     BinNode n;
@@ -136,7 +137,7 @@ struct Builder {
     }
     const float parent_area = box.area();
     const float split_cost =
-        kTraversalCost + kTriangleCost * (parent_area > 0.f ? best_cost / parent_area : float(count));
+        traversal_cost + kTriangleCost * (parent_area > 0.f ? best_cost / parent_area : float(count));
     const float leaf_cost = kTriangleCost * float(count);
     if (count <= max_leaf && (best_axis < 0 || leaf_cost <= split_cost)) return 0;
     if (best_axis < 0) return median_split(axis);
@@ -238,6 +239,7 @@ int build_bvh(const float *positions, uint32_t vertex_count, const uint32_t *ind
   b.pos = positions;
   b.idx = indices;
   b.max_leaf = opt.max_leaf;
+  b.traversal_cost = opt.traversal_cost;
   b.max_depth = opt.max_binary_depth;
   b.pbox.resize(triangle_count);
   b.cent.resize(size_t(triangle_count) * 3);
@@ -258,10 +260,59 @@ int build_bvh(const float *positions, uint32_t vertex_count, const uint32_t *ind
   }
   b.build(triangle_count);
 
-  // Collapse: each 8-wide node takes its binary node's children and keeps
-  // opening the inner child of largest surface area until it has 8.  Inner
-  // children are numbered consecutively; the leaf children's triangles are
+  // Collapse.  Cost-optimal (opt.collapse = 1), the wide-BVH dynamic program
+  // of Ylitie, Karras and Laine (HPG 2017): F[i][j] is the least expected cost
+  // of representing binary subtree i by exactly j slots of its parent (j = 1:
+  // one leaf, or one new wide node), each node visit and triangle test
+  // weighted by the surface area of what is entered; a wide node made from i
+  // takes the slot count m of least cost G[i][m].  Greedy (opt.collapse = 0):
+  // each wide node takes its binary node's children and keeps opening the
+  // inner child of largest surface area until it has 8.  Either way inner
+  // children are numbered consecutively and the leaf children's triangles are
   // appended in slot order, so a node's leaves are one run of records.
+  constexpr int W = kNodeWidth;
+  const size_t nbin = b.nodes.size();
+  std::vector<float> F, G;
+  std::vector<uint8_t> split_of, node_slots;
+  if (opt.collapse) {
+    F.assign(nbin * (W + 1), INFINITY);
+    G.assign(nbin * (W + 1), INFINITY);
+    split_of.assign(nbin * (W + 1), 0);
+    node_slots.assign(nbin, 0);
+    for (size_t i = nbin; i-- > 0;) {  // children follow their parent: reverse order is post-order
+      const BinNode &n = b.nodes[i];
+      float *Fi = &F[i * (W + 1)], *Gi = &G[i * (W + 1)];
+      if (n.left < 0) {
+        Fi[1] = n.box.area() * kTriangleCost * float(n.count);
+        continue;
+      }
+      const float *Fl = &F[size_t(n.left) * (W + 1)], *Fr = &F[size_t(n.right) * (W + 1)];
+      for (int m = 2; m <= W; ++m)
+        for (int j1 = 1; j1 < m; ++j1) {
+          const float c = Fl[j1] + Fr[m - j1];
+          if (c < Gi[m]) {
+            Gi[m] = c;
+            split_of[i * (W + 1) + m] = uint8_t(j1);
+          }
+        }
+      int bm = 2;
+      for (int m = 3; m <= W; ++m)
+        if (Gi[m] < Gi[bm]) bm = m;
+      node_slots[i] = uint8_t(bm);
+      Fi[1] = n.box.area() * opt.node_cost + Gi[bm];
+      for (int m = 2; m <= W; ++m) Fi[m] = Gi[m];
+    }
+  }
+  // the binary subtrees that fill the j slots of subtree i, in order
+  std::function<void(int, int, int *, int &)> slots_of = [&](int i, int j, int *kids, int &nk) {
+    if (j == 1) {
+      kids[nk++] = i;
+      return;
+    }
+    const int j1 = split_of[size_t(i) * (W + 1) + j];
+    slots_of(b.nodes[i].left, j1, kids, nk);
+    slots_of(b.nodes[i].right, j - j1, kids, nk);
+  };
   auto tri_rec = [&](uint32_t p) {
     const float *v0 = positions + size_t(indices[size_t(p) * 3]) * 3;
     const float *v1 = positions + size_t(indices[size_t(p) * 3 + 1]) * 3;
@@ -291,11 +342,13 @@ int build_bvh(const float *positions, uint32_t vertex_count, const uint32_t *ind
     int nk = 0;
     if (b.nodes[j.bin].left < 0) {
       kids[nk++] = j.bin;  // the root is a leaf
+    } else if (opt.collapse) {
+      slots_of(j.bin, node_slots[size_t(j.bin)], kids, nk);
     } else {
       kids[nk++] = b.nodes[j.bin].left;
       kids[nk++] = b.nodes[j.bin].right;
     }
-    while (nk < kNodeWidth) {
+    while (!opt.collapse && nk < kNodeWidth) {
       int best = -1;
       float best_area = -1.f;
       for (int k = 0; k < nk; ++k) {

@@ -13,6 +13,16 @@ namespace phx {
 struct BvhOptions {
   uint32_t max_leaf = 4;          // triangles per leaf (the reference uses <= 4, bvh.cpp:42)
   int max_binary_depth = 40;      // bound on the binary tree depth before collapsing
+  // SAH cost of a binary split relative to one triangle test.  On gfx950 an
+  // 8-wide node visit costs several triangle tests, so the split must save
+  // more than one test to pay for itself.
+  float traversal_cost = 1.0f;
+  // Collapse to 8-wide nodes: 1 = cost-optimal (dynamic program over the
+  // binary tree), 0 = greedy (open the largest-area inner child until 8).
+  int collapse = 1;
+  // Expected cost of a wide-node visit relative to one triangle test in the
+  // cost-optimal collapse (both are one traversal iteration).
+  float node_cost = 1.0f;
 };
 
 struct Bvh {
