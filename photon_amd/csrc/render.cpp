@@ -48,9 +48,10 @@ namespace {
 using namespace phx;
 using namespace phx_detail;
 
-// Per-sample radiance buffer budget of one streaming run (16 B per sample):
-// larger jobs are rendered in several runs of whole samples per pixel.
-constexpr uint64_t kSampleBudget = uint64_t(1) << 27;
+// Per-sample radiance buffer budget of one streaming run (16 B per sample,
+// 4 GB at most): larger jobs are rendered in several runs of whole samples
+// per pixel, each paying the queue's drain at its end.
+constexpr uint64_t kSampleBudget = uint64_t(1) << 28;
 // Iterations launched (or replayed as one graph) between two reads of the
 // queue counters.
 constexpr uint32_t kIterBatch = 8;
@@ -552,7 +553,7 @@ extern "C" int phx_autotune(phx_context *c, const phx_view *views, uint32_t coun
       {&phx_tuning::queue_chunk, {64, 128, 256, 512, 1024}},
       {&phx_tuning::refill_lanes, {8, 16, 32, 64}},
       {&phx_tuning::sort_by_material, {0, 4, 8, 15}},
-      {&phx_tuning::paths_in_flight, {1u << 23, 1u << 24, 1u << 25}},
+      {&phx_tuning::paths_in_flight, {1u << 23, 1u << 24, 1u << 25, 1u << 26}},
   };
   for (const auto &k : knobs) {
     const uint32_t was = cur.*(k.first);
