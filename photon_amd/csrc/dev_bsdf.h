@@ -288,7 +288,7 @@ PHX_DEV vec3 ggx_half(vec3 n, float a, Rng &rng) {
   const float ct = sqrtf((1.f - r1) / (1.f + (a2 - 1.f) * r1));
   const float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
   const float phi = 2.f * kPi * r2;
-  return normalize(t * (st * cosf(phi)) + b * (st * sinf(phi)) + n * ct);
+  return normalize(t * (st * __cosf(phi)) + b * (st * __sinf(phi)) + n * ct);
 }
 // disney_bsdf.h:381-521.  Delta samples carry pdf = 1 and the path weight in f.
 PHX_DEV BsdfSample bsdf_sample(const phx_material &m, vec3 wo, vec3 n, vec3 sn, Rng &rng) {
@@ -352,7 +352,8 @@ PHX_DEV BsdfSample bsdf_sample(const phx_material &m, vec3 wo, vec3 n, vec3 sn, 
       F = fresnel_dielectric(cos_i, eta);
     } else {
       const float f0 = dielectric_f0(m.ior);
-      F = f0 + (1.f - f0) * powf(1.f - cos_i, 5.f);
+      const float m1 = 1.f - cos_i, m2 = m1 * m1;  // (1 - cos_i)^5
+      F = f0 + (1.f - f0) * (m2 * m2 * m1);
     }
     const float lw = te / pt;
     out.pdf = 1.f;

@@ -182,8 +182,8 @@ PHX_DEV vec2 dir_to_uv(vec3 d) {
 PHX_DEV vec3 uv_to_dir(vec2 uv) {
   const float phi = uv.x * kTwoPi - kPi;
   const float theta = uv.y * kPi;
-  const float st = sinf(theta);
-  return {st * cosf(phi), cosf(theta), st * sinf(phi)};
+  const float st = __sinf(theta);
+  return {st * __cosf(phi), __cosf(theta), st * __sinf(phi)};
 }
 PHX_DEV void env_texel(const DevScene &s, vec3 dir, uint32_t &x, uint32_t &y, vec2 &uv) {
   uv = dir_to_uv(mat3mul(s.w2t, dir));
@@ -235,7 +235,7 @@ PHX_DEV vec3 env_sample(const DevScene &s, Rng &rng, float &pdf) {
   const float u = (float(col) + rng.next()) / float(s.env_w);
   const float v = (float(row) + rng.next()) / float(s.env_h);
   const vec3 dir = mat3mul(s.t2w, uv_to_dir(vec2{u, v}));
-  const float st = sinf(v * kPi);
+  const float st = __sinf(v * kPi);
   pdf = (rp * cp) * float(s.env_w) * float(s.env_h) / (kTwoPi * kPi * fmaxf(st, 1e-6f));
   return dir;
 }
