@@ -76,6 +76,25 @@ def test_trace_closest_cornell_vs_brute_force(gpu_ctx):
     assert exact == 1.0  # same float operations as the reference triangle test
 
 
+def test_trace_batch_sizes_across_stripes(gpu_ctx):
+    # The persistent traversal cuts the ray batch into 16 stripes of a
+    # multiple of 64 rays and clips chunks at stripe ends: batches smaller
+    # than one stripe, ragged sizes and sizes just past a stripe boundary must
+    # each trace every ray once.
+    torch = _torch()
+    s = scenes.cornell_box()
+    b = _backend(gpu_ctx, s)
+    org, d, tmin, tmax = scenes.random_rays(s, 70000, seed=29)
+    ref = oracle.OracleScene(s, brute=True).trace_closest(org, d, tmin, tmax, brute=True)
+    for n in (1, 63, 64, 65, 1000, 1023, 1025, 4097, 16 * 64 + 1, 65537, 70000):
+        gpu = _gpu_closest(b, org[:n], d[:n], tmin[:n], tmax[:n])
+        np.testing.assert_array_equal(gpu[1], ref[1][:n], err_msg=f"closest hits, n = {n}")
+        occ = torch.full((n,), 7, dtype=torch.int32, device="cuda")
+        b.trace_occluded(_dev(org[:n]), _dev(d[:n]), _dev(tmin[:n]), _dev(tmax[:n]), occ)
+        b.ctx.synchronize()
+        np.testing.assert_array_equal(occ.cpu().numpy() != 0, ref[1][:n] != NONE, err_msg=f"occluded, n = {n}")
+
+
 def test_trace_closest_soup_edge_cases(gpu_ctx):
     # inactive rays (tmax <= tmin), t = tmax inclusive, zero-length batches
     s = two_quads()
