@@ -149,7 +149,8 @@ __global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chun
       },
       [&](uint32_t i, bool hit, const TraceHit &h) {
         uint32_t w = h.tri;
-        if (hit && groups) w |= (ray_material(v, q.st[buf][i].y >> kItemShift, h.prim) % groups) << 28;
+        // one group: the shade kernel only parts hits from misses, no tag needed
+        if (hit && groups > 1) w |= (ray_material(v, q.st[buf][i].y >> kItemShift, h.prim) % groups) << 28;
         q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(w));
       });
   block_sum<B / 64>(&q.stats[kStatClosest], live);
