@@ -101,8 +101,10 @@ PHX_DEV void grad_route(const float g[kGradParams], const AdjHooks &h, const Dev
 // Replay of one wavefront iteration: pt_shade with derivative observers,
 // inline NEE visibility, gradient accumulation (per work-group in LDS for the
 // first lds_mats materials, flushed once), continuing paths to the next queue.
+// The work-group steps through tiles of B rays together: one striped fetch
+// and one slot claim per tile, not one per wave.
 template <class P, int B>
-__global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, AdjointArgs a, uint32_t chunk) {
+__global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, AdjointArgs a) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[ray_count_word(it)];
   if (count == 0) return;
@@ -113,14 +115,18 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
   const int buf = it & 1, nbuf = buf ^ 1;
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
-  const uint32_t lane = lane_id();
-  WorkCursor cur = first_chunk(chunk);
+  // the replay runs no shadow kernel: its fetch set serves this loop
+  uint32_t *set = &q.counters[fetch_set(kFetchShadow, it)];
+  const uint32_t slen = stripe_len(count);
+  uint32_t stripe = blockIdx.x % kStripes;  // thread 0's cursor
+  __shared__ uint32_t s_tile[2];
   while (true) {
-    // the replay runs no shadow kernel: its fetch counter serves this loop
-    const uint32_t base = next_batch(cur, &q.counters[fetch_set(kFetchShadow, it)], count, chunk);
-    if (base >= count) break;
-    const uint32_t i = base + lane;
-    const bool valid = i < count;
+    if (threadIdx.x == 0) fetch_tile(set, count, slen, B, stripe, s_tile[0], s_tile[1]);
+    __syncthreads();
+    const uint32_t base = s_tile[0], end = s_tile[1];
+    if (base == 0xffffffffu) break;  // uniform over the work-group
+    const uint32_t i = base + threadIdx.x;
+    const bool valid = i < end;
     ShadeOut out;
     out.cont = false;
     out.shadow = false;
@@ -176,7 +182,10 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
       if (h.nee_on || h.bsdf_on) grad_route(gv, h, s, a, glds);
       if (out.next.path_it >= r.sp.max_it) out.cont = false;
     }
-    const uint32_t ci = wave_alloc(&q.counters[ray_count_word(it + 1)], out.cont);
+    // (s_tile is read above before this claim's barriers; thread 0 rewrites it after them)
+    uint32_t ci = 0, unused = 0;
+    block_alloc_pair<B / 64>(reinterpret_cast<unsigned long long *>(&q.counters[ray_count_word(it + 1)]), out.cont, ci,
+                             false, unused);
     if (out.cont) {
       const PathState &n = out.next;
       q.o[nbuf][ci] = f4(n.ro, n.tmin);
@@ -216,9 +225,9 @@ hipError_t launch_shade_adjoint(const ViewArgs &v, const StreamQueues &q, const 
   auto go = [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
     if (ls.trace_block == 128)
-      hipLaunchKernelGGL((k_shade_adjoint<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, a, ls.chunk);
+      hipLaunchKernelGGL((k_shade_adjoint<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, a);
     else
-      hipLaunchKernelGGL((k_shade_adjoint<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, a, ls.chunk);
+      hipLaunchKernelGGL((k_shade_adjoint<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, a);
     return hipGetLastError();
   };
   switch (v.kind) {

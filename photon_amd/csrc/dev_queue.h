@@ -114,13 +114,7 @@ __device__ __forceinline__ void wave_count(unsigned long long *counter, bool pre
   if (int(lane_id()) == leader) atomicAdd(counter, (unsigned long long)__popcll(mask));
 }
 
-// Work distribution of the persistent kernels: each wave owns a chunk of
-// `chunk` items (a multiple of 64) at a time.  The first chunk of every wave is
-// static (wave index * chunk); later chunks come from one atomic per wave and
-// chunk on a counter that starts after the static ones.  (A plain atomic at
-// L2: a coherent pre-read of the counter would bypass L2 and serialize at
-// memory.)
-constexpr uint32_t kChunk = 256;  // default chunk
+constexpr uint32_t kChunk = 256;  // chunk of the seam-A traversal kernels and the resolve pass
 
 // Striped work fetch of the persistent traversal kernels.  The queue is cut
 // into kStripes stripes of equal length (a multiple of 64); each stripe has
@@ -158,6 +152,32 @@ __device__ __forceinline__ void fetch_chunk(uint32_t *set, uint32_t count, uint3
     stripe = stripe + 1u == kStripes ? 0u : stripe + 1u;
   }
 }
+// fetch_chunk for one thread (a work-group's leader fetching the group's tile
+// of n items).
+__device__ __forceinline__ void fetch_tile(uint32_t *set, uint32_t count, uint32_t slen, uint32_t n, uint32_t &stripe,
+                                           uint32_t &lo, uint32_t &hi) {
+  lo = 0xffffffffu;
+  hi = 0u;
+  for (uint32_t t = 0; t < kStripes; ++t) {
+    const uint32_t s_lo = stripe * slen;
+    const uint32_t s_hi = min(s_lo + slen, count);
+    if (s_lo < s_hi) {
+      const uint32_t b = atomicAdd(&set[stripe * kStripeStride], n);
+      if (b < s_hi - s_lo) {
+        lo = s_lo + b;
+        hi = min(lo + n, s_hi);
+        return;
+      }
+    }
+    stripe = stripe + 1u == kStripes ? 0u : stripe + 1u;
+  }
+}
+
+// Work distribution of the resolve pass: each wave owns a chunk of `chunk`
+// items (a multiple of 64) at a time.  The first chunk of every wave is static
+// (wave index * chunk); later chunks come from one atomic per wave and chunk on
+// a counter that starts after the static ones.  (A plain atomic at L2: a
+// coherent pre-read of the counter would bypass L2 and serialize at memory.)
 struct WorkCursor {
   uint32_t base;  // first item of the current chunk
   uint32_t sub;   // next 64-item sub-batch within the chunk
