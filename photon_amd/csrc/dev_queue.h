@@ -1,4 +1,5 @@
-// Wave-level queue helpers: one atomic per wave (ballot + lane rank).
+// Queue helpers of the wavefront kernels: work-group slot claims, statistics
+// sums and the striped work fetch of the persistent kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,44 +9,15 @@ namespace phx {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Slot in `counter`'s queue for every lane with `pred`; lanes without pred get
-// an unspecified value.  Every active lane of the wave must call it.
-__device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, bool pred) {
-  const uint64_t mask = __ballot(pred);
-  if (mask == 0) return 0;
-  const uint32_t lane = lane_id();
-  const uint32_t rank = __popcll(mask & ((uint64_t(1) << lane) - 1));
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (int(lane) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-  base = __shfl(base, leader);
-  return base + rank;
-}
-
-// wave_alloc on the two halves of a 64-bit counter (two queues counted side
-// by side): one atomic per wave claims the slots of both.  A half never
-// carries into the other as long as it stays below 2^32.  Every active lane of
-// the wave must call it.
-__device__ __forceinline__ void wave_alloc_pair(unsigned long long *counter, bool pred_lo, uint32_t &slot_lo,
-                                                bool pred_hi, uint32_t &slot_hi) {
-  const uint64_t ml = __ballot(pred_lo), mh = __ballot(pred_hi);
-  if ((ml | mh) == 0) return;
-  const uint32_t lane = lane_id();
-  const uint64_t below = (uint64_t(1) << lane) - 1;
-  const int leader = __ffsll((unsigned long long)(ml | mh)) - 1;
-  unsigned long long base = 0;
-  if (int(lane) == leader)
-    base = atomicAdd(counter, (unsigned long long)__popcll(ml) | ((unsigned long long)__popcll(mh) << 32));
-  const uint32_t lo = __shfl(uint32_t(base), leader), hi = __shfl(uint32_t(base >> 32), leader);
-  slot_lo = lo + uint32_t(__popcll(ml & below));
-  slot_hi = hi + uint32_t(__popcll(mh & below));
-}
-
-// wave_alloc_pair for a whole work-group of kWaves waves: one atomic per
-// work-group.  All waves of the GPU append to the same counter, and
-// device-scope atomics on one address are serialised: at one atomic per wave
-// the shade kernel spent most of its time queued on it.  Every thread of the
-// work-group must call this (it synchronises the work-group).
+// Queue slots for a work-group of kWaves waves, in two queues counted side by
+// side in the halves of one 64-bit counter: every thread with pred_lo gets a
+// slot of the low queue, every thread with pred_hi one of the high queue (the
+// others get unspecified values), with one atomic per work-group.  A half
+// never carries into the other while it stays below 2^32.  All waves of the
+// GPU append to the same counter, and device-scope atomics on one address are
+// serialised: at one atomic per wave the shade kernel spent most of its time
+// queued on it.  Every thread of the work-group must call this (it
+// synchronises the work-group).
 template <int kWaves>
 __device__ __forceinline__ void block_alloc_pair(unsigned long long *counter, bool pred_lo, uint32_t &slot_lo,
                                                  bool pred_hi, uint32_t &slot_hi) {
@@ -96,22 +68,6 @@ __device__ __forceinline__ void block_sum(unsigned long long *counter, uint32_t 
     for (int w = 0; w < kWaves; ++w) t += s_sum[w];
     if (t) atomicAdd(counter, (unsigned long long)t);
   }
-}
-
-// Adds the sum over the wave's lanes of `n` to a 64-bit statistic, one atomic
-// per wave.  Every lane of the wave must call it.
-__device__ __forceinline__ void wave_sum(unsigned long long *counter, uint32_t n) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
-  if (lane_id() == 0 && n) atomicAdd(counter, (unsigned long long)n);
-}
-
-// Adds the wave's count of `pred` lanes to a 64-bit statistic, one atomic per wave.
-__device__ __forceinline__ void wave_count(unsigned long long *counter, bool pred) {
-  const uint64_t mask = __ballot(pred);
-  if (mask == 0) return;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  if (int(lane_id()) == leader) atomicAdd(counter, (unsigned long long)__popcll(mask));
 }
 
 constexpr uint32_t kChunk = 256;  // chunk of the seam-A traversal kernels and the resolve pass
