@@ -94,6 +94,26 @@ def test_batch_split_into_runs(batch_setup, monkeypatch):
     _check_equal(pt.render(), plain[1], "plain split")
 
 
+def test_batch_of_many_small_views(batch_setup):
+    # 40 views of three scenes: item indices well past the first few travel in
+    # the queue entries, and each view still equals its own render
+    from photon_amd.backend import BatchRenderer
+    views, plain, make = batch_setup
+    scenes_ = [views[0][0], views[1][0], views[2][0]]
+    many = []
+    for k in range(40):
+        pt, cam, _ = views[k % 3]
+        p = abi.default_render_params(8 + (k % 5) * 3, 8 + (k % 4) * 2, 1 + k % 3, 6)
+        p.sample_offset = k
+        many.append((pt, cam, p))
+    br = BatchRenderer(make())
+    br.render(many)
+    for k in (0, 1, 2, 17, 38, 39):
+        pt, cam, p = many[k]
+        pt.camera, pt.params = cam, p
+        _check_equal(br.read(k), pt.render(), f"view {k} of 40")
+
+
 def test_launch_shape_does_not_change_results(batch_setup):
     views, plain, make = batch_setup
     pt, cam, p = views[1]
