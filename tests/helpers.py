@@ -1,4 +1,6 @@
 """Shared test scenes and metrics."""
+import os
+
 import numpy as np
 
 from photon_amd import abi, scenes
@@ -68,3 +70,36 @@ def random_material(rng, delta=False):
     m.ior = float(rng.uniform(1.2, 2.0))
     m.flags = abi.PHX_MATERIAL_FLAG_REFRACTIVE
     return m
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden(name):
+    """A known-answer fixture of tests/golden/ (written by make_golden.py)."""
+    import json
+    with open(os.path.join(GOLDEN, name)) as fh:
+        return json.load(fh)
+
+
+def golden_bsdf_inputs():
+    """Materials and float32 directions (normal +z) of bsdf_closed_forms.json,
+    with the expected f*cos and pdf."""
+    from photon_amd import abi
+    rows = golden("bsdf_closed_forms.json")
+    mats = []
+    for r in rows:
+        m = abi.default_material()
+        m.base_color[:] = tuple(r["base"])
+        m.roughness = r["roughness"]
+        if r["kind"] == "lambert":
+            m.specular = 0.0
+        else:
+            m.metallic = 1.0
+        mats.append(m)
+    wo = np.array([r["wo"] for r in rows], np.float32)
+    wi = np.array([r["wi"] for r in rows], np.float32)
+    ng = np.tile(np.array([0.0, 0.0, 1.0], np.float32), (len(rows), 1))
+    f = np.array([r["f"] for r in rows], np.float64)
+    pdf = np.array([r["pdf"] for r in rows], np.float64)
+    return mats, wo, wi, ng, f, pdf

@@ -206,6 +206,15 @@ def _rel_ok(x, ref, rtol=1e-5, floor=1e-8):
     return np.abs(x - ref) <= rtol * np.abs(ref) + floor
 
 
+def test_disney_eval_pdf_golden_closed_forms(gpu_ctx):
+    # the device BSDF at the closed forms of tests/golden/bsdf_closed_forms.json
+    from tests.helpers import golden_bsdf_inputs
+    mats, wo, wi, ng, f_ref, pdf_ref = golden_bsdf_inputs()
+    f, pdf = _bsdf_gpu(gpu_ctx, mats, wo, wi, ng, ng)
+    assert np.all(_rel_ok(f, f_ref)), np.max(np.abs(f - f_ref) / np.abs(f_ref))
+    assert np.all(_rel_ok(pdf, pdf_ref)), np.max(np.abs(pdf - pdf_ref) / np.abs(pdf_ref))
+
+
 def test_disney_eval_pdf_parity(gpu_ctx):
     rng = np.random.default_rng(31)
     n = 1 << 16

@@ -12,7 +12,7 @@ import pytest
 
 import oracle
 from photon_amd import abi, scenes
-from tests.helpers import random_dirs, single_triangle, triangle_soup, two_quads
+from tests.helpers import golden, golden_bsdf_inputs, random_dirs, single_triangle, triangle_soup, two_quads
 
 M32 = 0xFFFFFFFF
 
@@ -48,6 +48,15 @@ def test_rng_known_answers():
         u, f = oracle.rng_stream(p, s, d, 16)
         assert list(u) == _xorshift(seed, 16)
         np.testing.assert_array_equal(f, np.array([(v >> 8) / 16777216.0 for v in u], np.float32))
+
+
+def test_rng_matches_golden_streams():
+    # tests/golden/rng_streams.json: seeds and streams from an independent
+    # big-integer restatement (make_golden.py)
+    for r in golden("rng_streams.json"):
+        u, f = oracle.rng_stream(r["pixel"], r["sample"], r["dim"], 16)
+        assert list(u) == r["u32"], r
+        np.testing.assert_array_equal(f, np.array(r["f32"], np.float32))
 
 
 def test_xorshift_zero_seed_maps_to_one():
@@ -185,6 +194,15 @@ def test_bsdf_lambert_special_case():
     nol = wi[:, 2].astype(np.float64)
     np.testing.assert_allclose(f, base * nol[:, None] / math.pi, rtol=2e-6, atol=1e-9)
     np.testing.assert_allclose(pdf, nol / math.pi, rtol=2e-6, atol=1e-9)
+
+
+def test_bsdf_matches_golden_closed_forms():
+    # tests/golden/bsdf_closed_forms.json: Lambert and F = 1 metal GGX in double
+    mats, wo, wi, ng, f_ref, pdf_ref = golden_bsdf_inputs()
+    f, pdf = oracle.bsdf_eval(mats, wo, wi, ng, ng, use_double=True)
+    # the fixture's directions are rounded to float32 here: a few 1e-7 of slack
+    np.testing.assert_allclose(f, f_ref, rtol=2e-6)
+    np.testing.assert_allclose(pdf, pdf_ref, rtol=2e-6)
 
 
 def test_bsdf_metal_ggx_special_case():
