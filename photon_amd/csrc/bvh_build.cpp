@@ -183,9 +183,8 @@ struct Builder {
 
 // Quantises the children's boxes into node n's frame: per axis the origin is
 // the node box's low corner and the step a power of two 2^e with
-// org + 65535 * 2^e >= the node's high bound.  Each bound is rounded outward
-// until org + float(q) * 2^e, evaluated in float as the traversal does,
-// contains the child's bound.
+// org + 255 * 2^e >= the node's high bound.  Each bound is rounded outward
+// until the exact value org + q * 2^e contains the child's bound.
 void quantize(BvhNode &n, const Box *kids, int nk) {
   Box nb;
   for (int k = 0; k < nk; ++k) nb.grow(kids[k]);
@@ -193,13 +192,13 @@ void quantize(BvhNode &n, const Box *kids, int nk) {
     const float org = nb.lo[a];
     const float ext = nb.hi[a] - org;
     int e = -100;
-    if (ext > 0.f) e = std::max(-100, int(std::ceil(std::log2(double(ext) / 65535.0))));
+    if (ext > 0.f) e = std::max(-100, int(std::ceil(std::log2(double(ext) / 255.0))));
     // The exact value of a quantised plane, org + q * 2^e (exact in double).
     // Planes are rounded outward against it, which makes the float decode
     // (its rounding) and the traversal's fused slab test (which evaluates it
     // unrounded) conservative as well.
     auto dec = [&](uint32_t q, int ee) { return double(org) + double(q) * std::ldexp(1.0, ee); };
-    while (dec(65535u, e) < double(nb.hi[a])) ++e;
+    while (dec(255u, e) < double(nb.hi[a])) ++e;
     n.org[a] = org;
     n.ex[a] = int8_t(e);
     const float s = std::ldexp(1.f, e);
@@ -211,13 +210,13 @@ void quantize(BvhNode &n, const Box *kids, int nk) {
       }
       double ql = std::floor(double(kids[k].lo[a] - org) / double(s));
       double qh = std::ceil(double(kids[k].hi[a] - org) / double(s));
-      uint32_t lo = uint32_t(std::min(std::max(ql, 0.0), 65535.0));
-      uint32_t hi = uint32_t(std::min(std::max(qh, 0.0), 65535.0));
+      uint32_t lo = uint32_t(std::min(std::max(ql, 0.0), 255.0));
+      uint32_t hi = uint32_t(std::min(std::max(qh, 0.0), 255.0));
       while (lo > 0 && dec(lo, e) > double(kids[k].lo[a])) --lo;
-      while (hi < 65535u && dec(hi, e) < double(kids[k].hi[a])) ++hi;
+      while (hi < 255u && dec(hi, e) < double(kids[k].hi[a])) ++hi;
       if (hi < lo) hi = lo;
-      n.qlo[a][k] = uint16_t(lo);
-      n.qhi[a][k] = uint16_t(hi);
+      n.qlo[a][k] = uint8_t(lo);
+      n.qhi[a][k] = uint8_t(hi);
     }
   }
 }

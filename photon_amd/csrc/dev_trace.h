@@ -95,11 +95,11 @@ PHX_DEV void cswap(float &ka, int &va, float &kb, int &vb) {
 template <bool SORT>
 PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float tmin, float tmax,
                        int codes[kNodeWidth], float key[kNodeWidth]) {
-  // the node as eight 16-byte loads of one line, unpacked in registers
+  // the node as five 16-byte loads, unpacked in registers
   const uint4 *src = reinterpret_cast<const uint4 *>(nodes + node);
-  uint4 w[8];
+  uint4 w[5];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) w[i] = src[i];
+  for (int i = 0; i < 5; ++i) w[i] = src[i];
   const vec3 org{__uint_as_float(w[0].x), __uint_as_float(w[0].y), __uint_as_float(w[0].z)};
   const int ex = int(int8_t(w[0].w & 0xffu)), ey = int(int8_t((w[0].w >> 8) & 0xffu)),
             ez = int(int8_t((w[0].w >> 16) & 0xffu));
@@ -108,12 +108,16 @@ PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float t
   const uint32_t leaf_cnt = w[1].z & 0xffffu, nchild = (w[1].z >> 16) & 0xffu;
   const float Ax = ldexpf(inv.x, ex), Ay = ldexpf(inv.y, ey), Az = ldexpf(inv.z, ez);
   const float Bx = (org.x - o.x) * inv.x, By = (org.y - o.y) * inv.y, Bz = (org.z - o.z) * inv.z;
-  const uint4 nxw = inv.x >= 0.f ? w[2] : w[5], fxw = inv.x >= 0.f ? w[5] : w[2];
-  const uint4 nyw = inv.y >= 0.f ? w[3] : w[6], fyw = inv.y >= 0.f ? w[6] : w[3];
-  const uint4 nzw = inv.z >= 0.f ? w[4] : w[7], fzw = inv.z >= 0.f ? w[7] : w[4];
-  auto q16 = [](const uint4 &v, int k) {
-    const uint32_t d = k < 2 ? v.x : (k < 4 ? v.y : (k < 6 ? v.z : v.w));
-    return float((k & 1) ? (d >> 16) : (d & 0xffffu));
+  // 8 bytes per axis and bound: lo x = w2.xy, lo y = w2.zw, lo z = w3.xy,
+  // hi x = w3.zw, hi y = w4.xy, hi z = w4.zw
+  const uint2 lx{w[2].x, w[2].y}, ly{w[2].z, w[2].w}, lz{w[3].x, w[3].y};
+  const uint2 hx{w[3].z, w[3].w}, hy{w[4].x, w[4].y}, hz{w[4].z, w[4].w};
+  const uint2 nxw = inv.x >= 0.f ? lx : hx, fxw = inv.x >= 0.f ? hx : lx;
+  const uint2 nyw = inv.y >= 0.f ? ly : hy, fyw = inv.y >= 0.f ? hy : ly;
+  const uint2 nzw = inv.z >= 0.f ? lz : hz, fzw = inv.z >= 0.f ? hz : lz;
+  auto q16 = [](const uint2 &v, int k) {
+    const uint32_t d = k < 4 ? v.x : v.y;
+    return float((d >> (8 * (k & 3))) & 0xffu);
   };
   uint32_t leaf_first = tri_base;
   int cnt = 0;

@@ -124,10 +124,12 @@ PHX_DEV uint32_t ray_material(const P &v, uint32_t item, uint32_t prim) {
 
 // Closest hit over the ray queue of the current iteration (persistent waves,
 // per-lane refill).  With groups > 0 the hit record's top 4 bits carry the
-// hit's material group.
+// hit's material group.  Six waves per SIMD: the kernel fits 80 registers
+// without spills.
 template <class P, int B>
-__global__ __launch_bounds__(B) void k_extend(P v, StreamQueues q, uint32_t chunk, uint32_t refill,
-                                              uint32_t groups) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(P v, StreamQueues q,
+                                                                                      uint32_t chunk, uint32_t refill,
+                                                                                      uint32_t groups) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[ray_count_word(it)];
   if (count == 0) return;
@@ -253,9 +255,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // Any hit over the shadow queue (trace_occluded; persistent waves, per-lane
-// refill).
+// refill; six waves per SIMD as k_extend).
 template <class P, int B>
-__global__ __launch_bounds__(B) void k_shadow(P v, StreamQueues q, uint32_t chunk, uint32_t refill) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(P v, StreamQueues q,
+                                                                                      uint32_t chunk, uint32_t refill) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[shadow_count_word(it + 1)];
   if (count == 0) return;
