@@ -107,7 +107,8 @@ template <class P, int B>
 __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, AdjointArgs a) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[ray_count_word(it)];
-  if (count == 0) return;
+  const uint32_t units = fetch_units(count, B);
+  if (blockIdx.x >= units) return;  // work-groups beyond the fetchers take no part (count == 0 included)
   __shared__ int lds[kShortStack * B];
   __shared__ double glds[kAdjLdsMaterials * kGradParams];
   for (uint32_t k = threadIdx.x; k < a.lds_mats * kGradParams; k += B) glds[k] = 0.0;
@@ -117,11 +118,10 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
   LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   // the replay runs no shadow kernel: its fetch set serves this loop
   uint32_t *set = &q.counters[fetch_set(kFetchShadow, it)];
-  const uint32_t slen = stripe_len(count);
-  uint32_t stripe = blockIdx.x % kStripes;  // thread 0's cursor
+  uint32_t stripe = blockIdx.x % kStripes, left = fetch_walk(min(units, gridDim.x));  // thread 0's cursor
   __shared__ uint32_t s_tile[2];
   while (true) {
-    if (threadIdx.x == 0) fetch_tile(set, count, slen, B, stripe, s_tile[0], s_tile[1]);
+    if (threadIdx.x == 0) fetch_tile(set, count, units, B, stripe, left, s_tile[0], s_tile[1]);
     __syncthreads();
     const uint32_t base = s_tile[0], end = s_tile[1];
     if (base == 0xffffffffu) break;  // uniform over the work-group

@@ -73,55 +73,85 @@ __device__ __forceinline__ void block_sum(unsigned long long *counter, uint32_t 
 constexpr uint32_t kChunk = 256;  // chunk of the seam-A traversal kernels and the resolve pass
 
 // Striped work fetch of the persistent traversal kernels.  The queue is cut
-// into kStripes stripes of equal length (a multiple of 64); each stripe has
+// into chunks of `chunk` items, dealt round-robin to kStripes stripes (chunk
+// k belongs to stripe k mod kStripes), so every stripe holds an even sample
+// of the queue: the first iteration's queue is in pixel order, and stripes of
+// contiguous ranges differed in cost by whole image bands.  Each stripe has
 // its own chunk counter in a 128-byte line of its own (a fetch set of
-// kFetchSetWords words), so the waves of the GPU spread their chunk atomics
-// over kStripes addresses instead of queueing on one.  A wave starts in stripe
-// (its index mod kStripes) and moves on to the next stripe when its stripe is
-// used up.
+// kFetchSetWords words), so the fetchers spread their atomics over kStripes
+// addresses instead of queueing on one.  A fetcher (a wave, or a work-group's
+// leader) starts in its home stripe (its index mod kStripes) and moves on to
+// the next stripe when that one is used up.
+//
+// Every move costs a failed atomic, and device-scope atomics on one address
+// are serialised: with all 8192 waves of a persistent grid walking all
+// kStripes stripes, a launch over a short queue spent ~80 us in the walk
+// alone.  Three things keep the walk short:
+//  - only the first fetch_units(count, chunk) fetchers take part, enough to
+//    give each a chunk;
+//  - a fetcher moves past at most fetch_walk(fetchers) stripes in total,
+//    where fetchers = min(units, fetchers in the grid).  Once kStripes or
+//    more fetchers take part, every stripe is some fetcher's home, and a
+//    fetcher leaves its home stripe only when it is used up, so the home
+//    fetchers drain every stripe; the walk only balances the tail.  With
+//    fewer fetchers the walk covers every stripe;
+//  - the waves of a work-group share the stripes they found used up in LDS.
 constexpr uint32_t kStripes = 16;
 constexpr uint32_t kStripeStride = 32;  // words between the counters of a set
 constexpr uint32_t kFetchSetWords = kStripes * kStripeStride;
-__device__ __forceinline__ uint32_t stripe_len(uint32_t count) {
-  return ((count + kStripes - 1u) / kStripes + 63u) & ~63u;
+constexpr uint32_t kFetchWalk = 4;  // stripes a fetcher may try when every stripe has a home fetcher
+__device__ __forceinline__ uint32_t fetch_units(uint32_t count, uint32_t chunk) { return (count + chunk - 1u) / chunk; }
+__device__ __forceinline__ uint32_t fetch_walk(uint32_t fetchers) {
+  return fetchers >= kStripes ? kFetchWalk : kStripes;
 }
-// The wave's next chunk [lo, hi) of at most `chunk` items, from its current
-// stripe or the ones after it; lo = 0xffffffff when every stripe is used up.
-// Wave-uniform: every lane of the wave must call it.
-__device__ __forceinline__ void fetch_chunk(uint32_t *set, uint32_t count, uint32_t slen, uint32_t chunk,
-                                            uint32_t &stripe, uint32_t &lo, uint32_t &hi) {
+// Chunks of the queue's `units` that belong to `stripe`.
+__device__ __forceinline__ uint32_t stripe_units(uint32_t units, uint32_t stripe) {
+  return units > stripe ? (units - stripe + kStripes - 1u) / kStripes : 0u;
+}
+// The wave's next chunk [lo, hi) of the queue's `count` items (`units`
+// chunks), from its current stripe or the ones after it, trying at most
+// `left` more stripes (decremented for each one found used up); lo =
+// 0xffffffff when none is left.  `used` is the work-group's LDS mask of
+// stripes known to be used up (zero at the start of the fetch): a wave skips
+// those without an atomic and marks the ones it finds used up, so the waves of
+// a work-group pay one failed atomic per stripe between them.  Wave-uniform:
+// every lane of the wave must call it.
+__device__ __forceinline__ void fetch_chunk(uint32_t *set, uint32_t *used, uint32_t count, uint32_t units,
+                                            uint32_t chunk, uint32_t &stripe, uint32_t &left, uint32_t &lo,
+                                            uint32_t &hi) {
   lo = 0xffffffffu;
   hi = 0u;
-  for (uint32_t t = 0; t < kStripes; ++t) {
-    const uint32_t s_lo = stripe * slen;
-    const uint32_t s_hi = min(s_lo + slen, count);
-    if (s_lo < s_hi) {
-      uint32_t b = 0;
-      if (lane_id() == 0) b = atomicAdd(&set[stripe * kStripeStride], chunk);
-      b = __shfl(b, 0);
-      if (b < s_hi - s_lo) {
-        lo = s_lo + b;
-        hi = min(lo + chunk, s_hi);
-        return;
+  for (; left; --left) {
+    const uint32_t bit = 1u << stripe;
+    if (!(__hip_atomic_load(used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & bit)) {
+      if (stripe_units(units, stripe)) {
+        uint32_t k = 0;
+        if (lane_id() == 0) k = atomicAdd(&set[stripe * kStripeStride], 1u);
+        k = __shfl(k, 0);
+        if (k < stripe_units(units, stripe)) {
+          lo = (k * kStripes + stripe) * chunk;
+          hi = min(lo + chunk, count);
+          return;
+        }
       }
+      if (lane_id() == 0) __hip_atomic_fetch_or(used, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     stripe = stripe + 1u == kStripes ? 0u : stripe + 1u;
   }
 }
-// fetch_chunk for one thread (a work-group's leader fetching the group's tile
-// of n items).
-__device__ __forceinline__ void fetch_tile(uint32_t *set, uint32_t count, uint32_t slen, uint32_t n, uint32_t &stripe,
-                                           uint32_t &lo, uint32_t &hi) {
+// fetch_chunk for one thread (a work-group's leader fetching the group's
+// tiles of n items; no LDS mask, as there is one fetcher per work-group).
+__device__ __forceinline__ void fetch_tile(uint32_t *set, uint32_t count, uint32_t units, uint32_t n,
+                                           uint32_t &stripe, uint32_t &left, uint32_t &lo, uint32_t &hi) {
   lo = 0xffffffffu;
   hi = 0u;
-  for (uint32_t t = 0; t < kStripes; ++t) {
-    const uint32_t s_lo = stripe * slen;
-    const uint32_t s_hi = min(s_lo + slen, count);
-    if (s_lo < s_hi) {
-      const uint32_t b = atomicAdd(&set[stripe * kStripeStride], n);
-      if (b < s_hi - s_lo) {
-        lo = s_lo + b;
-        hi = min(lo + n, s_hi);
+  for (; left; --left) {
+    const uint32_t su = stripe_units(units, stripe);
+    if (su) {
+      const uint32_t k = atomicAdd(&set[stripe * kStripeStride], 1u);
+      if (k < su) {
+        lo = (k * kStripes + stripe) * n;
+        hi = min(lo + n, count);
         return;
       }
     }

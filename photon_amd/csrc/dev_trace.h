@@ -266,15 +266,21 @@ PHX_DEV bool traverse(const DevScene &s, vec3 o, vec3 d, float tmin, float tmax,
 constexpr uint32_t kNoRay = 0xffffffffu;
 
 // load(i, o, d, tmin, tmax, nodes, tris) reads ray i and the acceleration
-// structure it is traced against; done(i, hit, h) consumes its result.
+// structure it is traced against; done(i, hit, h) consumes its result.  Every
+// thread of the work-group must call it (it synchronises the work-group once,
+// at the start).
 template <bool ANY, class Load, class Done>
 PHX_DEV void traverse_stream(LaneStack &st, uint32_t *set, uint32_t count, uint32_t chunk, uint32_t refill_min,
                              Load load, Done done) {
   const uint32_t lane = __lane_id();
   uint32_t next = 0, end = 0;  // the wave's current chunk [next, end)
-  const uint32_t slen = stripe_len(count);
-  uint32_t stripe = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kStripes;
-  bool exhausted = false;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t units = fetch_units(count, chunk);
+  uint32_t stripe = wave % kStripes, left = fetch_walk(min(units, gridDim.x * blockDim.x / 64u));
+  bool exhausted = wave >= units;  // waves beyond the fetchers take no part
+  __shared__ uint32_t s_used;      // stripes the work-group's waves found used up
+  if (threadIdx.x == 0) s_used = 0u;
+  __syncthreads();
   uint32_t idx = kNoRay;
   vec3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 1.f}, inv{0.f, 0.f, 0.f};
   float tmin = 0.f, best = 0.f;
@@ -291,7 +297,7 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *set, uint32_t count, uint3
     if (!exhausted && (nidle >= refill_min || nidle == 64u)) {
       const uint32_t avail = end - next;
       uint32_t nb = kNoRay, nbe = 0;  // a new chunk [nb, nbe)
-      if (nidle > avail) fetch_chunk(set, count, slen, chunk, stripe, nb, nbe);
+      if (nidle > avail) fetch_chunk(set, &s_used, count, units, chunk, stripe, left, nb, nbe);
       if (idx == kNoRay) {
         const uint32_t rank = uint32_t(__popcll(idle & ((uint64_t(1) << lane) - 1)));
         uint32_t i = kNoRay;
