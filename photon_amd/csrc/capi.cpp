@@ -103,7 +103,13 @@ int phx_create(phx_context **out, int device) {
   if (hipSetDevice(device) != hipSuccess) return PHX_ERR_HIP;
   auto *c = new phx_context();
   c->device = device;
-  if (hipStreamCreate(&c->stream) != hipSuccess) {
+  if (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->stream2) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PHX_ERR_HIP;
   }
@@ -120,13 +126,17 @@ void phx_destroy(phx_context *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->stream2);
   // DBuf members free their memory in the destructor.
-  hipStream_t s = c->stream;
+  hipStream_t s = c->stream, s2 = c->stream2;
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(c->ev_fork);
+  (void)hipEventDestroy(c->ev_join);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   if (c->pinned) (void)hipHostFree(c->pinned);
   delete c;
   if (s) (void)hipStreamDestroy(s);
+  if (s2) (void)hipStreamDestroy(s2);
 }
 
 const char *phx_last_error(const phx_context *c) { return c ? c->error.c_str() : "no context"; }

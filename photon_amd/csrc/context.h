@@ -49,6 +49,11 @@ struct phx_context {
   using DBuf = phx_detail::DBuf;
   int device = 0;
   hipStream_t stream = nullptr;
+  // The bounce loop's second stream: an iteration's any-hit pass runs there
+  // beside the next iteration's closest-hit pass, forked from and joined to
+  // `stream` by the two events.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int cu_count = 256;
   std::string error;
   phx_tuning tuning{};

@@ -255,12 +255,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // Any hit over the shadow queue (trace_occluded; persistent waves, per-lane
-// refill; six waves per SIMD as k_extend).
+// refill; six waves per SIMD as k_extend).  It runs in the iteration after the
+// shade kernel that filled the queue, beside that iteration's k_extend: the
+// queue is the one counted for the current iteration.
 template <class P, int B>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(P v, StreamQueues q,
                                                                                       uint32_t chunk, uint32_t refill) {
   const uint32_t it = q.counters[kIter];
-  const uint32_t count = q.counters[shadow_count_word(it + 1)];
+  const uint32_t count = q.counters[shadow_count_word(it)];
   if (count == 0) return;
   __shared__ int lds[kShortStack * B];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,11 +310,11 @@ __device__ __forceinline__ void nee_accumulate_one(const P &v, const StreamQueue
 // pt_nee_accum_pack (pathtracer.cpp:1095-1121): the unoccluded shadow rays'
 // contributions, times the volume transmittance along them, into their
 // samples' radiance.  One thread per shadow ray, over the queue in strides of
-// the grid.
+// the grid; after k_shadow, on the same queue.
 template <class P>
 __global__ __launch_bounds__(256) void k_nee_accumulate(P v, StreamQueues q) {
   const uint32_t it = q.counters[kIter];
-  const uint32_t count = q.counters[shadow_count_word(it + 1)];
+  const uint32_t count = q.counters[shadow_count_word(it)];
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) nee_accumulate_one(v, q, i);
 }
 // Sums an item's samples of each pixel into the frame, in sample order.

@@ -158,13 +158,22 @@ LoopShape loop_shape(const phx_context *c, uint32_t capacity) {
   return ls;
 }
 
+// Per-kernel timing of a run on one stream: event pairs around the closest-hit,
+// shading and any-hit launches.
+enum TimedKind { kTimedExtend = 0, kTimedShade = 1, kTimedShadow = 2 };
 struct Timer {
   phx_context *c;
   hipStream_t st;
   bool on;
-  size_t n = 0;
-  hipError_t mark() {
+  size_t n = 0;            // events recorded
+  std::vector<int> kinds;  // the kind of event pair k (events 2k and 2k + 1)
+  hipError_t begin(int kind) {
     if (!on) return hipSuccess;
+    kinds.push_back(kind);
+    return record();
+  }
+  hipError_t end() { return on ? record() : hipSuccess; }
+  hipError_t record() {
     if (n == c->events.size()) {
       hipEvent_t e;
       hipError_t err = hipEventCreate(&e);
@@ -175,22 +184,56 @@ struct Timer {
   }
 };
 
+// The second stream of the bounce loop and its fork and join events.
+struct Fork {
+  hipStream_t st2;
+  hipEvent_t ev_fork, ev_join;
+};
+
+// Iterations of the bounce loop.  An iteration runs the any-hit pass over the
+// shadow queue that the previous iteration's shading filled (k_shadow, then
+// k_nee_accumulate) beside its own closest-hit pass, then shading and
+// regeneration.  With `fork` the any-hit pass runs on the second stream,
+// forked after the previous regeneration and joined before shading: the two
+// traversal passes share the GPU, so neither's tail of last slow rays leaves
+// it idle.  Shading must wait for the join, since it refills the shadow queue
+// and adds to the radiance of the samples k_nee_accumulate adds to, after it
+// as in the reference's order.  Without `fork` (timed runs, the serial
+// diagnostic) the same launches run on `st` alone.  The shadow queue of the
+// last iteration is left for trace_pending_shadow.
 hipError_t launch_iterations(const ViewArgs &va, const StreamQueues &q, const LoopShape &ls, uint32_t n, Timer *tm,
-                             hipStream_t st) {
+                             hipStream_t st, const Fork *fork) {
   for (uint32_t j = 0; j < n; ++j) {
     hipError_t e;
-    if (tm && (e = tm->mark()) != hipSuccess) return e;
+    if (fork) {
+      if ((e = hipEventRecord(fork->ev_fork, st)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(fork->st2, fork->ev_fork, 0)) != hipSuccess) return e;
+      if ((e = launch_shadow(va, q, ls, fork->st2)) != hipSuccess) return e;
+      if ((e = hipEventRecord(fork->ev_join, fork->st2)) != hipSuccess) return e;
+    } else {
+      if (tm && (e = tm->begin(kTimedShadow)) != hipSuccess) return e;
+      if ((e = launch_shadow(va, q, ls, st)) != hipSuccess) return e;
+      if (tm && (e = tm->end()) != hipSuccess) return e;
+    }
+    if (tm && (e = tm->begin(kTimedExtend)) != hipSuccess) return e;
     if ((e = launch_extend(va, q, ls, st)) != hipSuccess) return e;
-    if (tm && (e = tm->mark()) != hipSuccess) return e;
-    if (tm && (e = tm->mark()) != hipSuccess) return e;
+    if (tm && (e = tm->end()) != hipSuccess) return e;
+    if (fork && (e = hipStreamWaitEvent(st, fork->ev_join, 0)) != hipSuccess) return e;
+    if (tm && (e = tm->begin(kTimedShade)) != hipSuccess) return e;
     if ((e = launch_shade(va, q, ls, st)) != hipSuccess) return e;
-    if (tm && (e = tm->mark()) != hipSuccess) return e;
-    if (tm && (e = tm->mark()) != hipSuccess) return e;
-    if ((e = launch_shadow(va, q, ls, st)) != hipSuccess) return e;
-    if (tm && (e = tm->mark()) != hipSuccess) return e;
+    if (tm && (e = tm->end()) != hipSuccess) return e;
     if ((e = launch_regen(va, q, 1, ls, st)) != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+// The any-hit pass over the shadow queue of the last iteration that ran.
+hipError_t trace_pending_shadow(const ViewArgs &va, const StreamQueues &q, const LoopShape &ls, Timer *tm,
+                                hipStream_t st) {
+  hipError_t e;
+  if (tm && (e = tm->begin(kTimedShadow)) != hipSuccess) return e;
+  if ((e = launch_shadow(va, q, ls, st)) != hipSuccess) return e;
+  return tm ? tm->end() : hipSuccess;
 }
 
 // Iterations of the adjoint replay: closest hit, replayed shading with inline
@@ -208,18 +251,19 @@ hipError_t launch_replay_iterations(const ViewArgs &va, const StreamQueues &q, c
 
 // Captures kIterBatch iterations of the bounce loop as a graph, unless the
 // graph already held was captured with the same launch arguments.
-int ensure_graph(phx_context *c, const ViewArgs &va, const StreamQueues &q, const LoopShape &ls) {
-  std::vector<unsigned char> key(sizeof va + sizeof q + sizeof ls);
+int ensure_graph(phx_context *c, const ViewArgs &va, const StreamQueues &q, const LoopShape &ls, const Fork *fork) {
+  std::vector<unsigned char> key(sizeof va + sizeof q + sizeof ls + 1);
   std::memcpy(key.data(), &va, sizeof va);
   std::memcpy(key.data() + sizeof va, &q, sizeof q);
   std::memcpy(key.data() + sizeof va + sizeof q, &ls, sizeof ls);
+  key.back() = fork ? 1 : 0;
   if (c->graph && key == c->graph_key) return PHX_OK;
   if (c->graph) {
     (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
   }
   HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const hipError_t le = launch_iterations(va, q, ls, kIterBatch, nullptr, c->stream);
+  const hipError_t le = launch_iterations(va, q, ls, kIterBatch, nullptr, c->stream, fork);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(c->stream, &g);
   if (le != hipSuccess || ce != hipSuccess) {
@@ -313,6 +357,11 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   const bool trace_counts = tc && std::atoi(tc) > 0;
   const uint32_t batch = trace_counts ? uint32_t(std::atoi(tc)) : kIterBatch;
   Timer tm{c, st, c->tuning.time_kernels != 0};
+  // PHX_SERIAL_BOUNCE=1 (diagnostic): every launch of the bounce loop on one
+  // stream, the any-hit pass not beside the closest-hit pass.
+  const char *sb = std::getenv("PHX_SERIAL_BOUNCE");
+  const Fork fork_spec{c->stream2, c->ev_fork, c->ev_join};
+  const Fork *fork = (tm.on || (sb && std::atoi(sb) > 0)) ? nullptr : &fork_spec;
 
   HIP_TRY(c, launch_clear_u32(reinterpret_cast<uint32_t *>(q.stats), kStatWords * 2, st));
   HIP_TRY(c, launch_aov_init(total_pixels, aov, st));
@@ -350,7 +399,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
       }
     }
     const bool graph = use_graph && !adj && va.kind != 0 && !trace_counts && !tm.on;
-    if (graph && (r = ensure_graph(c, va, q, ls))) return r;
+    if (graph && (r = ensure_graph(c, va, q, ls, fork))) return r;
     for (int pass = 0; pass < (adj ? 2 : 1); ++pass) {
       const bool replay = pass == 1;
       HIP_TRY(c, launch_run_start(q, run.total, std::min(cap, run.total), st));
@@ -364,7 +413,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
           HIP_TRY(c, hipGraphLaunch(c->graph, st));
           it += kIterBatch;
         } else {
-          HIP_TRY(c, launch_iterations(va, q, ls, batch, tm.on ? &tm : nullptr, st));
+          HIP_TRY(c, launch_iterations(va, q, ls, batch, tm.on ? &tm : nullptr, st, fork));
           it += batch;
         }
         // the iteration counter, the samples started, then that iteration's queue
@@ -381,6 +430,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
         if (it >= kIterationGuard)
           return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");
       }
+      if (!replay) HIP_TRY(c, trace_pending_shadow(va, q, ls, tm.on ? &tm : nullptr, st));
     }
     if (adj) continue;
     for (const HostItem &h : run.items)
@@ -406,17 +456,19 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   c->stats.pixels = total_pixels;
   c->stats.extend_launches = c->stats.shade_launches = c->stats.shadow_launches = 0;
   c->stats.extend_ms = c->stats.shade_ms = c->stats.shadow_ms = 0.0;
-  for (size_t e = 0; e + 5 < tm.n; e += 6) {
-    float a = 0.f, b = 0.f, d = 0.f;
-    HIP_TRY(c, hipEventElapsedTime(&a, c->events[e], c->events[e + 1]));
-    HIP_TRY(c, hipEventElapsedTime(&b, c->events[e + 2], c->events[e + 3]));
-    HIP_TRY(c, hipEventElapsedTime(&d, c->events[e + 4], c->events[e + 5]));
-    c->stats.extend_ms += a;
-    c->stats.shade_ms += b;
-    c->stats.shadow_ms += d;
-    c->stats.extend_launches++;
-    c->stats.shade_launches++;
-    c->stats.shadow_launches++;
+  for (size_t k = 0; k < tm.kinds.size(); ++k) {
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->events[2 * k], c->events[2 * k + 1]));
+    if (tm.kinds[k] == kTimedExtend) {
+      c->stats.extend_ms += ms;
+      c->stats.extend_launches++;
+    } else if (tm.kinds[k] == kTimedShade) {
+      c->stats.shade_ms += ms;
+      c->stats.shade_launches++;
+    } else {
+      c->stats.shadow_ms += ms;
+      c->stats.shadow_launches++;
+    }
   }
   return PHX_OK;
 }

@@ -77,16 +77,18 @@ def test_trace_closest_cornell_vs_brute_force(gpu_ctx):
 
 
 def test_trace_batch_sizes_across_stripes(gpu_ctx):
-    # The persistent traversal cuts the ray batch into 16 stripes of a
-    # multiple of 64 rays and clips chunks at stripe ends: batches smaller
-    # than one stripe, ragged sizes and sizes just past a stripe boundary must
-    # each trace every ray once.
+    # The persistent traversal cuts the ray batch into 256-ray chunks dealt
+    # round-robin to 16 stripes; only as many waves as chunks fetch, and they
+    # walk every stripe when fewer than 16 take part, at most 4 otherwise.
+    # One ray, a ragged last chunk, fewer chunks than stripes, exactly 16
+    # chunks and sizes either side of it must each trace every ray once.
     torch = _torch()
     s = scenes.cornell_box()
     b = _backend(gpu_ctx, s)
     org, d, tmin, tmax = scenes.random_rays(s, 70000, seed=29)
     ref = oracle.OracleScene(s, brute=True).trace_closest(org, d, tmin, tmax, brute=True)
-    for n in (1, 63, 64, 65, 1000, 1023, 1025, 4097, 16 * 64 + 1, 65537, 70000):
+    for n in (1, 63, 64, 65, 255, 256, 257, 1000, 15 * 256 + 1, 16 * 256 - 1, 16 * 256, 16 * 256 + 1, 17 * 256,
+              65537, 70000):
         gpu = _gpu_closest(b, org[:n], d[:n], tmin[:n], tmax[:n])
         np.testing.assert_array_equal(gpu[1], ref[1][:n], err_msg=f"closest hits, n = {n}")
         occ = torch.full((n,), 7, dtype=torch.int32, device="cuda")
