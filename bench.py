@@ -357,7 +357,9 @@ def main():
         "mrays_per_s": (closest + shadow) / wall / 1e6,
         "rays": {"closest": closest, "shadow": shadow, "shade_events": shade},
         "kernel_ms_per_step": {"extend": ext_ms / args.steps, "shade": shd_ms / args.steps,
-                               "shadow": sdw_ms / args.steps},
+                               "shadow": sdw_ms / args.steps,
+                               "note": "HIP-event spans on each launch's stream; the shadow pass runs on a second "
+                                       "stream beside the next extend launch, so its span includes waiting for CUs"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_baseline": gpu_base,

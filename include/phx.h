@@ -274,7 +274,9 @@ typedef struct phx_stats {
   uint64_t bvh_bytes;
   double last_render_ms;
   /* With phx_tuning.time_kernels = 1: HIP-event time of every launch of the
-   * closest-hit (extend), shade and shadow kernels of the last render. */
+   * closest-hit (extend), shade and shadow kernels of the last render.  The
+   * shadow pass runs beside the next extend launch, so the three sums can
+   * exceed the render time. */
   uint64_t extend_launches;
   double extend_ms;
   uint64_t shade_launches;

@@ -158,22 +158,21 @@ LoopShape loop_shape(const phx_context *c, uint32_t capacity) {
   return ls;
 }
 
-// Per-kernel timing of a run on one stream: event pairs around the closest-hit,
-// shading and any-hit launches.
+// Per-kernel timing of a run: event pairs around the closest-hit, shading and
+// any-hit launches, each on the stream of its launch.
 enum TimedKind { kTimedExtend = 0, kTimedShade = 1, kTimedShadow = 2 };
 struct Timer {
   phx_context *c;
-  hipStream_t st;
   bool on;
   size_t n = 0;            // events recorded
   std::vector<int> kinds;  // the kind of event pair k (events 2k and 2k + 1)
-  hipError_t begin(int kind) {
+  hipError_t begin(int kind, hipStream_t st) {
     if (!on) return hipSuccess;
     kinds.push_back(kind);
-    return record();
+    return record(st);
   }
-  hipError_t end() { return on ? record() : hipSuccess; }
-  hipError_t record() {
+  hipError_t end(hipStream_t st) { return on ? record(st) : hipSuccess; }
+  hipError_t record(hipStream_t st) {
     if (n == c->events.size()) {
       hipEvent_t e;
       hipError_t err = hipEventCreate(&e);
@@ -198,30 +197,29 @@ struct Fork {
 // traversal passes share the GPU, so neither's tail of last slow rays leaves
 // it idle.  Shading must wait for the join, since it refills the shadow queue
 // and adds to the radiance of the samples k_nee_accumulate adds to, after it
-// as in the reference's order.  Without `fork` (timed runs, the serial
-// diagnostic) the same launches run on `st` alone.  The shadow queue of the
-// last iteration is left for trace_pending_shadow.
+// as in the reference's order.  Without `fork` (the serial diagnostic) the
+// same launches run on `st` alone.  The shadow queue of the last iteration is
+// left for trace_pending_shadow.
 hipError_t launch_iterations(const ViewArgs &va, const StreamQueues &q, const LoopShape &ls, uint32_t n, Timer *tm,
                              hipStream_t st, const Fork *fork) {
   for (uint32_t j = 0; j < n; ++j) {
     hipError_t e;
+    const hipStream_t st_any = fork ? fork->st2 : st;
     if (fork) {
       if ((e = hipEventRecord(fork->ev_fork, st)) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(fork->st2, fork->ev_fork, 0)) != hipSuccess) return e;
-      if ((e = launch_shadow(va, q, ls, fork->st2)) != hipSuccess) return e;
-      if ((e = hipEventRecord(fork->ev_join, fork->st2)) != hipSuccess) return e;
-    } else {
-      if (tm && (e = tm->begin(kTimedShadow)) != hipSuccess) return e;
-      if ((e = launch_shadow(va, q, ls, st)) != hipSuccess) return e;
-      if (tm && (e = tm->end()) != hipSuccess) return e;
     }
-    if (tm && (e = tm->begin(kTimedExtend)) != hipSuccess) return e;
+    if (tm && (e = tm->begin(kTimedShadow, st_any)) != hipSuccess) return e;
+    if ((e = launch_shadow(va, q, ls, st_any)) != hipSuccess) return e;
+    if (tm && (e = tm->end(st_any)) != hipSuccess) return e;
+    if (fork && (e = hipEventRecord(fork->ev_join, fork->st2)) != hipSuccess) return e;
+    if (tm && (e = tm->begin(kTimedExtend, st)) != hipSuccess) return e;
     if ((e = launch_extend(va, q, ls, st)) != hipSuccess) return e;
-    if (tm && (e = tm->end()) != hipSuccess) return e;
+    if (tm && (e = tm->end(st)) != hipSuccess) return e;
     if (fork && (e = hipStreamWaitEvent(st, fork->ev_join, 0)) != hipSuccess) return e;
-    if (tm && (e = tm->begin(kTimedShade)) != hipSuccess) return e;
+    if (tm && (e = tm->begin(kTimedShade, st)) != hipSuccess) return e;
     if ((e = launch_shade(va, q, ls, st)) != hipSuccess) return e;
-    if (tm && (e = tm->end()) != hipSuccess) return e;
+    if (tm && (e = tm->end(st)) != hipSuccess) return e;
     if ((e = launch_regen(va, q, 1, ls, st)) != hipSuccess) return e;
   }
   return hipSuccess;
@@ -231,9 +229,9 @@ hipError_t launch_iterations(const ViewArgs &va, const StreamQueues &q, const Lo
 hipError_t trace_pending_shadow(const ViewArgs &va, const StreamQueues &q, const LoopShape &ls, Timer *tm,
                                 hipStream_t st) {
   hipError_t e;
-  if (tm && (e = tm->begin(kTimedShadow)) != hipSuccess) return e;
+  if (tm && (e = tm->begin(kTimedShadow, st)) != hipSuccess) return e;
   if ((e = launch_shadow(va, q, ls, st)) != hipSuccess) return e;
-  return tm ? tm->end() : hipSuccess;
+  return tm ? tm->end(st) : hipSuccess;
 }
 
 // Iterations of the adjoint replay: closest hit, replayed shading with inline
@@ -356,12 +354,12 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   const char *tc = std::getenv("PHX_TRACE_COUNTS");
   const bool trace_counts = tc && std::atoi(tc) > 0;
   const uint32_t batch = trace_counts ? uint32_t(std::atoi(tc)) : kIterBatch;
-  Timer tm{c, st, c->tuning.time_kernels != 0};
+  Timer tm{c, c->tuning.time_kernels != 0};
   // PHX_SERIAL_BOUNCE=1 (diagnostic): every launch of the bounce loop on one
   // stream, the any-hit pass not beside the closest-hit pass.
   const char *sb = std::getenv("PHX_SERIAL_BOUNCE");
   const Fork fork_spec{c->stream2, c->ev_fork, c->ev_join};
-  const Fork *fork = (tm.on || (sb && std::atoi(sb) > 0)) ? nullptr : &fork_spec;
+  const Fork *fork = sb && std::atoi(sb) > 0 ? nullptr : &fork_spec;
 
   HIP_TRY(c, launch_clear_u32(reinterpret_cast<uint32_t *>(q.stats), kStatWords * 2, st));
   HIP_TRY(c, launch_aov_init(total_pixels, aov, st));
