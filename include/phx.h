@@ -468,6 +468,47 @@ typedef struct phx_render_settings {
 
 int phx_bridge_render(phx_context *ctx, const phx_camera_data *camera, const phx_render_settings *settings,
                       int reset);
+
+/* bridge::MaterialData (RenderBridge.h:27-56).  Each parameter source is
+ * given by its ParamSourceData kind: 0 constant, 1 attribute, 2 sampler. */
+typedef struct phx_bridge_material {
+  float base_color[3];
+  float opacity;
+  float metallic;
+  float roughness; /* perceptual: the GGX alpha is its square */
+  float ior;
+  float transmission;
+  float specular; /* weight of the dielectric specular lobe, 0..1 */
+  float emission[3];
+  float clearcoat;
+  float clearcoat_roughness;
+  float sheen;
+  int32_t base_color_src;
+  int32_t opacity_src;
+  int32_t metallic_src;
+  int32_t roughness_src;
+  int32_t emission_src;
+  int32_t transmission_src;
+  int32_t normal_src;
+  int32_t alpha_mode; /* 0 blend, 1 opaque, 2 mask */
+  float alpha_cutoff;
+  int32_t base_color_alpha;
+  int32_t thin_walled;
+  int32_t emitter_only;
+  int32_t hidden_to_camera;
+  int32_t emission_side; /* 0 front, 1 back, 2 both */
+} phx_bridge_material;
+
+/* convert_material (RenderBridge.cpp:324-370): the pt-level material the
+ * bridge renders a MaterialData with.  Roughness is squared (the BSDF takes
+ * the GGX alpha), specular halved (0.5 is the full dielectric reflectance),
+ * emission_strength is 1, alpha the opacity, the alpha mode recoded to
+ * PHX_ALPHA_*, REFRACTIVE always set and the emitter-only, hidden-to-camera
+ * and emission-side flags added.  Fields the bridge leaves alone keep
+ * pt::Material's defaults (material.h:35-82).  Only constant sources are
+ * supported: a material with an attribute or sampler source gives
+ * PHX_ERR_UNSUPPORTED and leaves *out unchanged. */
+int phx_convert_material(const phx_bridge_material *in, phx_material *out);
 uint32_t phx_bridge_accumulated_samples(const phx_context *ctx);
 int phx_bridge_read_color(phx_context *ctx, float *rgba);
 int phx_bridge_read_depth(phx_context *ctx, float *depth);

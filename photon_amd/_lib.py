@@ -57,6 +57,7 @@ def _sig(lib):
                                              C.c_float]),
         "phx_camera_orthographic": (C.c_int, [C.POINTER(abi.Camera), PF, PF, PF, C.c_float, C.c_float]),
         "phx_volume_majorants": (C.c_int, [C.POINTER(abi.Volume), PF, PF, C.c_uint32, PF, C.c_uint32, PU]),
+        "phx_convert_material": (C.c_int, [C.POINTER(abi.BridgeMaterial), C.POINTER(abi.Material)]),
         "phx_bridge_render": (C.c_int, [P, C.POINTER(abi.CameraData), C.POINTER(abi.RenderSettings), C.c_int]),
         "phx_bridge_accumulated_samples": (C.c_uint32, [P]),
         "phx_bridge_read_color": (C.c_int, [P, PF]),

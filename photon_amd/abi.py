@@ -129,6 +129,37 @@ class Stats(C.Structure):
     ]
 
 
+class BridgeMaterial(C.Structure):
+    """bridge::MaterialData (RenderBridge.h:27-56); sources by ParamSourceData kind."""
+    _fields_ = [
+        ("base_color", F3), ("opacity", C.c_float), ("metallic", C.c_float), ("roughness", C.c_float),
+        ("ior", C.c_float), ("transmission", C.c_float), ("specular", C.c_float), ("emission", F3),
+        ("clearcoat", C.c_float), ("clearcoat_roughness", C.c_float), ("sheen", C.c_float),
+        ("base_color_src", C.c_int32), ("opacity_src", C.c_int32), ("metallic_src", C.c_int32),
+        ("roughness_src", C.c_int32), ("emission_src", C.c_int32), ("transmission_src", C.c_int32),
+        ("normal_src", C.c_int32), ("alpha_mode", C.c_int32), ("alpha_cutoff", C.c_float),
+        ("base_color_alpha", C.c_int32), ("thin_walled", C.c_int32), ("emitter_only", C.c_int32),
+        ("hidden_to_camera", C.c_int32), ("emission_side", C.c_int32),
+    ]
+
+
+def bridge_material(**fields):
+    """A MaterialData with the bridge's defaults (RenderBridge.h:27-56), then `fields`."""
+    m = BridgeMaterial()
+    m.base_color[:] = (0.8, 0.8, 0.8)
+    m.opacity, m.metallic, m.roughness, m.ior = 1.0, 0.0, 1.0, 1.5
+    m.transmission, m.specular = 0.0, 0.0
+    m.clearcoat, m.clearcoat_roughness, m.sheen = 0.0, 0.03, 0.0
+    m.alpha_mode, m.alpha_cutoff = 1, 0.5
+    m.emission_side = 2
+    for k, v in fields.items():
+        if k in ("base_color", "emission"):
+            getattr(m, k)[:] = v
+        else:
+            setattr(m, k, v)
+    return m
+
+
 class CameraData(C.Structure):
     """bridge::CameraData (RenderBridge.h:128-144)."""
     _fields_ = [

@@ -480,6 +480,50 @@ int phx_trace_occluded(phx_context *c, const float *o, const float *d, const flo
   return PHX_OK;
 }
 
+int phx_convert_material(const phx_bridge_material *in, phx_material *out) {
+  if (!in || !out) return PHX_ERR_INVALID;
+  // The parameter sources (attributes, samplers) are not part of this
+  // library's material; only constants convert exactly.
+  for (int32_t kind : {in->base_color_src, in->opacity_src, in->metallic_src, in->roughness_src, in->emission_src,
+                       in->transmission_src, in->normal_src})
+    if (kind != 0) return PHX_ERR_UNSUPPORTED;
+  phx_material m{};
+  // pt::Material defaults of the fields convert_material does not set
+  m.sheen_tint = 0.5f;
+  m.specular_tint = 0.f;
+  m.anisotropic = 0.f;
+  m.subsurface = 0.f;
+  m.subsurface_color[0] = m.subsurface_color[1] = m.subsurface_color[2] = 1.f;
+  m.subsurface_radius = 1.f;
+  m.base_color_tex = m.normal_tex = m.roughness_tex = m.metallic_tex = m.emission_tex = m.alpha_tex = -1;
+  for (int k = 0; k < 3; ++k) {
+    m.base_color[k] = in->base_color[k];
+    m.emission[k] = in->emission[k];
+  }
+  m.metallic = in->metallic;
+  m.roughness = in->roughness * in->roughness;
+  m.ior = in->ior;
+  m.transmission = in->transmission;
+  m.specular = 0.5f * in->specular;
+  m.clearcoat = in->clearcoat;
+  m.clearcoat_roughness = in->clearcoat_roughness;
+  m.sheen = in->sheen;
+  m.emission_strength = 1.f;
+  m.alpha = in->opacity;
+  m.alpha_mode = in->alpha_mode == 0 ? PHX_ALPHA_BLEND : (in->alpha_mode == 2 ? PHX_ALPHA_MASK : PHX_ALPHA_OPAQUE);
+  m.alpha_cutoff = in->alpha_cutoff;
+  m.thin = in->thin_walled ? 1 : 0;
+  m.flags = PHX_MATERIAL_FLAG_REFRACTIVE;
+  if (in->emitter_only) m.flags |= PHX_MATERIAL_FLAG_EMITTER_ONLY;
+  if (in->hidden_to_camera) m.flags |= PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA;
+  if (in->emission_side == 2)
+    m.flags |= PHX_MATERIAL_FLAG_EMIT_BOTH_SIDES;
+  else if (in->emission_side == 1)
+    m.flags |= PHX_MATERIAL_FLAG_EMIT_BACK_ONLY;
+  *out = m;
+  return PHX_OK;
+}
+
 int phx_camera_perspective(phx_camera *out, const float f[3], const float at[3], const float up[3], float vfov,
                            float aspect, float aperture, float focus) {
   if (!out || !f || !at || !up) return PHX_ERR_INVALID;

@@ -17,6 +17,17 @@ from ._lib import PhxError, fptr, lib
 from .backend import Context
 
 
+def convert_material(bridge_material):
+    """convert_material (RenderBridge.cpp:324-370) through phx_convert_material:
+    the pt-level Material the bridge renders an abi.BridgeMaterial with.  Only
+    constant parameter sources are supported (PhxError otherwise)."""
+    out = abi.Material()
+    code = lib().phx_convert_material(C.byref(bridge_material), C.byref(out))
+    if code != 0:
+        raise PhxError(code, "phx_convert_material: attribute and sampler sources are not supported")
+    return out
+
+
 class Renderer:
     def __init__(self, device=0):
         self._device = device

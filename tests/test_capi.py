@@ -121,7 +121,8 @@ def test_ctypes_layouts_equal_the_c_header(tmp_path):
                "phx_volume": abi.Volume, "phx_scene_desc": abi.SceneDesc, "phx_camera": abi.Camera,
                "phx_render_params": abi.RenderParams, "phx_hit": abi.Hit, "phx_hit_compact": abi.HitCompact,
                "phx_tuning": abi.Tuning, "phx_stats": abi.Stats, "phx_view": abi.View,
-               "phx_camera_data": abi.CameraData, "phx_render_settings": abi.RenderSettings}
+               "phx_camera_data": abi.CameraData, "phx_render_settings": abi.RenderSettings,
+               "phx_bridge_material": abi.BridgeMaterial}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "phx.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
@@ -138,3 +139,69 @@ def test_ctypes_layouts_equal_the_c_header(tmp_path):
         assert int(got[cname]) == C.sizeof(cls), cname
         for fname, _ in cls._fields_:
             assert int(got[f"{cname}.{fname}"]) == getattr(cls, fname).offset, f"{cname}.{fname}"
+
+
+def _fields(m):
+    out = {}
+    for name, _ in m._fields_:
+        v = getattr(m, name)
+        out[name] = tuple(v) if hasattr(v, "__len__") else v
+    return out
+
+
+def test_convert_material_follows_the_bridge():
+    # convert_material (RenderBridge.cpp:324-370) on a material that sets every
+    # field the bridge reads
+    from photon_amd.renderer import convert_material
+    f32 = np.float32
+    bm = abi.bridge_material(base_color=(0.2, 0.3, 0.4), opacity=0.35, metallic=0.25, roughness=0.6, ior=1.33,
+                             transmission=0.7, specular=0.8, emission=(1.0, 2.0, 3.0), clearcoat=0.4,
+                             clearcoat_roughness=0.1, sheen=0.3, alpha_mode=2, alpha_cutoff=0.25, thin_walled=1,
+                             emitter_only=1, hidden_to_camera=1, emission_side=1)
+    m = convert_material(bm)
+    assert m.roughness == f32(0.6) * f32(0.6)          # the BSDF takes the GGX alpha
+    assert m.specular == f32(0.5) * f32(0.8)           # 0.5 is the full dielectric reflectance
+    assert tuple(m.base_color) == tuple(f32([0.2, 0.3, 0.4]))
+    assert tuple(m.emission) == (1.0, 2.0, 3.0) and m.emission_strength == 1.0
+    assert (m.metallic, m.ior, m.transmission) == (f32(0.25), f32(1.33), f32(0.7))
+    assert (m.clearcoat, m.clearcoat_roughness, m.sheen) == (f32(0.4), f32(0.1), f32(0.3))
+    assert m.alpha == f32(0.35) and m.alpha_mode == abi.PHX_ALPHA_MASK and m.alpha_cutoff == 0.25
+    assert m.thin == 1
+    assert m.flags == (abi.PHX_MATERIAL_FLAG_REFRACTIVE | abi.PHX_MATERIAL_FLAG_EMITTER_ONLY
+                       | abi.PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA | abi.PHX_MATERIAL_FLAG_EMIT_BACK_ONLY)
+    # fields the bridge leaves alone keep pt::Material's defaults (material.h:35-82)
+    d = abi.default_material()
+    for name in ("sheen_tint", "specular_tint", "anisotropic", "subsurface", "subsurface_radius"):
+        assert getattr(m, name) == getattr(d, name), name
+    assert tuple(m.subsurface_color) == (1.0, 1.0, 1.0)
+    assert (m.base_color_tex, m.normal_tex, m.roughness_tex, m.metallic_tex, m.emission_tex, m.alpha_tex) == (-1,) * 6
+    # alpha-mode and emission-side codings
+    modes = {0: abi.PHX_ALPHA_BLEND, 1: abi.PHX_ALPHA_OPAQUE, 2: abi.PHX_ALPHA_MASK, 7: abi.PHX_ALPHA_OPAQUE}
+    for code, want in modes.items():
+        assert convert_material(abi.bridge_material(alpha_mode=code)).alpha_mode == want, code
+    sides = {0: 0, 1: abi.PHX_MATERIAL_FLAG_EMIT_BACK_ONLY, 2: abi.PHX_MATERIAL_FLAG_EMIT_BOTH_SIDES}
+    for side, bit in sides.items():
+        assert convert_material(abi.bridge_material(emission_side=side)).flags == abi.PHX_MATERIAL_FLAG_REFRACTIVE | bit
+
+
+def test_convert_material_rejects_parameter_sources():
+    from photon_amd._lib import PhxError
+    from photon_amd.renderer import convert_material
+    for src in ("base_color_src", "opacity_src", "metallic_src", "roughness_src", "emission_src", "transmission_src",
+                "normal_src"):
+        for kind in (1, 2):  # attribute, sampler
+            with pytest.raises(PhxError) as e:
+                convert_material(abi.bridge_material(**{src: kind}))
+            assert e.value.code == -7, (src, kind)  # PHX_ERR_UNSUPPORTED
+
+
+def test_scene_materials_are_converted_bridge_materials():
+    # the scenes' diffuse and emitter materials are MaterialData after convert_material
+    from photon_amd import scenes
+    from photon_amd.renderer import convert_material
+    rgb = (0.73, 0.2, 0.05)
+    assert _fields(scenes.diffuse(rgb)) == _fields(convert_material(abi.bridge_material(base_color=rgb,
+                                                                                          emission_side=0)))
+    e = convert_material(abi.bridge_material(base_color=(0.0, 0.0, 0.0), emission=(17.0, 12.0, 4.0), emitter_only=1,
+                                             emission_side=0))
+    assert _fields(scenes.emitter((17.0, 12.0, 4.0))) == _fields(e)
