@@ -196,8 +196,9 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   if ((r = upload(c, c->normals, d->normals, d->normals ? size_t(d->vertex_count) * 3 : 0))) return r;
   if ((r = upload(c, c->texcoords, d->texcoords, d->texcoords ? size_t(d->vertex_count) * 2 : 0))) return r;
   if ((r = upload(c, c->colors, d->vertex_colors, d->vertex_colors ? size_t(d->vertex_count) * 3 : 0))) return r;
+  std::vector<uint32_t> mids;
   if (d->material_ids) {
-    std::vector<uint32_t> mids(d->material_ids, d->material_ids + d->triangle_count);
+    mids.assign(d->material_ids, d->material_ids + d->triangle_count);
     const uint32_t mc = std::max<uint32_t>(d->material_count, 1);
     for (auto &m : mids)
       if (m >= mc) m = 0;  // out-of-range ids fall back to material 0 (RenderBridge.cpp:479-484)
@@ -205,10 +206,21 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   } else {
     c->material_ids.release();
   }
+  {
+    std::vector<uint4> attr(bvh.tris.size());
+    for (size_t k = 0; k < bvh.tris.size(); ++k) {
+      uint32_t prim;
+      std::memcpy(&prim, &bvh.tris[k].v0.w, sizeof prim);
+      const uint32_t *ix = d->indices + size_t(prim) * 3;
+      attr[k] = make_uint4(ix[0], ix[1], ix[2], mids.empty() ? 0u : mids[prim]);
+    }
+    if ((r = upload(c, c->tri_attr, attr.data(), attr.size()))) return r;
+  }
   c->triangle_count = d->triangle_count;
   c->vertex_count = d->vertex_count;
   c->host_positions.assign(d->positions, d->positions + size_t(d->vertex_count) * 3);
   c->host_indices.assign(d->indices, d->indices + size_t(d->triangle_count) * 3);
+  c->host_material_ids = std::move(mids);
   c->px_built = false;
   // Bounds of the triangles' vertices (the BVH root box).
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -225,6 +237,7 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   s = DevScene{};
   s.nodes = c->nodes.as<BvhNode>();
   s.tris = c->tris.as<TriRec>();
+  s.tri_attr = c->tri_attr.as<uint4>();
   s.indices = c->indices.as<uint32_t>();
   s.positions = c->positions.as<float>();
   s.normals = c->normals.as<float>();
@@ -233,7 +246,7 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   s.material_ids = c->material_ids.as<uint32_t>();
   s.triangle_count = d->triangle_count;
   c->stats.bvh_nodes = bvh.nodes.size();
-  c->stats.bvh_bytes = bvh.nodes.size() * sizeof(BvhNode) + bvh.tris.size() * sizeof(TriRec);
+  c->stats.bvh_bytes = bvh.nodes.size() * sizeof(BvhNode) + bvh.tris.size() * (sizeof(TriRec) + sizeof(uint4));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->have_accel = true;
   return PHX_OK;

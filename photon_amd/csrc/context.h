@@ -61,7 +61,7 @@ struct phx_context {
 
   // geometry (seam A)
   bool have_accel = false;
-  DBuf nodes, tris, indices, positions, normals, texcoords, colors, material_ids;
+  DBuf nodes, tris, tri_attr, indices, positions, normals, texcoords, colors, material_ids;
   uint32_t triangle_count = 0, vertex_count = 0;
   float bounds_lo[3]{}, bounds_hi[3]{};
   uint32_t bvh_depth = 0;
@@ -110,9 +110,10 @@ struct phx_context {
   // BVH (built on first use) and full-width per-slot state
   std::vector<float> host_positions;
   std::vector<uint32_t> host_indices;
+  std::vector<uint32_t> host_material_ids;  // range-checked, empty without material ids
   bool px_built = false;
   uint32_t px_root = 0, px_slots = 0;
-  DBuf px_nodes, px_prims, px_tris, px_o, px_d, px_tp, px_st, px_hit, px_sh_o, px_sh_d, px_sh_c, px_flags, px_accum,
+  DBuf px_nodes, px_prims, px_tris, px_tri_attr, px_o, px_d, px_tp, px_st, px_hit, px_sh_o, px_sh_d, px_sh_c, px_flags, px_accum,
       px_count;
 };
 

@@ -65,6 +65,9 @@ struct alignas(16) TriRec {
 struct DevScene {
   const BvhNode *nodes;
   const TriRec *tris;
+  // In leaf order beside tris: the triangle's vertex indices and material id,
+  // so a hit's attributes load one dependent step after the hit, not two.
+  const uint4 *tri_attr;
   const uint32_t *indices;
   const float *positions;
   const float *normals;

@@ -76,10 +76,11 @@ struct SurfaceHit {
 PHX_DEV SurfaceHit surface_at(const DevScene &s, uint32_t tri, float t, float b1, float b2, vec3 ro, vec3 rd) {
   SurfaceHit h;
   const TriRec tr = s.tris[tri];
+  const uint4 ta = s.tri_attr[tri];
   h.prim = __float_as_uint(tr.v0.w);
   h.p = ro + rd * t;
   h.ng = normalize(cross(vec3{tr.e1.x, tr.e1.y, tr.e1.z}, vec3{tr.e2.x, tr.e2.y, tr.e2.z}));
-  const uint32_t i0 = s.indices[h.prim * 3], i1 = s.indices[h.prim * 3 + 1], i2 = s.indices[h.prim * 3 + 2];
+  const uint32_t i0 = ta.x, i1 = ta.y, i2 = ta.z;
   const float w0 = 1.f - b1 - b2;
   h.sn = h.ng;
   if (s.normals) {
@@ -99,7 +100,7 @@ PHX_DEV SurfaceHit surface_at(const DevScene &s, uint32_t tri, float t, float b1
               ld3(s.colors + size_t(i2) * 3) * b2;
     h.has_color = true;
   }
-  h.mid = s.material_ids ? s.material_ids[h.prim] : 0u;
+  h.mid = ta.w;  // the material id (0 without material ids), as material_ids[prim]
   // pathtracer.cpp:906-910
   if (!(dot(h.sn, h.sn) > 0.25f)) h.sn = h.ng;
   else if (dot(h.sn, h.ng) < 0.f) h.sn = h.sn * -1.f;

@@ -227,6 +227,12 @@ extern "C" int phx_render_reference_structure(phx_context *c, const phx_camera *
     if ((r = upload(c, c->px_nodes, bvh.nodes.data(), bvh.nodes.size()))) return r;
     if ((r = upload(c, c->px_prims, bvh.prim_ids.data(), bvh.prim_ids.size()))) return r;
     if ((r = upload(c, c->px_tris, recs.data(), recs.size()))) return r;
+    // the attribute records in the same (primitive) order
+    std::vector<uint4> attr(c->triangle_count);
+    for (uint32_t t = 0; t < c->triangle_count; ++t)
+      attr[t] = make_uint4(c->host_indices[t * 3], c->host_indices[t * 3 + 1], c->host_indices[t * 3 + 2],
+                           c->host_material_ids.empty() ? 0u : c->host_material_ids[t]);
+    if ((r = upload(c, c->px_tri_attr, attr.data(), attr.size()))) return r;
     c->px_root = bvh.root;
     HIP_TRY(c, hipStreamSynchronize(stm));
     c->px_built = true;
@@ -235,6 +241,7 @@ extern "C" int phx_render_reference_structure(phx_context *c, const phx_camera *
                 c->positions.as<float>(), c->px_root};
   DevScene ds = c->dev;
   ds.tris = c->px_tris.as<TriRec>();
+  ds.tri_attr = c->px_tri_attr.as<uint4>();
 
   // per-slot state (pathtracer.cpp:505-562)
   const size_t n = npix;
