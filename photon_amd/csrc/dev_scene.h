@@ -233,12 +233,12 @@ PHX_DEV uint32_t cdf_search(const float *cdf, uint32_t n, const uint32_t *guide,
   return lo;
 }
 // Marginal then conditional CDF search, each over a table with a leading 0
-// (environment_map.h:56-115).
-PHX_DEV vec3 env_sample(const DevScene &s, Rng &rng, float &pdf) {
+// (environment_map.h:56-115).  Also returns the sampled texel (col, row).
+PHX_DEV vec3 env_sample(const DevScene &s, Rng &rng, float &pdf, uint32_t &col, uint32_t &row) {
   const float u1 = rng.next(), u2 = rng.next();
-  const uint32_t row = cdf_search(s.env_marginal, s.env_h, s.env_guide_m, u2);
+  row = cdf_search(s.env_marginal, s.env_h, s.env_guide_m, u2);
   const float *c = s.env_cond + size_t(row) * (s.env_w + 1);
-  const uint32_t col = cdf_search(c, s.env_w, s.env_guide_c + size_t(row) * (kEnvGuide + 1), u1);
+  col = cdf_search(c, s.env_w, s.env_guide_c + size_t(row) * (kEnvGuide + 1), u1);
   const float rp = s.env_marginal[row + 1] - s.env_marginal[row];
   const float cp = c[col + 1] - c[col];
   const float u = (float(col) + rng.next()) / float(s.env_w);
@@ -424,10 +424,18 @@ PHX_DEV Direct sample_direct(const DevScene &s, vec3 p, Rng &rng) {
   }
   if (s.has_env) {
     float ep = 0.f;
-    const vec3 wi = env_sample(s, rng, ep);
+    uint32_t col, row;
+    const vec3 wi = env_sample(s, rng, ep, col, row);
+    // The radiance is the texel the direction maps back to, as env_eval does;
+    // that is the sampled texel but at texel edges, so it is loaded beside the
+    // direction's arithmetic and only reloaded when the two differ.
+    const vec3 Ls = ld3(s.env_pixels + (size_t(row) * s.env_w + col) * 3);
     if (!(ep > 0.f)) return ds;
+    uint32_t x, y;
+    vec2 uv;
+    env_texel(s, wi, x, y, uv);
     ds.wi = wi;
-    ds.L = env_eval(s, wi);
+    ds.L = (x == col && y == row) ? Ls : ld3(s.env_pixels + (size_t(y) * s.env_w + x) * 3);
     ds.pdf = ep * pick_pdf;
     ds.dist = 1e30f;
     ds.mis = true;
