@@ -117,9 +117,13 @@ struct NoHooks {
   PHX_DEV void bsdf(const phx_material &, const SurfaceHit &, vec3, vec3, vec3, vec3, const BsdfSample &) {}
 };
 
-template <class Hooks>
+// VOL_PRE: the volume event was sampled ahead (k_volume) and is given by
+// vol_ev (scatter distance or -1, albedo) and vol_rng (the RNG state after its
+// draws); otherwise it is sampled here.  The draws are the same either way.
+template <class Hooks, bool VOL_PRE = false>
 PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState &ps, bool hr_hit, uint32_t htri,
-                        float ht, float b1, float b2, Rng &rng, ShadeOut &out, Hooks &hooks) {
+                        float ht, float b1, float b2, Rng &rng, ShadeOut &out, Hooks &hooks,
+                        float4 vol_ev = make_float4(-1.f, 0.f, 0.f, 0.f), uint32_t vol_rng = 0u) {
   out.cont = false;
   out.shadow = false;
   out.add = vec3{0.f, 0.f, 0.f};
@@ -140,8 +144,14 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
   if (s.vol_count > 0) {
     // Scattering inside a volume in front of the hit (:789-857).
     vec3 alb{0.f, 0.f, 0.f};
-    const float t_end = hr_hit ? ht : ps.tmax;
-    const float ts = vol_sample_scatter(s, ro, rd, ps.tmin, t_end, rng, alb);
+    float ts;
+    if (VOL_PRE) {
+      ts = vol_ev.x;
+      alb = vec3{vol_ev.y, vol_ev.z, vol_ev.w};
+      rng.s = vol_rng;
+    } else {
+      ts = vol_sample_scatter(s, ro, rd, ps.tmin, hr_hit ? ht : ps.tmax, rng, alb);
+    }
     if (ts >= 0.f) {
       const vec3 p = ro + rd * ts;
       if (primary) {

@@ -119,6 +119,11 @@ struct StreamQueues {
   uint32_t *counters;
   unsigned long long *stats;
   int *spill;        // per-thread traversal stack spill
+  // Per ray of the current queue, from k_volume (runs whose scenes hold
+  // volumes): the volume event (scatter distance or -1, albedo) and the RNG
+  // state after its draws.
+  float4 *vol_ev;
+  uint32_t *vol_rng;
 };
 
 struct FrameBuffers {
@@ -139,6 +144,7 @@ struct LoopShape {
   uint32_t chunk;        // work items a wave takes per fetch (multiple of 64)
   uint32_t refill;       // idle lanes of a traversal wave that trigger a refill (1..64)
   uint32_t groups;       // material groups of sorted shading (0: queue order)
+  uint32_t volumes;      // a scene of the run holds volumes: k_volume runs before k_shade
 };
 
 // RenderParams -> the shading constants of one frame (render.cpp).
@@ -164,6 +170,7 @@ hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopSha
 // With ls.groups > 0 the rays of each work-group are shaded in material-group order.
 hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
+hipError_t launch_volume(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 // Sums the item's samples of each pixel into accum (first: overwrite).
 hipError_t launch_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum, bool first,
                          hipStream_t st);

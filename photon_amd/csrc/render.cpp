@@ -216,6 +216,11 @@ hipError_t launch_iterations(const ViewArgs &va, const StreamQueues &q, const Lo
     if (tm && (e = tm->begin(kTimedExtend, st)) != hipSuccess) return e;
     if ((e = launch_extend(va, q, ls, st)) != hipSuccess) return e;
     if (tm && (e = tm->end(st)) != hipSuccess) return e;
+    if (ls.volumes) {  // the shading's volume events, counted as shading
+      if (tm && (e = tm->begin(kTimedShade, st)) != hipSuccess) return e;
+      if ((e = launch_volume(va, q, ls, st)) != hipSuccess) return e;
+      if (tm && (e = tm->end(st)) != hipSuccess) return e;
+    }
     if (fork && (e = hipStreamWaitEvent(st, fork->ev_join, 0)) != hipSuccess) return e;
     if (tm && (e = tm->begin(kTimedShade, st)) != hipSuccess) return e;
     if ((e = launch_shade(va, q, ls, st)) != hipSuccess) return e;
@@ -311,7 +316,12 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   HIP_TRY(c, c->counters.reserve(kCounterWords * 4));
   HIP_TRY(c, c->stats_buf.reserve(kStatWords * 8));
   if (!c->pinned) HIP_TRY(c, hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 64, hipHostMallocDefault));
-  const LoopShape ls = loop_shape(c, cap);
+  LoopShape ls = loop_shape(c, cap);
+  for (const phx_context *sc : scenes) ls.volumes |= sc->dev.vol_count > 0 ? 1u : 0u;
+  if (ls.volumes) {
+    HIP_TRY(c, c->vol_ev.reserve(size_t(cap) * 16));
+    HIP_TRY(c, c->vol_rng.reserve(size_t(cap) * 4));
+  }
   int r = ensure_trace_scratch(c, size_t(ls.trace_grid) * ls.trace_block);
   if (r) return r;
 
@@ -348,6 +358,8 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   q.counters = c->counters.as<uint32_t>();
   q.stats = c->stats_buf.as<unsigned long long>();
   q.spill = c->spill.as<int>();
+  q.vol_ev = ls.volumes ? c->vol_ev.as<float4>() : nullptr;
+  q.vol_rng = ls.volumes ? c->vol_rng.as<uint32_t>() : nullptr;
 
   hipStream_t st = c->stream;
   // PHX_TRACE_COUNTS=N: read the queue length every N iterations and print it.

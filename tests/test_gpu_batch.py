@@ -154,3 +154,27 @@ def test_batch_errors(batch_setup):
     bad.adaptive_sampling = 1
     with pytest.raises(PhxError):
         br.render([(views[0][0], views[0][1], bad)])
+
+
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_batch_with_a_volume_scene_equals_plain_renders(use_graph):
+    # A fog box (volume events sampled ahead of shading) batched with a scene
+    # without volumes, and two views of the fog box alone: each view equals
+    # its own render.
+    from photon_amd.backend import BatchRenderer, Context
+    fog = scenes.fog_box(n=32, macro=8, env_size=(64, 32))
+    cb = scenes.cornell_box()
+    pf, pc = _tracer(lambda: Context(0), fog), _tracer(lambda: Context(0), cb)
+    mixed = [_view(pf, fog, 24, 16, 4, 12), _view(pc, cb, 16, 16, 4, 8), _view(pf, fog, 16, 24, 2, 12, offset=3,
+                                                                             orbit=0.5)]
+    plain = []
+    for pt, cam, p in mixed:
+        pt.camera, pt.params = cam, p
+        plain.append(pt.render())
+    for views, refs, tag in ((mixed, plain, "fog + cornell"), ([mixed[0], mixed[2]], [plain[0], plain[2]], "fog")):
+        br = BatchRenderer(Context(0))
+        br.ctx.set_tuning(use_graph=use_graph)
+        for rep in range(2):
+            br.render(views)
+            for v in range(len(views)):
+                _check_equal(br.read(v), refs[v], f"{tag} graph={use_graph} rep={rep} view {v}")
