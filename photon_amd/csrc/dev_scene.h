@@ -303,10 +303,13 @@ PHX_DEV TexTaps tex_taps(const DevScene &s, int id, float u, float v) {
 }
 
 // Material at a hit with atlas textures applied (pathtracer.cpp:186-247).
+// TEX = false leaves the texture lookups out of the code (for scenes without
+// textures, where they never run).
+template <bool TEX = true>
 PHX_DEV phx_material resolve_material(const DevScene &s, uint32_t mid, vec2 uv, bool has_color, vec3 color,
                                       float &alpha) {
   phx_material m = s.materials[mid];
-  const bool tex = s.tex_count > 0;
+  const bool tex = TEX && s.tex_count > 0;
   alpha = m.alpha;
   if (m.alpha_tex >= 0 && tex) alpha = tex_sample(s, m.alpha_tex, uv.x, uv.y).x;
   if (m.base_color_tex >= 0 && tex) {
@@ -367,6 +370,9 @@ struct Direct {
   float pdf, dist;
   bool mis, valid;
 };
+// LIGHTS = false leaves the analytic lights out of the code (for scenes
+// without them, where that branch never runs).
+template <bool LIGHTS = true>
 PHX_DEV Direct sample_direct(const DevScene &s, vec3 p, Rng &rng) {
   Direct ds{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, 0.f, false, false};
   const uint32_t nstrat = s.strategy_count;
@@ -374,7 +380,7 @@ PHX_DEV Direct sample_direct(const DevScene &s, vec3 p, Rng &rng) {
   const float pick_pdf = 1.f / float(nstrat);
   uint32_t pick = uint32_t(rng.next() * float(nstrat));
   if (pick >= nstrat) pick = nstrat - 1;
-  if (pick < s.light_count) {
+  if (LIGHTS && pick < s.light_count) {
     const LightSample ls = sample_light(s.lights[pick], p, rng);
     if (!(ls.pdf > 0.f)) return ds;
     ds.wi = ls.wi;

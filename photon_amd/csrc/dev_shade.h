@@ -120,7 +120,10 @@ struct NoHooks {
 // VOL_PRE: the volume event was sampled ahead (k_volume) and is given by
 // vol_ev (scatter distance or -1, albedo) and vol_rng (the RNG state after its
 // draws); otherwise it is sampled here.  The draws are the same either way.
-template <class Hooks, bool VOL_PRE = false>
+// LEAN: the scene has no volumes, no textures and no analytic lights, and
+// the code for them is left out (it would never run); a smaller shading
+// kernel keeps more of the rest in registers.
+template <class Hooks, bool VOL_PRE = false, bool LEAN = false>
 PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState &ps, bool hr_hit, uint32_t htri,
                         float ht, float b1, float b2, Rng &rng, ShadeOut &out, Hooks &hooks,
                         float4 vol_ev = make_float4(-1.f, 0.f, 0.f, 0.f), uint32_t vol_rng = 0u) {
@@ -141,7 +144,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
   out.next = ps;
   out.next.path_it = ps.path_it + 1u;
 
-  if (s.vol_count > 0) {
+  if (!LEAN && s.vol_count > 0) {
     // Scattering inside a volume in front of the hit (:789-857).
     vec3 alb{0.f, 0.f, 0.f};
     float ts;
@@ -162,7 +165,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
       if (depth >= w.max_depth || emission_only) return;
       const vec3 tpv = tp * alb;
       const float phase = 0.25f * kInvPi;
-      const Direct ds = sample_direct(s, p, rng);
+      const Direct ds = sample_direct<!LEAN>(s, p, rng);
       if (ds.valid) {
         const float wm = ds.mis ? power_heuristic(ds.pdf, phase) : 1.f;
         out.sc = clamp_contribution(tpv * ds.L * (phase * wm / ds.pdf), int(depth), w.clamp_direct, w.clamp_indirect);
@@ -221,7 +224,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
   const SurfaceHit sh = surface_at(s, htri, ht, b1, b2, ro, rd);
   const vec3 p = sh.p, ng = sh.ng, wo = -rd;
   float alpha = 1.f;
-  const phx_material mat = resolve_material(s, sh.mid, sh.uv, sh.has_color, sh.color, alpha);
+  const phx_material mat = resolve_material<!LEAN>(s, sh.mid, sh.uv, sh.has_color, sh.color, alpha);
   const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0 && primary;
   if (hidden || (alpha < 1.f && rng.next() >= alpha)) {
     // Stochastic transparency (:929-951): continue behind the surface.
@@ -252,7 +255,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
     n = n * -1.f;
     sn = sn * -1.f;
   }
-  if (mat.normal_tex >= 0 && s.tex_count > 0) {
+  if (!LEAN && mat.normal_tex >= 0 && s.tex_count > 0) {
     vec3 nm = tex_sample(s, mat.normal_tex, sh.uv.x, sh.uv.y);
     nm = nm * 2.f - vec3{1.f, 1.f, 1.f};
     vec3 ta, bi;
@@ -276,7 +279,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
   if ((mat.flags & PHX_MATERIAL_FLAG_EMITTER_ONLY) != 0 || depth >= w.max_depth || emission_only) return;
   {
     // Next event estimation (:1006-1035).
-    const Direct ds = sample_direct(s, p, rng);
+    const Direct ds = sample_direct<!LEAN>(s, p, rng);
     if (ds.valid) {
       float bp;
       const vec3 f = bsdf_eval_pdf(mat, wo, ds.wi, n, sn, bp);

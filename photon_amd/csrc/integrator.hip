@@ -187,7 +187,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_e
   block_sum<B / 64>(&q.stats[kStatClosest], live);
 }
 
-template <class P, bool SORTED>
+template <class P, bool SORTED, bool LEAN>
 __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, uint32_t groups, uint32_t it,
                                            uint32_t count, uint32_t tile) {
   uint32_t i = tile + threadIdx.x;
@@ -248,8 +248,8 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
       vol_rng = q.vol_rng[i];
     }
     NoHooks hooks;
-    shade_path<NoHooks, true>(s, r.sp, ps, htri != 0xffffffffu, htri, hit.x, hit.y, hit.z, rng, out, hooks, vol_ev,
-                              vol_rng);
+    shade_path<NoHooks, true, LEAN>(s, r.sp, ps, htri != 0xffffffffu, htri, hit.x, hit.y, hit.z, rng, out, hooks,
+                                    vol_ev, vol_rng);
     if (out.add.x != 0.f || out.add.y != 0.f || out.add.z != 0.f || out.add_alpha != 0.f) {
       const float4 L = q.sample_L[g];
       q.sample_L[g] = make_float4(L.x + out.add.x, L.y + out.add.y, L.z + out.add.z, L.w + out.add_alpha);
@@ -281,14 +281,14 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
 // shadow rays go to compacted queues, one atomic per work-group.  The grid covers the
 // path slots and work-groups beyond the queue exit at once: a grid-stride loop
 // over the queue measured slower, since the loop raises the spill traffic.
-template <class P, bool SORTED>
+template <class P, bool SORTED, bool LEAN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(P v, StreamQueues q,
                                                                                       uint32_t groups) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[ray_count_word(it)];
   const uint32_t tile = blockIdx.x * 256u;
   if (tile >= count) return;
-  shade_tile<P, SORTED>(v, q, groups, it, count, tile);
+  shade_tile<P, SORTED, LEAN>(v, q, groups, it, count, tile);
 }
 
 // Any hit over the shadow queue (trace_occluded; persistent waves, per-lane
@@ -508,9 +508,15 @@ hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShap
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
     if (ls.groups) {
-      hipLaunchKernelGGL((k_shade<P, true>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, ls.groups);
+      if (ls.lean)
+        hipLaunchKernelGGL((k_shade<P, true, true>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, ls.groups);
+      else
+        hipLaunchKernelGGL((k_shade<P, true, false>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, ls.groups);
     } else {
-      hipLaunchKernelGGL((k_shade<P, false>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, 0u);
+      if (ls.lean)
+        hipLaunchKernelGGL((k_shade<P, false, true>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, 0u);
+      else
+        hipLaunchKernelGGL((k_shade<P, false, false>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, 0u);
     }
     return hipGetLastError();
   });

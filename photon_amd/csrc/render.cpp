@@ -317,7 +317,11 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   HIP_TRY(c, c->stats_buf.reserve(kStatWords * 8));
   if (!c->pinned) HIP_TRY(c, hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 64, hipHostMallocDefault));
   LoopShape ls = loop_shape(c, cap);
-  for (const phx_context *sc : scenes) ls.volumes |= sc->dev.vol_count > 0 ? 1u : 0u;
+  ls.lean = 1u;
+  for (const phx_context *sc : scenes) {
+    ls.volumes |= sc->dev.vol_count > 0 ? 1u : 0u;
+    if (sc->dev.vol_count > 0 || sc->dev.tex_count > 0 || sc->dev.light_count > 0) ls.lean = 0u;
+  }
   if (ls.volumes) {
     HIP_TRY(c, c->vol_ev.reserve(size_t(cap) * 16));
     HIP_TRY(c, c->vol_rng.reserve(size_t(cap) * 4));
