@@ -155,7 +155,9 @@ PHX_DEV uint32_t ray_material(const P &v, uint32_t item, uint32_t prim) {
 // per-lane refill).  With groups > 0 the hit record's top 4 bits carry the
 // hit's material group.  Six waves per SIMD: the kernel fits 80 registers
 // without spills.
-template <class P, int B>
+// TAG: the run shades by material group (more than one), so hits carry their
+// group; without it the tagging code is left out, which spares registers.
+template <class P, int B, bool TAG>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_extend(P v, StreamQueues q,
                                                                                       uint32_t chunk, uint32_t refill,
                                                                                       uint32_t groups) {
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_e
       [&](uint32_t i, bool hit, const TraceHit &h) {
         uint32_t w = h.tri;
         // one group: the shade kernel only parts hits from misses, no tag needed
-        if (hit && groups > 1) w |= (ray_material(v, q.st[buf][i].y >> kItemShift, h.prim) % groups) << 28;
+        if (TAG && hit) w |= (ray_material(v, q.st[buf][i].y >> kItemShift, h.prim) % groups) << 28;
         q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(w));
       });
   block_sum<B / 64>(&q.stats[kStatClosest], live);
@@ -495,11 +497,18 @@ hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead
 hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st) {
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
-    if (ls.trace_block == 128)
-      hipLaunchKernelGGL((k_extend<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk, ls.refill,
+    const bool tag = ls.groups > 1;
+    if (ls.trace_block == 128 && tag)
+      hipLaunchKernelGGL((k_extend<P, 128, true>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk, ls.refill,
+                         ls.groups);
+    else if (ls.trace_block == 128)
+      hipLaunchKernelGGL((k_extend<P, 128, false>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk, ls.refill,
+                         ls.groups);
+    else if (tag)
+      hipLaunchKernelGGL((k_extend<P, 256, true>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill,
                          ls.groups);
     else
-      hipLaunchKernelGGL((k_extend<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill,
+      hipLaunchKernelGGL((k_extend<P, 256, false>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill,
                          ls.groups);
     return hipGetLastError();
   });
