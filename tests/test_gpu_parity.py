@@ -1,6 +1,6 @@
 """GPU parity: the HIP kernels, called through the C ABI, against the oracle.
 
-Bars (from the north star): closest-hit primitive identical for >= 99.99% of
+Bars: closest-hit primitive identical for >= 99.99% of
 rays with t within 1e-5 relative; Disney eval/pdf within 1e-5 relative (with a
 1e-8 absolute floor near zero); images within relMSE 1e-3 of the oracle at the
 same sample streams (observed far below it: the GPU draws the same random
