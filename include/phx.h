@@ -6,7 +6,9 @@
  * failure on a context.  One context serves one host thread at a time (the
  * reference's backends are driven from a single thread, RenderBridge.cpp:66-77);
  * the library starts no host threads.  All kernels of a context run on the
- * context's HIP stream, or on the stream passed to a call that takes one.
+ * context's HIP stream, or on the stream passed to a call that takes one; the
+ * bounce loop of a render also runs its any-hit pass on a second stream of the
+ * context, joined to the first before every shading launch.
  *
  * Data types mirror the reference's scene model so a scene converted for the
  * reference converts 1:1 for this backend:
