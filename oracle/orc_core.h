@@ -11,10 +11,12 @@
 //
 // Pinning (tests/test_oracle.py): RNG, hash, power heuristic, ONB, triangle
 // test, reference BVH (against brute force), Disney special cases (Lambert,
-// metal GGX, sheen, clearcoat, anisotropy limit) and invariants, env map,
-// camera, majorants, ratio/delta tracking and two integrator closed forms.
+// metal GGX, sheen, clearcoat, anisotropy limit, subsurface at normal
+// incidence) and invariants, env map, camera, majorants, ratio/delta tracking
+// and two integrator closed forms.
 // Parity unpinned (no reference vector or closed form fixes them):
-//   - the subsurface term of eval_diffuse (disney_bsdf.h:177-193);
+//   - the subsurface term of eval_diffuse (disney_bsdf.h:177-193) away from
+//     NoL = 1, where its Fresnel factor is not 1 (at NoL = 1 it is pinned);
 //   - the lobe-selection probabilities of mixed materials, disney_lobe_weights
 //     (disney_bsdf.h:148-175): only their self-consistency is checked;
 //   - transcendentals at the ulp level (the reference goes through Kokkos

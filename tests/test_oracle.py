@@ -196,6 +196,39 @@ def test_bsdf_lambert_special_case():
     np.testing.assert_allclose(pdf, nol / math.pi, rtol=2e-6, atol=1e-9)
 
 
+def test_bsdf_subsurface_at_normal_incidence():
+    # specular = metallic = transmission = sheen = clearcoat = 0, so the
+    # diffuse weight is 1 and no other lobe contributes.  At NoL = 1 the
+    # subsurface factor's Fresnel term is 1 (its (1 - NoL)^5 vanishes) and
+    # eval_diffuse (disney_bsdf.h:177-193) reduces to Lambert scaled by
+    # 1.25 / (1 + 0.001): subsurface = 1 gives the subsurface colour times that,
+    # subsurface = 0.5 the mean of Lambert in the base colour and the scaled
+    # lobe in the mean colour, whatever the outgoing direction.
+    rng = np.random.default_rng(11)
+    n = 300
+    wo, _, ng = _frame_tuple(n, rng)
+    wi = ng.copy()
+    k = 1.25 / 1.001 / math.pi
+    for s_amt in (1.0, 0.5):
+        mats = []
+        for _ in range(n):
+            m = abi.default_material()
+            m.base_color[:] = tuple(rng.uniform(0, 1, 3))
+            m.subsurface_color[:] = tuple(rng.uniform(0, 1, 3))
+            m.specular = 0.0
+            m.subsurface = s_amt
+            m.roughness = float(rng.uniform(0.01, 1))
+            mats.append(m)
+        f, _ = oracle.bsdf_eval(mats, wo, wi, ng, ng, use_double=True)
+        base = np.array([list(m.base_color) for m in mats], np.float64)
+        ssc = np.array([list(m.subsurface_color) for m in mats], np.float64)
+        if s_amt == 1.0:
+            want = ssc * k
+        else:
+            want = 0.5 * base / math.pi + 0.5 * (0.5 * (base + ssc)) * k
+        np.testing.assert_allclose(f, want, rtol=2e-6, atol=1e-9, err_msg=f"subsurface {s_amt}")
+
+
 def test_bsdf_matches_golden_closed_forms():
     # tests/golden/bsdf_closed_forms.json: Lambert and F = 1 metal GGX in double
     mats, wo, wi, ng, f_ref, pdf_ref = golden_bsdf_inputs()
