@@ -48,7 +48,9 @@ struct BinNode {
   uint32_t first = 0, count = 0;
 };
 
-constexpr int kBins = 16;
+// SAH bins per axis: on the 737K-triangle frame 16 bins gave 44.0 ms, 32
+// bins 43.2 ms (1.6 % fewer nodes, +0.08 s of build), 64 bins 43.3 ms.
+constexpr int kBins = 32;
 constexpr float kTriangleCost = 1.0f;
 
 struct Builder {
