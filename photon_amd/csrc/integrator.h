@@ -119,6 +119,7 @@ struct StreamQueues {
   uint32_t *counters;
   unsigned long long *stats;
   int *spill;        // per-thread traversal stack spill
+  int *spill_shadow; // the same for k_shadow, which runs beside k_extend
   // Per ray of the current queue, from k_volume (runs whose scenes hold
   // volumes): the volume event (scatter distance or -1, albedo) and the RNG
   // state after its draws.

@@ -305,7 +305,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_s
   if (count == 0) return;
   __shared__ int lds[kShortStack * B];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
+  LaneStack st{&lds[threadIdx.x], B, q.spill_shadow + size_t(gtid) * (kMaxStack - kShortStack), 0};
   uint32_t live = 0;
   traverse_stream<true>(
       st, &q.counters[fetch_set(kFetchShadow, it)], count, chunk, refill,

@@ -335,7 +335,10 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
     HIP_TRY(c, c->vol_ev.reserve(size_t(cap) * 16));
     HIP_TRY(c, c->vol_rng.reserve(size_t(cap) * 4));
   }
-  int r = ensure_trace_scratch(c, size_t(ls.trace_grid) * ls.trace_block);
+  // Two spill areas, one per traversal pass: k_shadow runs beside k_extend
+  // on the second stream, and a thread of each may spill at the same index.
+  const size_t trace_threads = size_t(ls.trace_grid) * ls.trace_block;
+  int r = ensure_trace_scratch(c, 2 * trace_threads);
   if (r) return r;
 
   // Item tables of the runs that have several items: first[] is padded to a
@@ -371,6 +374,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   q.counters = c->counters.as<uint32_t>();
   q.stats = c->stats_buf.as<unsigned long long>();
   q.spill = c->spill.as<int>();
+  q.spill_shadow = q.spill + trace_threads * (kMaxStack - kShortStack);
   q.vol_ev = ls.volumes ? c->vol_ev.as<float4>() : nullptr;
   q.vol_rng = ls.volumes ? c->vol_rng.as<uint32_t>() : nullptr;
 
