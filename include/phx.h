@@ -8,7 +8,7 @@
  * the library starts no host threads.  All kernels of a context run on the
  * context's HIP stream, or on the stream passed to a call that takes one; the
  * bounce loop of a render also runs its any-hit pass on a second stream of the
- * context, joined to the first before every shading launch.
+ * context, joined to the first before each pass that adds its results.
  *
  * Data types mirror the reference's scene model so a scene converted for the
  * reference converts 1:1 for this backend:

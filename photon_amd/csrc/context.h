@@ -79,7 +79,7 @@ struct phx_context {
   DBuf adj_gm, adj_gt;
 
   // wavefront storage
-  DBuf q_o[2], q_d[2], q_tp[2], q_st[2], q_hit, sh_o, sh_d, sh_c, sample_L, aov_s, counters, stats_buf, spill, fetch;
+  DBuf q_o[2], q_d[2], q_tp[2], q_st[2], q_hit, sh_o[2], sh_d[2], sh_c[2], sample_L, aov_s, counters, stats_buf, spill, fetch;
   DBuf vol_ev, vol_rng;  // k_volume's results, sized for the queue when a run's scenes hold volumes
   uint32_t q_capacity = 0;
   uint32_t trace_grid = 0;  // persistent grid of the seam-A trace kernels and the resolve pass

@@ -111,9 +111,11 @@ struct StreamQueues {
   float4 *tp[2];
   uint4 *st[2];
   float4 *hit;       // (t, b1, b2, leaf triangle bits)
-  float4 *sh_o;      // (origin, path_it bits); shadow rays start at t = 0
-  float4 *sh_d;      // (direction, tmax)
-  float4 *sh_c;      // (NEE contribution, sample id bits)
+  // Two shadow queues by iteration parity: the shading of iteration it fills
+  // queue (it + 1) & 1 while the any-hit pass of iteration it drains it & 1.
+  float4 *sh_o[2];   // (origin, path_it bits); shadow rays start at t = 0
+  float4 *sh_d[2];   // (direction, tmax)
+  float4 *sh_c[2];   // (NEE contribution, sample id bits)
   float4 *sample_L;  // per sample of the run: (radiance, alpha)
   uint32_t *aov_s;   // per pixel: first sample index with a camera-ray event
   uint32_t *counters;
@@ -171,7 +173,10 @@ hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead
 hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 // With ls.groups > 0 the rays of each work-group are shaded in material-group order.
 hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
+// The any-hit pass over the current iteration's shadow queue.
 hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
+// The unoccluded contributions of that queue into their samples' radiance.
+hipError_t launch_nee(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 hipError_t launch_volume(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 // Sums the item's samples of each pixel into accum (first: overwrite).
 hipError_t launch_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum, bool first,

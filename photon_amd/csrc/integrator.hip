@@ -273,9 +273,10 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
   }
   if (out.shadow) {
     // the shading path's iteration keys the transmittance stream
-    q.sh_o[si] = f4(out.so, __uint_as_float((out.next.path_it - 1u) | (out.next.item << 8)));
-    q.sh_d[si] = f4(out.sd, out.stmax);
-    q.sh_c[si] = f4(out.sc, __uint_as_float(g));
+    const uint32_t sb = (it + 1u) & 1u;
+    q.sh_o[sb][si] = f4(out.so, __uint_as_float((out.next.path_it - 1u) | (out.next.item << 8)));
+    q.sh_d[sb][si] = f4(out.sd, out.stmax);
+    q.sh_c[sb][si] = f4(out.sc, __uint_as_float(g));
   }
 }
 
@@ -295,14 +296,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Any hit over the shadow queue (trace_occluded; persistent waves, per-lane
 // refill; six waves per SIMD as k_extend).  It runs in the iteration after the
-// shade kernel that filled the queue, beside that iteration's k_extend: the
-// queue is the one counted for the current iteration.
+// shade kernel that filled the queue, beside that iteration's k_extend and
+// k_shade: the queue is the one counted for the current iteration, and that
+// iteration's shading fills the other one.
 template <class P, int B>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_shadow(P v, StreamQueues q,
                                                                                       uint32_t chunk, uint32_t refill) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[shadow_count_word(it)];
   if (count == 0) return;
+  const float4 *sh_o = q.sh_o[it & 1u], *sh_d = q.sh_d[it & 1u];
+  float4 *sh_c = q.sh_c[it & 1u];
   __shared__ int lds[kShortStack * B];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], B, q.spill_shadow + size_t(gtid) * (kMaxStack - kShortStack), 0};
@@ -310,7 +314,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_s
   traverse_stream<true>(
       st, &q.counters[fetch_set(kFetchShadow, it)], count, chunk, refill,
       [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const BvhNode *&nodes, const TriRec *&tris) {
-        const float4 a = q.sh_o[i], b = q.sh_d[i];
+        const float4 a = sh_o[i], b = sh_d[i];
         o = xyz(a);
         d = xyz(b);
         tmin = 0.f;
@@ -320,22 +324,22 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_s
       },
       [&](uint32_t i, bool occ, const TraceHit &) {
         // occluded entries lose their sample id; k_nee_accumulate adds the rest
-        if (occ) q.sh_c[i].w = __uint_as_float(kNoRay);
+        if (occ) sh_c[i].w = __uint_as_float(kNoRay);
       });
   block_sum<B / 64>(&q.stats[kStatShadow], live);
 }
 
 template <class P>
-__device__ __forceinline__ void nee_accumulate_one(const P &v, const StreamQueues &q, uint32_t i) {
-  const float4 c = q.sh_c[i];
+__device__ __forceinline__ void nee_accumulate_one(const P &v, const StreamQueues &q, uint32_t sb, uint32_t i) {
+  const float4 c = q.sh_c[sb][i];
   const uint32_t g = __float_as_uint(c.w);
   if (g == kNoRay) return;
   vec3 contrib = xyz(c);
-  const uint32_t key = __float_as_uint(reinterpret_cast<const float *>(&q.sh_o[i])[3]);  // path_it | item << 8
+  const uint32_t key = __float_as_uint(reinterpret_cast<const float *>(&q.sh_o[sb][i])[3]);  // path_it | item << 8
   const RunItem &r = v.item_at(key >> 8);
   const DevScene &s = v.scene_of(r);
   if (s.vol_count > 0) {
-    const float4 o = q.sh_o[i], d = q.sh_d[i];
+    const float4 o = q.sh_o[sb][i], d = q.sh_d[sb][i];
     const uint32_t l = g - r.first;
     Rng rng = make_rng(l % r.sp.npix, r.sample_base + l / r.sp.npix, 0x10000u + (key & 0xffu));
     contrib = contrib * vol_transmittance(s, xyz(o), xyz(d), 0.f, d.w, rng);
@@ -354,7 +358,8 @@ template <class P>
 __global__ __launch_bounds__(256) void k_nee_accumulate(P v, StreamQueues q) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[shadow_count_word(it)];
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) nee_accumulate_one(v, q, i);
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u)
+    nee_accumulate_one(v, q, it & 1u, i);
 }
 // Sums an item's samples of each pixel into the frame, in sample order.
 __global__ __launch_bounds__(256) void k_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum,
@@ -537,8 +542,12 @@ hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopSha
       hipLaunchKernelGGL((k_shadow<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk, ls.refill);
     else
       hipLaunchKernelGGL((k_shadow<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
+  });
+}
+hipError_t launch_nee(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st) {
+  return with_view(v, [&](const auto &p) {
+    using P = std::decay_t<decltype(p)>;
     hipLaunchKernelGGL(k_nee_accumulate<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q);
     return hipGetLastError();
   });

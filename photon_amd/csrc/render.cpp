@@ -196,16 +196,19 @@ struct Fork {
 };
 
 // Iterations of the bounce loop.  An iteration runs the any-hit pass over the
-// shadow queue that the previous iteration's shading filled (k_shadow, then
-// k_nee_accumulate) beside its own closest-hit pass, then shading and
-// regeneration.  With `fork` the any-hit pass runs on the second stream,
-// forked after the previous regeneration and joined before shading: the two
-// traversal passes share the GPU, so neither's tail of last slow rays leaves
-// it idle.  Shading must wait for the join, since it refills the shadow queue
-// and adds to the radiance of the samples k_nee_accumulate adds to, after it
-// as in the reference's order.  Without `fork` (the serial diagnostic) the
-// same launches run on `st` alone.  The shadow queue of the last iteration is
-// left for trace_pending_shadow.
+// shadow queue that the previous iteration's shading filled (k_shadow) beside
+// its own closest-hit pass and shading, then adds that queue's unoccluded
+// contributions (k_nee_accumulate) and regenerates.  With `fork` the any-hit
+// pass runs on the second stream, forked after the previous regeneration and
+// joined before k_nee_accumulate: it shares the GPU with k_extend and k_shade,
+// so no pass's tail of last slow rays leaves the GPU idle.  Shading fills the
+// other shadow queue of the pair (by iteration parity).  k_nee_accumulate
+// must follow both the any-hit pass and shading, since shading adds to the
+// same samples' radiance; a sample's sum thus takes the emission found by
+// bounce i before the direct light of bounce i - 1, and every schedule (serial
+// or forked, graph or not, batched or plain) keeps that order.  Without
+// `fork` (the serial diagnostic) the same launches run on `st` alone.  The
+// shadow queue of the last iteration is left for trace_pending_shadow.
 hipError_t launch_iterations(const ViewArgs &va, const StreamQueues &q, const LoopShape &ls, uint32_t n, Timer *tm,
                              hipStream_t st, const Fork *fork) {
   for (uint32_t j = 0; j < n; ++j) {
@@ -227,10 +230,13 @@ hipError_t launch_iterations(const ViewArgs &va, const StreamQueues &q, const Lo
       if ((e = launch_volume(va, q, ls, st)) != hipSuccess) return e;
       if (tm && (e = tm->end(st)) != hipSuccess) return e;
     }
-    if (fork && (e = hipStreamWaitEvent(st, fork->ev_join, 0)) != hipSuccess) return e;
     if (tm && (e = tm->begin(kTimedShade, st)) != hipSuccess) return e;
     if ((e = launch_shade(va, q, ls, st)) != hipSuccess) return e;
     if (tm && (e = tm->end(st)) != hipSuccess) return e;
+    // The shadow queue's contributions follow the shading of the next bounce
+    // into the samples' radiance; regeneration then reuses their slots.
+    if (fork && (e = hipStreamWaitEvent(st, fork->ev_join, 0)) != hipSuccess) return e;
+    if ((e = launch_nee(va, q, ls, st)) != hipSuccess) return e;
     if ((e = launch_regen(va, q, 1, ls, st)) != hipSuccess) return e;
   }
   return hipSuccess;
@@ -242,6 +248,7 @@ hipError_t trace_pending_shadow(const ViewArgs &va, const StreamQueues &q, const
   hipError_t e;
   if (tm && (e = tm->begin(kTimedShadow, st)) != hipSuccess) return e;
   if ((e = launch_shadow(va, q, ls, st)) != hipSuccess) return e;
+  if ((e = launch_nee(va, q, ls, st)) != hipSuccess) return e;
   return tm ? tm->end(st) : hipSuccess;
 }
 
@@ -313,9 +320,11 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
       HIP_TRY(c, c->q_st[b].reserve(size_t(cap) * 16));
     }
     HIP_TRY(c, c->q_hit.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->sh_o.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->sh_d.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->sh_c.reserve(size_t(cap) * 16));
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(c, c->sh_o[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, c->sh_d[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, c->sh_c[b].reserve(size_t(cap) * 16));
+    }
     c->q_capacity = cap;
   }
   HIP_TRY(c, c->sample_L.reserve(size_t(max_total) * 16));
@@ -366,9 +375,11 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
     q.st[b] = c->q_st[b].as<uint4>();
   }
   q.hit = c->q_hit.as<float4>();
-  q.sh_o = c->sh_o.as<float4>();
-  q.sh_d = c->sh_d.as<float4>();
-  q.sh_c = c->sh_c.as<float4>();
+  for (int b = 0; b < 2; ++b) {
+    q.sh_o[b] = c->sh_o[b].as<float4>();
+    q.sh_d[b] = c->sh_d[b].as<float4>();
+    q.sh_c[b] = c->sh_c[b].as<float4>();
+  }
   q.sample_L = c->sample_L.as<float4>();
   q.aov_s = aov;
   q.counters = c->counters.as<uint32_t>();
