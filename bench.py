@@ -359,7 +359,7 @@ def main():
         "kernel_ms_per_step": {"extend": ext_ms / args.steps, "shade": shd_ms / args.steps,
                                "shadow": sdw_ms / args.steps,
                                "note": "HIP-event spans on each launch's stream; the shadow pass runs on a second "
-                                       "stream beside the next extend launch, so its span includes waiting for CUs"},
+                                       "stream beside the next extend and shade launches, so the shade and shadow spans include waiting for CUs"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_baseline": gpu_base,

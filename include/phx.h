@@ -257,7 +257,7 @@ typedef struct phx_tuning {
   uint32_t trace_block;          /* work-group size of the traversal kernels: 128 or 256 (256) */
   uint32_t trace_groups_per_cu;  /* persistent traversal work-groups per CU, 1..32 (8) */
   uint32_t refill_lanes;         /* idle lanes of a traversal wave that make it take new rays, 1..64
-                                    (0: 12 when the run's deepest BVH has 6+ levels, else 20) */
+                                    (0: 12 when the run's deepest BVH has 6+ levels, else 40) */
   uint32_t queue_chunk;          /* rays a wave takes per queue fetch, 64..4096, multiple of 64 (128) */
   uint32_t paths_in_flight;      /* path slots of the wavefront (2^25) */
   uint32_t sort_by_material;     /* material groups of sorted shading, 0..15 (0: queue order) */
@@ -278,8 +278,8 @@ typedef struct phx_stats {
   double last_render_ms;
   /* With phx_tuning.time_kernels = 1: HIP-event time of every launch of the
    * closest-hit (extend), shade and shadow kernels of the last render.  The
-   * shadow pass runs beside the next extend launch, so the three sums can
-   * exceed the render time. */
+   * shadow pass runs beside the next extend and shade launches, so the three
+   * sums can exceed the render time. */
   uint64_t extend_launches;
   double extend_ms;
   uint64_t shade_launches;
