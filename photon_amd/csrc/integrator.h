@@ -173,7 +173,8 @@ hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead
 hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 // With ls.groups > 0 the rays of each work-group are shaded in material-group order.
 hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
-// The any-hit pass over the current iteration's shadow queue.
+// The any-hit pass over the current iteration's shadow queue, and (runs with
+// volumes) the transmittance along its unoccluded rays.
 hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 // The unoccluded contributions of that queue into their samples' radiance.
 hipError_t launch_nee(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);

@@ -329,37 +329,54 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_s
   block_sum<B / 64>(&q.stats[kStatShadow], live);
 }
 
+// The volume transmittance along the unoccluded shadow rays of the current
+// queue (the ratio tracking of pt_nee_accum_pack, pathtracer.cpp:1095-1121),
+// folded into their contributions, for runs whose scenes hold volumes.  It
+// follows k_shadow on the same stream, beside the next bounce's closest-hit
+// and shading passes, so this chain of dependent density lookups is off the
+// path that k_nee_accumulate's join puts after shading.  The draws come from
+// the shadow ray's own stream, so the result does not depend on where it runs.
 template <class P>
-__device__ __forceinline__ void nee_accumulate_one(const P &v, const StreamQueues &q, uint32_t sb, uint32_t i) {
-  const float4 c = q.sh_c[sb][i];
-  const uint32_t g = __float_as_uint(c.w);
-  if (g == kNoRay) return;
-  vec3 contrib = xyz(c);
-  const uint32_t key = __float_as_uint(reinterpret_cast<const float *>(&q.sh_o[sb][i])[3]);  // path_it | item << 8
-  const RunItem &r = v.item_at(key >> 8);
-  const DevScene &s = v.scene_of(r);
-  if (s.vol_count > 0) {
-    const float4 o = q.sh_o[sb][i], d = q.sh_d[sb][i];
+__global__ __launch_bounds__(256) void k_shadow_transmit(P v, StreamQueues q) {
+  const uint32_t it = q.counters[kIter];
+  const uint32_t count = q.counters[shadow_count_word(it)];
+  const uint32_t sb = it & 1u;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
+    const float4 c = q.sh_c[sb][i];
+    const uint32_t g = __float_as_uint(c.w);
+    if (g == kNoRay) continue;
+    const float4 o = q.sh_o[sb][i];
+    const uint32_t key = __float_as_uint(o.w);  // path_it | item << 8
+    const RunItem &r = v.item_at(key >> 8);
+    const DevScene &s = v.scene_of(r);
+    if (s.vol_count == 0) continue;
+    const float4 d = q.sh_d[sb][i];
     const uint32_t l = g - r.first;
     Rng rng = make_rng(l % r.sp.npix, r.sample_base + l / r.sp.npix, 0x10000u + (key & 0xffu));
-    contrib = contrib * vol_transmittance(s, xyz(o), xyz(d), 0.f, d.w, rng);
-  }
-  if (finite3(contrib)) {
-    const float4 L = q.sample_L[g];
-    q.sample_L[g] = make_float4(L.x + contrib.x, L.y + contrib.y, L.z + contrib.z, L.w);
+    const vec3 contrib = xyz(c) * vol_transmittance(s, xyz(o), xyz(d), 0.f, d.w, rng);
+    q.sh_c[sb][i] = make_float4(contrib.x, contrib.y, contrib.z, c.w);
   }
 }
 
 // pt_nee_accum_pack (pathtracer.cpp:1095-1121): the unoccluded shadow rays'
-// contributions, times the volume transmittance along them, into their
-// samples' radiance.  One thread per shadow ray, over the queue in strides of
-// the grid; after k_shadow, on the same queue.
+// contributions (with their volume transmittance from k_shadow_transmit) into
+// their samples' radiance.  One thread per shadow ray, over the queue in
+// strides of the grid; after k_shadow and k_shade, on the same queue.
 template <class P>
-__global__ __launch_bounds__(256) void k_nee_accumulate(P v, StreamQueues q) {
+__global__ __launch_bounds__(256) void k_nee_accumulate(P, StreamQueues q) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[shadow_count_word(it)];
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u)
-    nee_accumulate_one(v, q, it & 1u, i);
+  const uint32_t sb = it & 1u;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
+    const float4 c = q.sh_c[sb][i];
+    const uint32_t g = __float_as_uint(c.w);
+    if (g == kNoRay) continue;
+    const vec3 contrib = xyz(c);
+    if (finite3(contrib)) {
+      const float4 L = q.sample_L[g];
+      q.sample_L[g] = make_float4(L.x + contrib.x, L.y + contrib.y, L.z + contrib.z, L.w);
+    }
+  }
 }
 // Sums an item's samples of each pixel into the frame, in sample order.
 __global__ __launch_bounds__(256) void k_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum,
@@ -542,6 +559,9 @@ hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopSha
       hipLaunchKernelGGL((k_shadow<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, ls.chunk, ls.refill);
     else
       hipLaunchKernelGGL((k_shadow<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, ls.chunk, ls.refill);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !ls.volumes) return e;
+    hipLaunchKernelGGL(k_shadow_transmit<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q);
     return hipGetLastError();
   });
 }
