@@ -178,3 +178,37 @@ def test_batch_with_a_volume_scene_equals_plain_renders(use_graph):
             br.render(views)
             for v in range(len(views)):
                 _check_equal(br.read(v), refs[v], f"{tag} graph={use_graph} rep={rep} view {v}")
+
+
+def _overlapping_soup(n, seed):
+    # Large, heavily overlapping triangles: most children of a node are entered,
+    # so traversal stacks run past the LDS part into the spill area; about one
+    # triangle in twelve emits, so the any-hit pass traces as deep.
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-1, 1, (n, 1, 3))
+    pos = (centers + rng.normal(scale=0.6, size=(n, 3, 3))).reshape(-1, 3)
+    mids = (rng.random(n) < 1 / 12).astype(np.uint32)
+    return scenes.Scene(pos, np.arange(3 * n, dtype=np.uint32).reshape(-1, 3),
+                        [scenes.diffuse((0.6, 0.5, 0.4)), scenes.emitter((4.0, 4.0, 4.0), front_only=False)],
+                        material_ids=mids)
+
+
+def test_forked_bounce_loop_equals_serial(monkeypatch):
+    # The any-hit pass runs on a second stream beside the closest-hit and
+    # shading passes; on one stream (PHX_SERIAL_BOUNCE=1) the same launches run
+    # in turn.  Both traversal passes spill deep stacks here, each to its own
+    # area, and the images must agree bit for bit.
+    from photon_amd.backend import Context, HipBackend, PathTracer, make_camera
+    s = _overlapping_soup(6000, seed=11)
+    pt = PathTracer(HipBackend(context=Context(0)))
+    pt.set_scene(s)
+    pt.camera = make_camera((0.3, 0.2, 4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, 1.0)
+    pt.params = abi.default_render_params(96, 96, 8, 8)
+    forked = pt.render()
+    assert pt.ctx.stats().shadow_rays > 0
+    monkeypatch.setenv("PHX_SERIAL_BOUNCE", "1")
+    serial = pt.render()
+    monkeypatch.delenv("PHX_SERIAL_BOUNCE")
+    _check_equal(forked, serial, "forked vs serial")
+    assert float(np.abs(forked.color).sum()) > 0.0
+    _check_equal(pt.render(), forked, "forked again")
