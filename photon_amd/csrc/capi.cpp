@@ -413,6 +413,7 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
 
   // Volumes.
   s.vol_count = 0;
+  s.vol_tf_count = 0;
   if (d->volume_count && d->volumes) {
     for (uint32_t i = 0; i < d->volume_count; ++i) {
       const phx_volume &v = d->volumes[i];
@@ -428,6 +429,7 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
     if ((r = upload(c, c->vol_tf, reinterpret_cast<const float4 *>(d->volume_tf), d->volume_tf_count))) return r;
     if ((r = upload(c, c->majorants, d->volume_majorants, d->volume_majorant_count))) return r;
     s.vol_count = d->volume_count;
+    s.vol_tf_count = d->volume_tf_count;
   }
   s.volumes = c->volumes.as<phx_volume>();
   s.vol_values = c->vol_values.as<float>();

@@ -98,6 +98,7 @@ struct DevScene {
   uint32_t env_w, env_h;
   uint32_t tex_count;
   uint32_t vol_count;
+  uint32_t vol_tf_count;  // float4 entries of vol_tf, all volumes together
   uint32_t strategy_count;
   uint32_t triangle_count;
   int32_t has_env, env_visible;

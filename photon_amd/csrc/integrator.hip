@@ -69,8 +69,30 @@ __global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahe
 // eight it spills and was slower) against k_shade's four, so more of that
 // latency is hidden, and k_shade no longer carries the loop.  k_shade picks
 // the event and the RNG state up.  C4 frame 74.1 -> 60.9 ms.
+// Transfer-function entries the tracking kernels stage in LDS (one 256-entry
+// table).  In a run of one scene whose tables fit, the table lookup that
+// follows each density gather is an LDS read instead of a second dependent
+// trip to the caches; the values, and so the results, are the same.  Every
+// thread of the work-group calls this.
+constexpr uint32_t kTfLds = 256;
 template <class P>
-__global__ __launch_bounds__(256) void k_volume(P v, StreamQueues q) {
+__device__ __forceinline__ P with_tf_in_lds(const P &v, float4 *s_tf) {
+  P w = v;
+  if constexpr (std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value) {
+    const uint32_t n = v.scene.vol_tf_count;
+    if (n > 0 && n <= kTfLds) {
+      for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) s_tf[j] = v.scene.vol_tf[j];
+      __syncthreads();
+      w.scene.vol_tf = s_tf;
+    }
+  }
+  return w;
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void k_volume(P v0, StreamQueues q) {
+  __shared__ float4 s_tf[kTfLds];
+  const P v = with_tf_in_lds(v0, s_tf);
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[ray_count_word(it)];
   const int buf = it & 1;
@@ -337,7 +359,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_s
 // path that k_nee_accumulate's join puts after shading.  The draws come from
 // the shadow ray's own stream, so the result does not depend on where it runs.
 template <class P>
-__global__ __launch_bounds__(256) void k_shadow_transmit(P v, StreamQueues q) {
+__global__ __launch_bounds__(256) void k_shadow_transmit(P v0, StreamQueues q) {
+  __shared__ float4 s_tf[kTfLds];
+  const P v = with_tf_in_lds(v0, s_tf);
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[shadow_count_word(it)];
   const uint32_t sb = it & 1u;
