@@ -61,14 +61,6 @@ __global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahe
   }
 }
 
-// The volume part of pt_shade (pathtracer.cpp:789-857), ahead of k_shade:
-// the free-flight sample through the scene's volumes in front of each ray's
-// hit, made with the shading event's own RNG stream (its first draws).  The
-// delta-tracking loop is a chain of dependent density lookups; here it runs
-// in a small kernel at six waves per SIMD (81 registers, no spills; capped at
-// eight it spills and was slower) against k_shade's four, so more of that
-// latency is hidden, and k_shade no longer carries the loop.  k_shade picks
-// the event and the RNG state up.  C4 frame 74.1 -> 60.9 ms.
 // Transfer-function entries the tracking kernels stage in LDS (one 256-entry
 // table).  In a run of one scene whose tables fit, the table lookup that
 // follows each density gather is an LDS read instead of a second dependent
@@ -89,9 +81,20 @@ __device__ __forceinline__ P with_tf_in_lds(const P &v, float4 *s_tf) {
   return w;
 }
 
+// The volume part of pt_shade (pathtracer.cpp:789-857), ahead of k_shade:
+// the free-flight sample through the scene's volumes in front of each ray's
+// hit, made with the shading event's own RNG stream (its first draws).  The
+// delta-tracking loop is a chain of dependent density lookups; here it runs
+// in a small kernel at six waves per SIMD (80 registers and a 12-byte spill
+// under that cap; uncapped it takes 81 registers and five waves, and the C4
+// frame is 57.2 instead of 54.6 ms; capped at eight it spills more and was
+// slower) against k_shade's four, so more of that latency is hidden, and
+// k_shade no longer carries the loop.  k_shade picks the event and the RNG
+// state up.  C4 frame 74.1 -> 60.9 ms when it was split out.  Batches with
+// per-lane scenes (ItemTable<false>) would spill 148 bytes under the cap and
+// use k_volume_wide.
 template <class P>
-__global__ __launch_bounds__(256) void k_volume(P v0, StreamQueues q) {
-  __shared__ float4 s_tf[kTfLds];
+__device__ __forceinline__ void volume_events(const P &v0, const StreamQueues &q, float4 *s_tf) {
   const P v = with_tf_in_lds(v0, s_tf);
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[ray_count_word(it)];
@@ -110,6 +113,16 @@ __global__ __launch_bounds__(256) void k_volume(P v0, StreamQueues q) {
     q.vol_ev[i] = make_float4(ts, alb.x, alb.y, alb.z);
     q.vol_rng[i] = rng.s;
   }
+}
+template <class P>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_volume(P v, StreamQueues q) {
+  __shared__ float4 s_tf[kTfLds];
+  volume_events(v, q, s_tf);
+}
+template <class P>
+__global__ __launch_bounds__(256) void k_volume_wide(P v, StreamQueues q) {
+  __shared__ float4 s_tf[kTfLds];
+  volume_events(v, q, s_tf);
 }
 
 // The emitter records of the emissive CDF (em_record), rebuilt whenever the
@@ -599,7 +612,10 @@ hipError_t launch_nee(const ViewArgs &v, const StreamQueues &q, const LoopShape 
 hipError_t launch_volume(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st) {
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
-    hipLaunchKernelGGL(k_volume<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q);
+    if constexpr (std::is_same<P, ItemTable<false>>::value)
+      hipLaunchKernelGGL(k_volume_wide<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q);
+    else
+      hipLaunchKernelGGL(k_volume<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q);
     return hipGetLastError();
   });
 }
