@@ -86,13 +86,13 @@ __device__ __forceinline__ P with_tf_in_lds(const P &v, float4 *s_tf) {
 // hit, made with the shading event's own RNG stream (its first draws).  The
 // delta-tracking loop is a chain of dependent density lookups; here it runs
 // in a small kernel at six waves per SIMD (80 registers and a 12-byte spill
-// under that cap; uncapped it takes 81 registers and five waves, and the C4
-// frame is 57.2 instead of 54.6 ms; capped at eight it spills more and was
-// slower) against k_shade's four, so more of that latency is hidden, and
-// k_shade no longer carries the loop.  k_shade picks the event and the RNG
-// state up.  C4 frame 74.1 -> 60.9 ms when it was split out.  Batches with
-// per-lane scenes (ItemTable<false>) would spill 148 bytes under the cap and
-// use k_volume_wide.
+// under that cap; uncapped it takes 81 registers and five waves, and the
+// 256^3 fog-box frame is 57.2 instead of 54.6 ms; capped at eight it spills
+// more and was slower) against k_shade's four, so more of that latency is
+// hidden, and k_shade no longer carries the loop.  k_shade picks the event
+// and the RNG state up.  The fog-box frame went from 74.1 to 60.9 ms when
+// the loop was split out.  Batches with per-lane scenes (ItemTable<false>)
+// would spill 148 bytes under the cap and use k_volume_wide.
 template <class P>
 __device__ __forceinline__ void volume_events(const P &v0, const StreamQueues &q, float4 *s_tf) {
   const P v = with_tf_in_lds(v0, s_tf);
