@@ -10,17 +10,25 @@
  * bounce loop of a render also runs its any-hit pass on a second stream of the
  * context, joined to the first before each pass that adds its results.
  *
- * Data types mirror the reference's scene model so a scene converted for the
- * reference converts 1:1 for this backend:
- *   phx_material   <- pt::Material       (include/photon/pt/material.h:35-82)
- *   phx_light      <- pt::Light          (include/photon/pt/light.h:19-35)
- *   phx_volume     <- pt::TfVolume       (include/photon/pt/tf_volume.h:14-33)
- *   phx_texture_info <- pt::TextureInfo  (include/photon/pt/texture.h:11-17)
- *   phx_camera     <- pt::Camera         (include/photon/pt/camera.h:11-26)
- *   phx_render_params <- pt::RenderParams (include/photon/pt/pathtracer.h:16-55)
- *   phx_hit        <- pt::HitResult      (include/photon/pt/backend/ray_backend.h:15-29)
- * Per-parameter attribute/sampler sources of the ANARI device (material.h:16-19,
- * ParamSource) are not part of this ABI; texture-atlas ids are.
+ * Data types mirror the reference's scene model.  These have the reference
+ * type's exact size, field order and field offsets (x86-64 and gfx950 agree),
+ * so an array of the reference type can be handed over as is:
+ *   phx_material   == pt::Material       (include/photon/pt/material.h:16-19,35-82)
+ *   phx_light      == pt::Light          (include/photon/pt/light.h:19-35)
+ *   phx_volume     == pt::TfVolume       (include/photon/pt/tf_volume.h:14-33)
+ *   phx_texture_info == pt::TextureInfo  (include/photon/pt/texture.h:11-17)
+ *   phx_sampler_info == pt::SamplerInfo  (include/photon/pt/sampler.h:39-52)
+ *   phx_camera     == pt::Camera         (include/photon/pt/camera.h:11-26)
+ *   phx_hit        == pt::HitResult      (include/photon/pt/backend/ray_backend.h:15-29)
+ *   phx_bridge_material == bridge::MaterialData (src/photon/anari/RenderBridge.h:21-56)
+ *   phx_light_data == bridge::LightData   (RenderBridge.h:73-85)
+ *   phx_camera_data == bridge::CameraData (RenderBridge.h:128-144)
+ *   phx_render_settings == bridge::RenderSettings (RenderBridge.h:146-170)
+ * The C++ bool members of the reference types are uint8_t here.
+ * phx_render_params follows pt::RenderParams (pathtracer.h:16-55) field by
+ * field, with the background image as a device pointer and its size instead
+ * of a Kokkos::View; the bridge's std::vector-holding types (SceneData,
+ * SamplerData, EnvironmentData, VolumeData) become pointer + count structs.
  */
 #ifndef PHX_H
 #define PHX_H
@@ -32,7 +40,7 @@
 extern "C" {
 #endif
 
-#define PHX_ABI_VERSION 2
+#define PHX_ABI_VERSION 3
 
 enum {
   PHX_OK = 0,
@@ -63,7 +71,27 @@ enum {
 #define PHX_LIGHT_SPOT 2
 #define PHX_LIGHT_AREA 3
 
-/* pt::Material (material.h:35-82) without the ParamSource members. 36 words. */
+/* ParamSourceKind (material.h:14). */
+#define PHX_SOURCE_CONSTANT 0
+#define PHX_SOURCE_ATTRIBUTE 1
+#define PHX_SOURCE_SAMPLER 2
+
+/* Attribute slots (sampler.h:11-19): 0-3 and the colour are per-vertex
+ * attributes of the mesh; the world position and normal of the surface point
+ * are computed. */
+#define PHX_ATTRIBUTE_0 0
+#define PHX_ATTRIBUTE_COLOR 4
+#define PHX_ATTRIBUTE_MESH_COUNT 5
+#define PHX_ATTRIBUTE_WORLD_POSITION 5
+#define PHX_ATTRIBUTE_WORLD_NORMAL 6
+
+/* pt::ParamSource (material.h:16-19): where a material parameter comes from. */
+typedef struct phx_param_source {
+  int32_t kind;  /* PHX_SOURCE_* */
+  int32_t index; /* attribute slot or sampler id */
+} phx_param_source;
+
+/* pt::Material (material.h:35-82). 200 bytes. */
 typedef struct phx_material {
   float base_color[3];
   float metallic;
@@ -77,26 +105,34 @@ typedef struct phx_material {
   float sheen_tint;
   float specular_tint;
   float anisotropic;
+  float emission[3];
   float emission_strength;
   float subsurface;
-  float emission[3];
-  float subsurface_radius;
   float subsurface_color[3];
+  float subsurface_radius;
   float alpha;
   int32_t thin;
-  int32_t flags;
-  int32_t alpha_mode;
-  float alpha_cutoff;
   int32_t base_color_tex;
   int32_t normal_tex;
   int32_t roughness_tex;
   int32_t metallic_tex;
   int32_t emission_tex;
   int32_t alpha_tex;
-  int32_t pad_[2];
+  phx_param_source base_color_src;
+  phx_param_source alpha_src;
+  phx_param_source metallic_src;
+  phx_param_source roughness_src;
+  phx_param_source emission_src;
+  phx_param_source transmission_src;
+  phx_param_source normal_src; /* tangent-space normal map: the value c encodes 2c - 1 */
+  int32_t alpha_mode;          /* PHX_ALPHA_* */
+  float alpha_cutoff;
+  int32_t flags;               /* PHX_MATERIAL_FLAG_* */
+  int32_t base_color_alpha;    /* 1: the base colour source's alpha multiplies alpha */
+  int32_t roughness_src_squared; /* 1: a sourced roughness is perceptual and is squared */
 } phx_material;
 
-/* pt::Light (light.h:19-35). 24 words. */
+/* pt::Light (light.h:19-35). 92 bytes. */
 typedef struct phx_light {
   uint32_t type;
   float position[3];
@@ -111,8 +147,32 @@ typedef struct phx_light {
   float edge1[3];
   float edge2[3];
   float radius;
-  uint32_t pad_;
 } phx_light;
+
+/* SamplerType and WrapMode (sampler.h:36-37). */
+#define PHX_SAMPLER_IMAGE1D 0
+#define PHX_SAMPLER_IMAGE2D 1
+#define PHX_SAMPLER_IMAGE3D 2
+#define PHX_SAMPLER_TRANSFORM 3
+#define PHX_WRAP_CLAMP 0
+#define PHX_WRAP_REPEAT 1
+#define PHX_WRAP_MIRROR 2
+
+/* pt::SamplerInfo (sampler.h:39-52): an image (1-3D, RGBA texels in the
+ * scene's sampler texel array, row by row from coordinate 0) or a transform,
+ * looked up with an attribute of the surface point.  200 bytes. */
+typedef struct phx_sampler_info {
+  uint32_t type;         /* PHX_SAMPLER_* */
+  int32_t in_attribute;  /* PHX_ATTRIBUTE_* slot */
+  uint32_t linear;
+  uint32_t wrap[3];      /* PHX_WRAP_* */
+  uint32_t offset;       /* first texel */
+  uint32_t size[3];
+  float in_transform[16];  /* row-major 4x4 */
+  float in_offset[4];
+  float out_transform[16];
+  float out_offset[4];
+} phx_sampler_info;
 
 /* pt::TextureInfo (texture.h:11-17): texels are RGB floats in one atlas. */
 typedef struct phx_texture_info {
@@ -122,7 +182,7 @@ typedef struct phx_texture_info {
   uint32_t pad_;
 } phx_texture_info;
 
-/* pt::TfVolume (tf_volume.h:14-33). 36 words. */
+/* pt::TfVolume (tf_volume.h:14-33). 132 bytes. */
 typedef struct phx_volume {
   float world_to_local[12]; /* row-major 3x4 */
   float origin[3];
@@ -138,7 +198,6 @@ typedef struct phx_volume {
   uint32_t linear;
   uint32_t macro_offset;
   uint32_t macro_dims[3];
-  uint32_t pad_[3];
 } phx_volume;
 
 /* Everything pt::Scene (scene.h:41-72) holds, as host arrays. */
@@ -169,6 +228,17 @@ typedef struct phx_scene_desc {
   const phx_texture_info *texture_infos;
   uint32_t texture_count;
 
+  /* Per-vertex attributes (MeshAttributes, sampler.h:25-34): RGBA records,
+   * slot k's records start at attribute_offset[k] (-1: the slot is absent). */
+  const float *attributes;
+  uint32_t attribute_count; /* records */
+  int32_t attribute_offset[PHX_ATTRIBUTE_MESH_COUNT];
+  /* Samplers (SamplerTable, sampler.h:56-139) and their RGBA texels. */
+  const phx_sampler_info *samplers;
+  uint32_t sampler_count;
+  const float *sampler_texels;
+  uint32_t sampler_texel_count; /* texels */
+
   /* Equirect environment (environment_map.h:13-25): row 0 is +Y of the map. */
   uint32_t env_width;
   uint32_t env_height;
@@ -188,7 +258,7 @@ typedef struct phx_scene_desc {
   float ray_epsilon; /* 0: derived from the bounds (pathtracer.cpp:489-502) */
 } phx_scene_desc;
 
-/* pt::Camera (camera.h:11-26) in its derived form. */
+/* pt::Camera (camera.h:11-26) in its derived form. 104 bytes. */
 typedef struct phx_camera {
   float origin[3];
   float lower_left[3];
@@ -197,12 +267,15 @@ typedef struct phx_camera {
   float u[3], v[3], w[3];
   float lens_radius;
   float focus_dist;
-  int32_t is_ortho;
+  uint8_t is_ortho;
+  uint8_t pad_[3];
   float clip_near;
   float clip_far;
 } phx_camera;
 
-/* pt::RenderParams (pathtracer.h:16-55) without the background image. */
+/* pt::RenderParams (pathtracer.h:16-55).  The background image is a device
+ * array of background_height rows of background_width RGBA floats, rows from
+ * the bottom of the frame (NULL: the background colour). */
 typedef struct phx_render_params {
   uint32_t width;
   uint32_t height;
@@ -223,9 +296,12 @@ typedef struct phx_render_params {
   float clamp_indirect_sum;
   int32_t bottom_up;
   float image_region[4];
+  const float *background_image;
+  uint32_t background_width;
+  uint32_t background_height;
 } phx_render_params;
 
-/* pt::HitResult (ray_backend.h:15-29). 24 words. */
+/* pt::HitResult (ray_backend.h:15-29). 88 bytes. */
 typedef struct phx_hit {
   float t;
   float position[3];
@@ -238,9 +314,9 @@ typedef struct phx_hit {
   uint32_t geom_id;
   uint32_t inst_id;
   uint32_t material_id;
-  uint32_t hit;
-  uint32_t has_interpolated_color;
-  uint32_t pad_;
+  uint8_t hit;
+  uint8_t has_interpolated_color;
+  uint8_t pad_[2];
 } phx_hit;
 
 /* Compact closest-hit record used by the integrator: t, primitive, (b1, b2). */
@@ -436,11 +512,14 @@ int phx_volume_majorants(phx_volume *vol, const float *values, const float *tf_r
  * Progressive accumulation over frames (RenderBridge.cpp:828-928): `reset`
  * drops the accumulated samples; sample_offset advances by the frame's spp.
  * read_* give the accumulated frame, rows starting at the bottom. */
+
+/* bridge::CameraData (RenderBridge.h:128-144). 84 bytes. */
 typedef struct phx_camera_data {
   float position[3];
   float direction[3];
   float up[3];
-  int32_t orthographic;
+  uint8_t orthographic;
+  uint8_t pad_[3];
   float fovy; /* radians */
   float height;
   float aspect;
@@ -451,15 +530,22 @@ typedef struct phx_camera_data {
   float clip_far;
 } phx_camera_data;
 
+/* bridge::RenderSettings (RenderBridge.h:146-170). 96 bytes.  The background
+ * image is host memory: background_height rows of background_width RGBA
+ * floats, rows from the bottom (NULL: the background colour). */
 typedef struct phx_render_settings {
   uint32_t width;
   uint32_t height;
   uint32_t pixel_samples;
   uint32_t max_depth;
   float background[4];
+  const float *background_image;
+  uint32_t background_width;
+  uint32_t background_height;
   float ambient_color[3];
   float ambient_radiance;
-  int32_t adaptive_sampling;
+  uint8_t adaptive_sampling;
+  uint8_t pad_[3];
   float throughput_clamp;
   int32_t max_diffuse_bounces;
   int32_t max_glossy_bounces;
@@ -472,8 +558,8 @@ typedef struct phx_render_settings {
 int phx_bridge_render(phx_context *ctx, const phx_camera_data *camera, const phx_render_settings *settings,
                       int reset);
 
-/* bridge::MaterialData (RenderBridge.h:27-56).  Each parameter source is
- * given by its ParamSourceData kind: 0 constant, 1 attribute, 2 sampler. */
+/* bridge::MaterialData (RenderBridge.h:21-56). 132 bytes.  Sources are
+ * ParamSourceData {kind: 0 constant, 1 vertex attribute, 2 sampler; index}. */
 typedef struct phx_bridge_material {
   float base_color[3];
   float opacity;
@@ -486,32 +572,124 @@ typedef struct phx_bridge_material {
   float clearcoat;
   float clearcoat_roughness;
   float sheen;
-  int32_t base_color_src;
-  int32_t opacity_src;
-  int32_t metallic_src;
-  int32_t roughness_src;
-  int32_t emission_src;
-  int32_t transmission_src;
-  int32_t normal_src;
+  phx_param_source base_color_src;
+  phx_param_source opacity_src;
+  phx_param_source metallic_src;
+  phx_param_source roughness_src;
+  phx_param_source emission_src;
+  phx_param_source transmission_src;
+  phx_param_source normal_src; /* tangent-space normal map, encoded as 0.5 + 0.5 n */
   int32_t alpha_mode; /* 0 blend, 1 opaque, 2 mask */
   float alpha_cutoff;
-  int32_t base_color_alpha;
-  int32_t thin_walled;
-  int32_t emitter_only;
-  int32_t hidden_to_camera;
+  uint8_t base_color_alpha;
+  uint8_t thin_walled;
+  uint8_t emitter_only;
+  uint8_t hidden_to_camera;
   int32_t emission_side; /* 0 front, 1 back, 2 both */
 } phx_bridge_material;
 
 /* convert_material (RenderBridge.cpp:324-370): the pt-level material the
  * bridge renders a MaterialData with.  Roughness is squared (the BSDF takes
  * the GGX alpha), specular halved (0.5 is the full dielectric reflectance),
- * emission_strength is 1, alpha the opacity, the alpha mode recoded to
- * PHX_ALPHA_*, REFRACTIVE always set and the emitter-only, hidden-to-camera
- * and emission-side flags added.  Fields the bridge leaves alone keep
- * pt::Material's defaults (material.h:35-82).  Only constant sources are
- * supported: a material with an attribute or sampler source gives
- * PHX_ERR_UNSUPPORTED and leaves *out unchanged. */
+ * emission_strength is 1, alpha the opacity (1 when the opacity has a
+ * source, which then scales it), the sources copied with
+ * roughness_src_squared = 1, the alpha mode recoded to PHX_ALPHA_*,
+ * REFRACTIVE always set and the emitter-only, hidden-to-camera and
+ * emission-side flags added.  Fields the bridge leaves alone keep
+ * pt::Material's defaults (material.h:35-82). */
 int phx_convert_material(const phx_bridge_material *in, phx_material *out);
+
+/* bridge::LightData (RenderBridge.h:73-85). 56 bytes. */
+typedef struct phx_light_data {
+  int32_t type; /* 0 point, 1 directional, 2 spot */
+  float position[3];
+  float direction[3];
+  float color[3];
+  float intensity;
+  float radius;
+  float spot_angle;
+  float spot_falloff;
+} phx_light_data;
+
+/* bridge::SamplerData (RenderBridge.h:58-71); the texel vector as pointer +
+ * count (floats: 4 per RGBA texel). */
+typedef struct phx_sampler_data {
+  int32_t type; /* 0 image1D, 1 image2D, 2 image3D, 3 transform */
+  int32_t in_attribute;
+  uint8_t linear;
+  uint8_t pad_[3];
+  int32_t wrap[3]; /* 0 clampToEdge, 1 repeat, 2 mirrorRepeat */
+  uint32_t size[3];
+  float in_transform[16];
+  float in_offset[4];
+  float out_transform[16];
+  float out_offset[4];
+  const float *texels;
+  uint64_t texel_float_count;
+} phx_sampler_data;
+
+/* bridge::EnvironmentData (RenderBridge.h:87-97): RGB radiance rows, row 0
+ * the bottom (-up) row, as pointer + count (floats). */
+typedef struct phx_environment_data {
+  uint8_t present;
+  uint8_t visible;
+  uint8_t pad_[2];
+  uint32_t width;
+  uint32_t height;
+  const float *pixels;
+  uint64_t pixel_float_count;
+  float world_to_map[9]; /* world -> map rotation, row-major 3x3 */
+} phx_environment_data;
+
+/* bridge::VolumeData (RenderBridge.h:99-110), vectors as pointer + count. */
+typedef struct phx_volume_data {
+  float world_to_local[12];
+  float origin[3];
+  float spacing[3];
+  uint32_t dims[3];
+  const float *values; /* dims[0] * dims[1] * dims[2], x fastest */
+  uint64_t value_count;
+  const float *transfer; /* RGBA table spanning value_range, A the opacity */
+  uint64_t transfer_float_count;
+  float value_range[2];
+  float density_scale;
+  uint8_t linear;
+  uint8_t pad_[3];
+} phx_volume_data;
+
+/* bridge::SceneData (RenderBridge.h:112-125): every std::vector as pointer +
+ * element count (float counts for float vectors), so a caller passes
+ * v.data(), v.size(). */
+typedef struct phx_scene_data {
+  const float *positions;
+  uint64_t position_count; /* floats, 3 per vertex */
+  const float *normals;
+  uint64_t normal_count; /* floats; used when >= 3 per vertex */
+  const float *attributes[PHX_ATTRIBUTE_MESH_COUNT];
+  uint64_t attribute_count[PHX_ATTRIBUTE_MESH_COUNT]; /* floats; a slot is used when >= 4 per vertex */
+  const uint32_t *indices;
+  uint64_t index_count;
+  const uint32_t *material_ids;
+  uint64_t material_id_count;
+  const phx_bridge_material *materials;
+  uint32_t material_count;
+  const phx_sampler_data *samplers;
+  uint32_t sampler_count;
+  const phx_light_data *lights;
+  uint32_t light_count;
+  phx_environment_data environment;
+  const phx_volume_data *volumes;
+  uint32_t volume_count;
+} phx_scene_data;
+
+/* Renderer::set_scene (RenderBridge.h:189) via Impl::build_scene
+ * (RenderBridge.cpp:441-826): converts SceneData as the bridge does
+ * (materials by convert_material, samplers, lights, the environment map,
+ * volumes with macro-cell majorants, the emissive-triangle list, ray_epsilon
+ * from the bounds; an empty mesh becomes one degenerate triangle) and sets the
+ * result as the context's scene (phx_set_scene).  Also resets the
+ * accumulated frame of seam B. */
+int phx_set_scene_data(phx_context *ctx, const phx_scene_data *scene);
 uint32_t phx_bridge_accumulated_samples(const phx_context *ctx);
 int phx_bridge_read_color(phx_context *ctx, float *rgba);
 int phx_bridge_read_depth(phx_context *ctx, float *depth);

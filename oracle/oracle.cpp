@@ -243,6 +243,11 @@ struct Scene {
   EnvMap env;
   bool has_env{false}, env_visible{true};
   Atlas atlas;
+  // MeshAttributes and SamplerTable (sampler.h:25-139)
+  std::vector<float> attrs;  // RGBA records
+  int attr_off[PHX_ATTRIBUTE_MESH_COUNT]{-1, -1, -1, -1, -1};
+  std::vector<phx_sampler_info> samplers;
+  std::vector<float> sampler_texels;  // RGBA
   Volumes vol;
   float ray_eps{1e-3f};
   bool brute{false};  // trace by brute force instead of the BVH
@@ -334,6 +339,12 @@ static void scene_load(Scene &s, const phx_scene_desc &d) {
     s.atlas.pixels.assign(d.texture_pixels, d.texture_pixels + size_t(d.texture_pixel_count) * 3);
     s.atlas.infos.assign(d.texture_infos, d.texture_infos + d.texture_count);
   }
+  for (int k = 0; k < PHX_ATTRIBUTE_MESH_COUNT; ++k) s.attr_off[k] = d.attributes ? d.attribute_offset[k] : -1;
+  if (d.attributes) s.attrs.assign(d.attributes, d.attributes + size_t(d.attribute_count) * 4);
+  if (d.sampler_count && d.samplers) {
+    s.samplers.assign(d.samplers, d.samplers + d.sampler_count);
+    if (d.sampler_texels) s.sampler_texels.assign(d.sampler_texels, d.sampler_texels + size_t(d.sampler_texel_count) * 4);
+  }
   if (d.volume_count) {
     s.vol.vols.assign(d.volumes, d.volumes + d.volume_count);
     s.vol.values.assign(d.volume_values, d.volume_values + d.volume_value_count);
@@ -400,18 +411,187 @@ static bool occluded(const Scene &s, const V3<float> &o, const V3<float> &d, flo
   return ch.hit;
 }
 
-// pathtracer.cpp:186-247 (texture members only).
-template <class T> static Mat<T> resolve_material(const Scene &s, uint32_t mid, const V2<T> &uv, bool has_color,
-                                                  const V3<T> &color, T &alpha, const Mat<T> *override_mats) {
+// A point of the mesh as the material resolve sees it (SurfaceQuery,
+// pathtracer.cpp:95-108).
+template <class T> struct Query {
+  uint32_t prim{0}, mat{0};
+  T b1{0}, b2{0};
+  V3<T> p, sn, color;
+  V2<T> uv;
+  bool has_color{false};
+};
+// The query of a hit: `hsn` is the hit's shading normal after the flat /
+// side fix of pathtracer.cpp:906-910.
+template <class T> static Query<T> query_of(const Hit<T> &hr, const V3<T> &hsn) {
+  Query<T> q;
+  q.prim = hr.prim;
+  q.mat = hr.mat;
+  q.b1 = hr.bary.x;
+  q.b2 = hr.bary.y;
+  q.p = hr.p;
+  q.sn = hsn;
+  q.uv = hr.uv;
+  q.color = hr.color;
+  q.has_color = hr.has_color;
+  return q;
+}
+// fetch_attribute (pathtracer.cpp:111-123)
+template <class T> static V4<T> fetch_attribute(const Scene &s, int slot, const Query<T> &q) {
+  if (slot == PHX_ATTRIBUTE_WORLD_POSITION) return {q.p.x, q.p.y, q.p.z, T(1)};
+  if (slot == PHX_ATTRIBUTE_WORLD_NORMAL) return {q.sn.x, q.sn.y, q.sn.z, T(1)};
+  if (slot < 0 || slot >= PHX_ATTRIBUTE_MESH_COUNT || s.attr_off[slot] < 0) return {T(0), T(0), T(0), T(1)};
+  const size_t base = size_t(s.attr_off[slot]);
+  auto rec = [&](uint32_t vtx) {
+    const float *a = &s.attrs[(base + vtx) * 4];
+    return V4<T>{T(a[0]), T(a[1]), T(a[2]), T(a[3])};
+  };
+  const V4<T> a0 = rec(s.mesh.idx[q.prim * 3]), a1 = rec(s.mesh.idx[q.prim * 3 + 1]), a2 = rec(s.mesh.idx[q.prim * 3 + 2]);
+  const T w0 = T(1) - q.b1 - q.b2;
+  return {a0.x * w0 + a1.x * q.b1 + a2.x * q.b2, a0.y * w0 + a1.y * q.b1 + a2.y * q.b2,
+          a0.z * w0 + a1.z * q.b1 + a2.z * q.b2, a0.w * w0 + a1.w * q.b1 + a2.w * q.b2};
+}
+// SamplerTable::wrap_index (sampler.h:62-71)
+static int wrap_index(int i, int n, uint32_t mode) {
+  if (mode == PHX_WRAP_REPEAT) return ((i % n) + n) % n;
+  if (mode == PHX_WRAP_MIRROR) {
+    const int j = ((i % (2 * n)) + 2 * n) % (2 * n);
+    return j < n ? j : 2 * n - 1 - j;
+  }
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+// mul_mat4 (math_vec4.h:37-43) then the offset
+template <class T> static V4<T> mat4_apply(const float *m, const float *o, const V4<T> &v) {
+  V4<T> r{T(m[0]) * v.x + T(m[1]) * v.y + T(m[2]) * v.z + T(m[3]) * v.w,
+          T(m[4]) * v.x + T(m[5]) * v.y + T(m[6]) * v.z + T(m[7]) * v.w,
+          T(m[8]) * v.x + T(m[9]) * v.y + T(m[10]) * v.z + T(m[11]) * v.w,
+          T(m[12]) * v.x + T(m[13]) * v.y + T(m[14]) * v.z + T(m[15]) * v.w};
+  r.x += T(o[0]);
+  r.y += T(o[1]);
+  r.z += T(o[2]);
+  r.w += T(o[3]);
+  return r;
+}
+// SamplerTable::sample (sampler.h:73-138)
+template <class T> static V4<T> sampler_sample(const Scene &s, int id, const V4<T> &attr) {
+  if (id < 0 || size_t(id) >= s.samplers.size()) return {T(0), T(0), T(0), T(1)};
+  const phx_sampler_info &si = s.samplers[size_t(id)];
+  V4<T> c = mat4_apply<T>(si.in_transform, si.in_offset, attr);
+  V4<T> result = c;
+  if (si.type != PHX_SAMPLER_TRANSFORM) {
+    const int dims = si.type == PHX_SAMPLER_IMAGE1D ? 1 : (si.type == PHX_SAMPLER_IMAGE2D ? 2 : 3);
+    const T coord[3] = {c.x, c.y, c.z};
+    int i0[3] = {0, 0, 0}, i1[3] = {0, 0, 0};
+    T frac[3] = {T(0), T(0), T(0)};
+    for (int d = 0; d < dims; ++d) {
+      const int n = int(si.size[d]);
+      T x = coord[d];
+      if (!(x > T(-1e6) && x < T(1e6))) x = T(0);
+      if (si.linear) {
+        const T f = x * T(n) - T(0.5);
+        const T fl = std::floor(f);
+        frac[d] = f - fl;
+        i0[d] = wrap_index(int(fl), n, si.wrap[d]);
+        i1[d] = wrap_index(int(fl) + 1, n, si.wrap[d]);
+      } else {
+        i0[d] = wrap_index(int(std::floor(x * T(n))), n, si.wrap[d]);
+        i1[d] = i0[d];
+      }
+    }
+    const uint32_t sx = si.size[0], sxy = si.size[0] * si.size[1], base = si.offset;
+    auto texel = [&](uint32_t i) {
+      const float *t = &s.sampler_texels[size_t(i) * 4];
+      return V4<T>{T(t[0]), T(t[1]), T(t[2]), T(t[3])};
+    };
+    V4<T> acc{};
+    const int nz = dims > 2 ? 2 : 1, ny = dims > 1 ? 2 : 1;
+    for (int kz = 0; kz < nz; ++kz) {
+      const T wz = dims > 2 ? (kz ? frac[2] : T(1) - frac[2]) : T(1);
+      const uint32_t z = uint32_t(kz ? i1[2] : i0[2]);
+      for (int ky = 0; ky < ny; ++ky) {
+        const T wy = dims > 1 ? (ky ? frac[1] : T(1) - frac[1]) : T(1);
+        const uint32_t y = uint32_t(ky ? i1[1] : i0[1]);
+        const V4<T> a = texel(base + z * sxy + y * sx + uint32_t(i0[0]));
+        const V4<T> b = texel(base + z * sxy + y * sx + uint32_t(i1[0]));
+        const T wt = wy * wz;
+        acc.x = acc.x + (a.x * (T(1) - frac[0]) + b.x * frac[0]) * wt;
+        acc.y = acc.y + (a.y * (T(1) - frac[0]) + b.y * frac[0]) * wt;
+        acc.z = acc.z + (a.z * (T(1) - frac[0]) + b.z * frac[0]) * wt;
+        acc.w = acc.w + (a.w * (T(1) - frac[0]) + b.w * frac[0]) * wt;
+      }
+    }
+    result = acc;
+  }
+  return mat4_apply<T>(si.out_transform, si.out_offset, result);
+}
+// eval_source (pathtracer.cpp:125-133)
+template <class T> static V4<T> eval_source(const Scene &s, const phx_param_source &src, const Query<T> &q) {
+  if (src.kind == PHX_SOURCE_ATTRIBUTE) return fetch_attribute<T>(s, src.index, q);
+  if (src.index < 0 || size_t(src.index) >= s.samplers.size()) return {T(0), T(0), T(0), T(1)};
+  return sampler_sample<T>(s, src.index, fetch_attribute<T>(s, s.samplers[size_t(src.index)].in_attribute, q));
+}
+// apply_normal_map (pathtracer.cpp:135-176)
+template <class T>
+static V3<T> apply_normal_map(const Scene &s, const phx_param_source &src, const Query<T> &q, const V3<T> &sn) {
+  const V4<T> c = eval_source<T>(s, src, q);
+  const V3<T> nm{T(2) * c.x - T(1), T(2) * c.y - T(1), T(2) * c.z - T(1)};
+  int slot = src.index;
+  if (src.kind != PHX_SOURCE_ATTRIBUTE) {
+    if (src.index < 0 || size_t(src.index) >= s.samplers.size()) return sn;
+    slot = s.samplers[size_t(src.index)].in_attribute;
+  }
+  Query<T> corner = q;
+  corner.b1 = T(0);
+  corner.b2 = T(0);
+  const V4<T> a0 = fetch_attribute<T>(s, slot, corner);
+  corner.b1 = T(1);
+  const V4<T> a1 = fetch_attribute<T>(s, slot, corner);
+  corner.b1 = T(0);
+  corner.b2 = T(1);
+  const V4<T> a2 = fetch_attribute<T>(s, slot, corner);
+  const T du1 = a1.x - a0.x, dv1 = a1.y - a0.y, du2 = a2.x - a0.x, dv2 = a2.y - a0.y;
+  const T det = du1 * dv2 - du2 * dv1;
+  if (!(std::fabs(det) > T(1e-20))) return sn;
+  const Mesh &m = s.mesh;
+  const V3<T> p0 = v3<T>(&m.pos[size_t(m.idx[q.prim * 3]) * 3]);
+  const V3<T> e1 = v3<T>(&m.pos[size_t(m.idx[q.prim * 3 + 1]) * 3]) - p0;
+  const V3<T> e2 = v3<T>(&m.pos[size_t(m.idx[q.prim * 3 + 2]) * 3]) - p0;
+  V3<T> t = (e1 * dv2 - e2 * dv1) * (T(1) / det);
+  V3<T> b = (e2 * du1 - e1 * du2) * (T(1) / det);
+  t = t - sn * dot(sn, t);
+  b = b - sn * dot(sn, b);
+  if (!(dot(t, t) > T(0)) || !(dot(b, b) > T(0))) return sn;
+  const V3<T> n = normalize(t) * nm.x + normalize(b) * nm.y + sn * nm.z;
+  return dot(n, n) > T(0) ? normalize(n) : sn;
+}
+// resolve_material (pathtracer.cpp:186-247): textures, then sources, then
+// the clamps and the alpha mode.
+template <class T> static Mat<T> resolve_material(const Scene &s, const Query<T> &q, T &alpha, const Mat<T> *override_mats) {
+  const uint32_t mid = q.mat;
   Mat<T> m = override_mats ? override_mats[mid] : (s.mats.empty() ? Mat<T>{} : mat_from<T>(s.mats[mid]));
   const bool tex = !s.atlas.infos.empty();
   alpha = m.alpha;
-  if (m.alpha_tex >= 0 && tex) alpha = s.atlas.sample<T>(m.alpha_tex, uv.x, uv.y).x;
-  if (m.base_color_tex >= 0 && tex) m.base_color = s.atlas.sample<T>(m.base_color_tex, uv.x, uv.y);
-  else if (has_color) m.base_color = color;
-  if (m.roughness_tex >= 0 && tex) m.roughness = s.atlas.sample<T>(m.roughness_tex, uv.x, uv.y).x;
-  if (m.metallic_tex >= 0 && tex) m.metallic = s.atlas.sample<T>(m.metallic_tex, uv.x, uv.y).x;
+  if (m.alpha_tex >= 0 && tex) alpha = s.atlas.sample<T>(m.alpha_tex, q.uv.x, q.uv.y).x;
+  if (m.base_color_tex >= 0 && tex) m.base_color = s.atlas.sample<T>(m.base_color_tex, q.uv.x, q.uv.y);
+  else if (q.has_color) m.base_color = q.color;
+  if (m.roughness_tex >= 0 && tex) m.roughness = s.atlas.sample<T>(m.roughness_tex, q.uv.x, q.uv.y).x;
+  if (m.metallic_tex >= 0 && tex) m.metallic = s.atlas.sample<T>(m.metallic_tex, q.uv.x, q.uv.y).x;
   auto c01 = [](T x) { return x < T(0) ? T(0) : (x > T(1) ? T(1) : x); };
+  if (m.base_color_src.kind != PHX_SOURCE_CONSTANT) {
+    const V4<T> c = eval_source<T>(s, m.base_color_src, q);
+    m.base_color = {fmaxT(c.x, T(0)), fmaxT(c.y, T(0)), fmaxT(c.z, T(0))};
+    if (m.base_color_alpha) alpha *= c.w;
+  }
+  if (m.alpha_src.kind != PHX_SOURCE_CONSTANT) alpha *= eval_source<T>(s, m.alpha_src, q).x;
+  if (m.metallic_src.kind != PHX_SOURCE_CONSTANT) m.metallic = eval_source<T>(s, m.metallic_src, q).x;
+  if (m.roughness_src.kind != PHX_SOURCE_CONSTANT) {
+    const T r = c01(eval_source<T>(s, m.roughness_src, q).x);
+    m.roughness = m.roughness_src_squared ? r * r : r;
+  }
+  if (m.transmission_src.kind != PHX_SOURCE_CONSTANT) m.transmission = eval_source<T>(s, m.transmission_src, q).x;
+  if (m.emission_src.kind != PHX_SOURCE_CONSTANT) {
+    const V4<T> e = eval_source<T>(s, m.emission_src, q);
+    m.emission = {fmaxT(e.x, T(0)), fmaxT(e.y, T(0)), fmaxT(e.z, T(0))};
+  }
   m.metallic = c01(m.metallic);
   m.roughness = c01(m.roughness);
   m.transmission = c01(m.transmission);
@@ -419,6 +599,39 @@ template <class T> static Mat<T> resolve_material(const Scene &s, uint32_t mid, 
   else if (m.alpha_mode == PHX_ALPHA_MASK) alpha = alpha >= m.alpha_cutoff ? T(1) : T(0);
   else alpha = c01(alpha);
   return m;
+}
+// sample_background_image (pathtracer.cpp:370-389) over the RGBA rows of the
+// render parameters' background image.
+template <class T> static V4<T> background_at(const phx_render_params &rp, T bs, T bt) {
+  const int W = int(rp.background_width), H = int(rp.background_height);
+  const T fx = bs * T(W) - T(0.5), fy = bt * T(H) - T(0.5);
+  const T flx = std::floor(fx), fly = std::floor(fy);
+  const T tx = fx - flx, ty = fy - fly;
+  auto cl = [](int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); };
+  const int x0 = cl(int(flx), W), x1 = cl(int(flx) + 1, W), y0 = cl(int(fly), H), y1 = cl(int(fly) + 1, H);
+  auto px = [&](int x, int y) {
+    const float *p = rp.background_image + (size_t(y) * W + x) * 4;
+    return V4<T>{T(p[0]), T(p[1]), T(p[2]), T(p[3])};
+  };
+  const V4<T> p00 = px(x0, y0), p01 = px(x1, y0), p10 = px(x0, y1), p11 = px(x1, y1);
+  auto lerp4 = [](const V4<T> &a, const V4<T> &b, T t) {
+    return V4<T>{a.x * (T(1) - t) + b.x * t, a.y * (T(1) - t) + b.y * t, a.z * (T(1) - t) + b.z * t,
+                 a.w * (T(1) - t) + b.w * t};
+  };
+  return lerp4(lerp4(p00, p01, tx), lerp4(p10, p11, tx), ty);
+}
+// The camera-ray background of pixel idx (pathtracer.cpp:862-877): the
+// image's colour times its alpha, and the alpha.
+template <class T> static V3<T> camera_background(const phx_render_params &rp, uint32_t idx, T &bg_alpha) {
+  if (rp.background_image && rp.background_width && rp.background_height) {
+    const uint32_t x = idx % rp.width, y = idx / rp.width;
+    const T bs = (T(x) + T(0.5)) / T(rp.width), bt = (T(y) + T(0.5)) / T(rp.height);
+    const V4<T> bg = background_at<T>(rp, bs, rp.bottom_up ? bt : T(1) - bt);
+    bg_alpha = bg.w;
+    return V3<T>{bg.x, bg.y, bg.z} * bg.w;
+  }
+  bg_alpha = T(rp.background_alpha);
+  return v3<T>(rp.background_color) * T(rp.background_alpha);
 }
 template <class T> static bool emits_on_side(const Mat<T> &m, bool front) {  // pathtracer.cpp:250-256
   if (m.flags & PHX_MATERIAL_FLAG_EMIT_BOTH_SIDES) return true;
@@ -489,10 +702,16 @@ template <class T> static Direct<T> sample_direct(const Scene &s, const V3<T> &p
     const T cl = -dot(wi, ln);
     if (std::fabs(cl) < T(1e-6)) return ds;
     // resolve_material at the light point: its texcoords are not interpolated
-    // there (SurfaceQuery::uv stays 0), pathtracer.cpp:333-341.
+    // there (SurfaceQuery::uv stays 0), pathtracer.cpp:325-341.
     T alpha = T(1);
-    const uint32_t mid = m.mat.empty() ? 0u : m.mat[prim];
-    const Mat<T> em = resolve_material<T>(s, mid, V2<T>{}, false, V3<T>{}, alpha, ov);
+    Query<T> q;
+    q.prim = prim;
+    q.mat = m.mat.empty() ? 0u : m.mat[prim];
+    q.b1 = b1;
+    q.b2 = b2;
+    q.p = lp;
+    q.sn = ln;
+    const Mat<T> em = resolve_material<T>(s, q, alpha, ov);
     if (!emits_on_side(em, cl > T(0))) return ds;
     ds.wi = wi;
     ds.L = material_emission(em) * alpha;
@@ -545,7 +764,6 @@ static void render_pixel(const Scene &s, const Params &P, uint32_t idx, PixelOut
   const T eps = T(s.ray_eps);
   const T cd = T(rp.clamp_direct_sum), ci = T(rp.clamp_indirect_sum);
   const V3<T> ambient = v3<T>(rp.ambient_color) * T(rp.ambient_radiance);
-  const V3<T> bg = v3<T>(rp.background_color);
   const uint32_t x = idx % w, y = idx / w;
   out = PixelOut<T>{};
   for (uint32_t sidx = 0; sidx < rp.samples_per_pixel; ++sidx) {
@@ -630,8 +848,9 @@ static void render_pixel(const Scene &s, const Params &P, uint32_t idx, PixelOut
             L = s.env.evaluate<T>(rd);
             out.alpha += T(1);
           } else {
-            L = bg * T(rp.background_alpha);
-            out.alpha += T(rp.background_alpha);
+            T ba;
+            L = camera_background<T>(rp, idx, ba);
+            out.alpha += ba;
           }
         } else {
           L = ambient;
@@ -654,7 +873,8 @@ static void render_pixel(const Scene &s, const Params &P, uint32_t idx, PixelOut
         if (!(dot(hsn, hsn) > T(0.25))) hsn = hr.n;
         else if (dot(hsn, hr.n) < T(0)) hsn = hsn * T(-1);
         T alpha = T(1);
-        const Mat<T> mat = resolve_material<T>(s, hr.mat, hr.uv, hr.has_color, hr.color, alpha, ov);
+        const Query<T> hq = query_of<T>(hr, hsn);
+        const Mat<T> mat = resolve_material<T>(s, hq, alpha, ov);
         const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0 && primary;
         if (hidden || (alpha < T(1) && draw<T>(rng) >= alpha)) {
           ro = p;
@@ -688,6 +908,7 @@ static void render_pixel(const Scene &s, const Params &P, uint32_t idx, PixelOut
           onb_from_normal(sn, ta, bi);
           sn = normalize(ta * nm.x + bi * nm.y + sn * nm.z);
         }
+        if (mat.normal_src.kind != PHX_SOURCE_CONSTANT) sn = apply_normal_map<T>(s, mat.normal_src, hq, sn);
         if (primary) {  // :974-983 (the AOVs come from pixel_aov)
           primary = false;
           out.alpha += T(1);
@@ -834,7 +1055,6 @@ static V3<double> detached_pixel(const Scene &s, const Params &P, uint32_t idx, 
   const bool has_vol = !s.vol.empty();
   const float eps = s.ray_eps;
   const V3<float> ambient = v3<float>(rp.ambient_color) * rp.ambient_radiance;
-  const V3<float> bg = v3<float>(rp.background_color);
   const uint32_t x = idx % w, y = idx / w;
   V3<double> acc{};
   auto wv = [](const V3<float> &v) { return V3<double>(v); };
@@ -918,7 +1138,8 @@ static V3<double> detached_pixel(const Scene &s, const Params &P, uint32_t idx, 
       if (!hr.hit) {
         V3<float> L{};
         if (primary) {
-          L = (s.has_env && s.env_visible) ? s.env.evaluate<float>(rd) : bg * rp.background_alpha;
+          float ba;
+          L = (s.has_env && s.env_visible) ? s.env.evaluate<float>(rd) : camera_background<float>(rp, idx, ba);
         } else {
           L = ambient;
           if (s.has_env) {
@@ -938,7 +1159,8 @@ static V3<double> detached_pixel(const Scene &s, const Params &P, uint32_t idx, 
         if (!(dot(hsn, hsn) > 0.25f)) hsn = hr.n;
         else if (dot(hsn, hr.n) < 0.f) hsn = hsn * -1.f;
         float alpha = 1.f;
-        const Mat<float> mat = resolve_material<float>(s, hr.mat, hr.uv, hr.has_color, hr.color, alpha, nullptr);
+        const Query<float> hq = query_of<float>(hr, hsn);
+        const Mat<float> mat = resolve_material<float>(s, hq, alpha, nullptr);
         const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0 && primary;
         if (hidden || (alpha < 1.f && draw<float>(rng) >= alpha)) {
           ro = p;
@@ -972,6 +1194,7 @@ static V3<double> detached_pixel(const Scene &s, const Params &P, uint32_t idx, 
           onb_from_normal(sn, ta, bi);
           sn = normalize(ta * nm.x + bi * nm.y + sn * nm.z);
         }
+        if (mat.normal_src.kind != PHX_SOURCE_CONSTANT) sn = apply_normal_map<float>(s, mat.normal_src, hq, sn);
         const Mat<double> mw = detached_material(s, E, hr.mat, hr.uv, hr.has_color, hr.color);
         if (primary) primary = false;
         if (material_may_emit(mat) && emits_on_side(mat, front)) {
@@ -1106,7 +1329,8 @@ template <class T> static void pixel_aov(const Scene &s, const Params &P, uint32
       if (!(dot(hsn, hsn) > T(0.25))) hsn = hr.n;
       else if (dot(hsn, hr.n) < T(0)) hsn = hsn * T(-1);
       T alpha = T(1);
-      const Mat<T> mat = resolve_material<T>(s, hr.mat, hr.uv, hr.has_color, hr.color, alpha, nullptr);
+      const Query<T> hq = query_of<T>(hr, hsn);
+      const Mat<T> mat = resolve_material<T>(s, hq, alpha, nullptr);
       const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0;
       if (hidden || (alpha < T(1) && draw<T>(rng) >= alpha)) {
         ro = hr.p;
@@ -1131,6 +1355,7 @@ template <class T> static void pixel_aov(const Scene &s, const Params &P, uint32
         onb_from_normal(sn, ta, bi);
         sn = normalize(ta * nm.x + bi * nm.y + sn * nm.z);
       }
+      if (mat.normal_src.kind != PHX_SOURCE_CONSTANT) sn = apply_normal_map<T>(s, mat.normal_src, hq, sn);
       out.depth = path_dist + hr.t;
       out.normal = normalize(sn);
       out.albedo = mat.base_color;

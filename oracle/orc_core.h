@@ -188,6 +188,10 @@ template <class T> struct Mat {
   int thin{0}, flags{0}, alpha_mode{0};
   T alpha_cutoff{T(0.5)};
   int base_color_tex{-1}, normal_tex{-1}, roughness_tex{-1}, metallic_tex{-1}, emission_tex{-1}, alpha_tex{-1};
+  // material.h:62-81
+  phx_param_source base_color_src{}, alpha_src{}, metallic_src{}, roughness_src{}, emission_src{}, transmission_src{},
+      normal_src{};
+  int base_color_alpha{0}, roughness_src_squared{0};
 };
 template <class T> inline Mat<T> mat_from(const phx_material &m) {
   Mat<T> o;
@@ -218,14 +222,24 @@ template <class T> inline Mat<T> mat_from(const phx_material &m) {
   o.metallic_tex = m.metallic_tex;
   o.emission_tex = m.emission_tex;
   o.alpha_tex = m.alpha_tex;
+  o.base_color_src = m.base_color_src;
+  o.alpha_src = m.alpha_src;
+  o.metallic_src = m.metallic_src;
+  o.roughness_src = m.roughness_src;
+  o.emission_src = m.emission_src;
+  o.transmission_src = m.transmission_src;
+  o.normal_src = m.normal_src;
+  o.base_color_alpha = m.base_color_alpha;
+  o.roughness_src_squared = m.roughness_src_squared;
   return o;
 }
 // material.h:84-87
 template <class T> inline V3<T> material_emission(const Mat<T> &m) { return m.emission * m.emission_strength; }
-// material.h:97-103 (no attribute/sampler sources in this ABI)
+// material.h:97-103
 template <class T> inline bool material_may_emit(const Mat<T> &m) {
   if (m.emission_strength <= T(0)) return false;
-  return m.emission_tex >= 0 || m.emission.x > T(0) || m.emission.y > T(0) || m.emission.z > T(0);
+  return m.emission_src.kind != PHX_SOURCE_CONSTANT || m.emission_tex >= 0 || m.emission.x > T(0) ||
+         m.emission.y > T(0) || m.emission.z > T(0);
 }
 
 // ---- Disney BSDF (disney_bsdf.h) ---------------------------------------------

@@ -60,6 +60,7 @@ def _sig(lib):
         "phx_convert_material": (C.c_int, [C.POINTER(abi.BridgeMaterial), C.POINTER(abi.Material)]),
         "phx_bridge_render": (C.c_int, [P, C.POINTER(abi.CameraData), C.POINTER(abi.RenderSettings), C.c_int]),
         "phx_bridge_accumulated_samples": (C.c_uint32, [P]),
+        "phx_set_scene_data": (C.c_int, [P, C.POINTER(abi.SceneData)]),
         "phx_bridge_read_color": (C.c_int, [P, PF]),
         "phx_bridge_read_depth": (C.c_int, [P, PF]),
         "phx_bridge_read_normal": (C.c_int, [P, PF]),

@@ -8,30 +8,47 @@ import ctypes as C
 F3 = C.c_float * 3
 
 
+class ParamSource(C.Structure):
+    """pt::ParamSource (material.h:16-19) / bridge::ParamSourceData."""
+    _fields_ = [("kind", C.c_int32), ("index", C.c_int32)]
+
+
+_SOURCES = ("base_color_src", "alpha_src", "metallic_src", "roughness_src", "emission_src", "transmission_src",
+            "normal_src")
+
+
 class Material(C.Structure):
-    """pt::Material (reference include/photon/pt/material.h:35-82), no ParamSources."""
+    """pt::Material (reference include/photon/pt/material.h:35-82), 200 bytes."""
     _fields_ = [
         ("base_color", F3), ("metallic", C.c_float),
         ("roughness", C.c_float), ("ior", C.c_float), ("transmission", C.c_float), ("specular", C.c_float),
         ("clearcoat", C.c_float), ("clearcoat_roughness", C.c_float), ("sheen", C.c_float), ("sheen_tint", C.c_float),
-        ("specular_tint", C.c_float), ("anisotropic", C.c_float), ("emission_strength", C.c_float),
-        ("subsurface", C.c_float),
-        ("emission", F3), ("subsurface_radius", C.c_float),
-        ("subsurface_color", F3), ("alpha", C.c_float),
-        ("thin", C.c_int32), ("flags", C.c_int32), ("alpha_mode", C.c_int32), ("alpha_cutoff", C.c_float),
+        ("specular_tint", C.c_float), ("anisotropic", C.c_float),
+        ("emission", F3), ("emission_strength", C.c_float), ("subsurface", C.c_float),
+        ("subsurface_color", F3), ("subsurface_radius", C.c_float), ("alpha", C.c_float), ("thin", C.c_int32),
         ("base_color_tex", C.c_int32), ("normal_tex", C.c_int32), ("roughness_tex", C.c_int32),
         ("metallic_tex", C.c_int32), ("emission_tex", C.c_int32), ("alpha_tex", C.c_int32),
-        ("pad_", C.c_int32 * 2),
+    ] + [(n, ParamSource) for n in _SOURCES] + [
+        ("alpha_mode", C.c_int32), ("alpha_cutoff", C.c_float), ("flags", C.c_int32),
+        ("base_color_alpha", C.c_int32), ("roughness_src_squared", C.c_int32),
     ]
 
 
 class Light(C.Structure):
-    """pt::Light (light.h:19-35)."""
+    """pt::Light (light.h:19-35), 92 bytes."""
     _fields_ = [
         ("type", C.c_uint32), ("position", F3), ("direction", F3), ("color", F3), ("intensity", C.c_float),
         ("spot_angle", C.c_float), ("spot_falloff", C.c_float), ("mesh_prim_begin", C.c_uint32),
         ("mesh_prim_count", C.c_uint32), ("area", C.c_float), ("edge1", F3), ("edge2", F3), ("radius", C.c_float),
-        ("pad_", C.c_uint32),
+    ]
+
+
+class SamplerInfo(C.Structure):
+    """pt::SamplerInfo (sampler.h:39-52), 200 bytes."""
+    _fields_ = [
+        ("type", C.c_uint32), ("in_attribute", C.c_int32), ("linear", C.c_uint32), ("wrap", C.c_uint32 * 3),
+        ("offset", C.c_uint32), ("size", C.c_uint32 * 3), ("in_transform", C.c_float * 16),
+        ("in_offset", C.c_float * 4), ("out_transform", C.c_float * 16), ("out_offset", C.c_float * 4),
     ]
 
 
@@ -40,13 +57,12 @@ class TextureInfo(C.Structure):
 
 
 class Volume(C.Structure):
-    """pt::TfVolume (tf_volume.h:14-33)."""
+    """pt::TfVolume (tf_volume.h:14-33), 132 bytes."""
     _fields_ = [
         ("world_to_local", C.c_float * 12), ("origin", F3), ("spacing", F3), ("dims", C.c_uint32 * 3),
         ("data_offset", C.c_uint32), ("tf_offset", C.c_uint32), ("tf_size", C.c_uint32),
         ("value_lo", C.c_float), ("value_hi", C.c_float), ("density_scale", C.c_float), ("majorant", C.c_float),
         ("linear", C.c_uint32), ("macro_offset", C.c_uint32), ("macro_dims", C.c_uint32 * 3),
-        ("pad_", C.c_uint32 * 3),
     ]
 
 
@@ -62,6 +78,9 @@ class SceneDesc(C.Structure):
         ("emissive_count", C.c_uint32),
         ("texture_pixels", C.POINTER(C.c_float)), ("texture_pixel_count", C.c_uint32),
         ("texture_infos", C.POINTER(TextureInfo)), ("texture_count", C.c_uint32),
+        ("attributes", C.POINTER(C.c_float)), ("attribute_count", C.c_uint32), ("attribute_offset", C.c_int32 * 5),
+        ("samplers", C.POINTER(SamplerInfo)), ("sampler_count", C.c_uint32),
+        ("sampler_texels", C.POINTER(C.c_float)), ("sampler_texel_count", C.c_uint32),
         ("env_width", C.c_uint32), ("env_height", C.c_uint32), ("env_pixels", C.POINTER(C.c_float)),
         ("env_world_to_tex", C.c_float * 9), ("env_visible", C.c_int32),
         ("volumes", C.POINTER(Volume)), ("volume_count", C.c_uint32),
@@ -77,13 +96,14 @@ class Camera(C.Structure):
     _fields_ = [
         ("origin", F3), ("lower_left", F3), ("horizontal", F3), ("vertical", F3),
         ("u", F3), ("v", F3), ("w", F3),
-        ("lens_radius", C.c_float), ("focus_dist", C.c_float), ("is_ortho", C.c_int32),
+        ("lens_radius", C.c_float), ("focus_dist", C.c_float), ("is_ortho", C.c_uint8), ("pad_", C.c_uint8 * 3),
         ("clip_near", C.c_float), ("clip_far", C.c_float),
     ]
 
 
 class RenderParams(C.Structure):
-    """pt::RenderParams (pathtracer.h:16-55) without the background image."""
+    """pt::RenderParams (pathtracer.h:16-55); the background image is a device
+    pointer to RGBA rows from the bottom of the frame."""
     _fields_ = [
         ("width", C.c_uint32), ("height", C.c_uint32), ("samples_per_pixel", C.c_uint32),
         ("max_depth", C.c_uint32), ("sample_offset", C.c_uint32),
@@ -94,6 +114,7 @@ class RenderParams(C.Structure):
         ("max_transmission_bounces", C.c_int32), ("max_volume_bounces", C.c_int32),
         ("clamp_direct_sum", C.c_float), ("clamp_indirect_sum", C.c_float),
         ("bottom_up", C.c_int32), ("image_region", C.c_float * 4),
+        ("background_image", C.c_void_p), ("background_width", C.c_uint32), ("background_height", C.c_uint32),
     ]
 
 
@@ -103,7 +124,7 @@ class Hit(C.Structure):
         ("t", C.c_float), ("position", F3), ("normal", F3), ("shading_normal", F3),
         ("uv", C.c_float * 2), ("bary", C.c_float * 2), ("interpolated_color", F3),
         ("prim_id", C.c_uint32), ("geom_id", C.c_uint32), ("inst_id", C.c_uint32), ("material_id", C.c_uint32),
-        ("hit", C.c_uint32), ("has_interpolated_color", C.c_uint32), ("pad_", C.c_uint32),
+        ("hit", C.c_uint8), ("has_interpolated_color", C.c_uint8), ("pad_", C.c_uint8 * 2),
     ]
 
 
@@ -130,21 +151,22 @@ class Stats(C.Structure):
 
 
 class BridgeMaterial(C.Structure):
-    """bridge::MaterialData (RenderBridge.h:27-56); sources by ParamSourceData kind."""
+    """bridge::MaterialData (RenderBridge.h:21-56), 132 bytes."""
     _fields_ = [
         ("base_color", F3), ("opacity", C.c_float), ("metallic", C.c_float), ("roughness", C.c_float),
         ("ior", C.c_float), ("transmission", C.c_float), ("specular", C.c_float), ("emission", F3),
         ("clearcoat", C.c_float), ("clearcoat_roughness", C.c_float), ("sheen", C.c_float),
-        ("base_color_src", C.c_int32), ("opacity_src", C.c_int32), ("metallic_src", C.c_int32),
-        ("roughness_src", C.c_int32), ("emission_src", C.c_int32), ("transmission_src", C.c_int32),
-        ("normal_src", C.c_int32), ("alpha_mode", C.c_int32), ("alpha_cutoff", C.c_float),
-        ("base_color_alpha", C.c_int32), ("thin_walled", C.c_int32), ("emitter_only", C.c_int32),
-        ("hidden_to_camera", C.c_int32), ("emission_side", C.c_int32),
+        ("base_color_src", ParamSource), ("opacity_src", ParamSource), ("metallic_src", ParamSource),
+        ("roughness_src", ParamSource), ("emission_src", ParamSource), ("transmission_src", ParamSource),
+        ("normal_src", ParamSource), ("alpha_mode", C.c_int32), ("alpha_cutoff", C.c_float),
+        ("base_color_alpha", C.c_uint8), ("thin_walled", C.c_uint8), ("emitter_only", C.c_uint8),
+        ("hidden_to_camera", C.c_uint8), ("emission_side", C.c_int32),
     ]
 
 
 def bridge_material(**fields):
-    """A MaterialData with the bridge's defaults (RenderBridge.h:27-56), then `fields`."""
+    """A MaterialData with the bridge's defaults (RenderBridge.h:27-56), then
+    `fields`; a source is given as (kind, index) or a kind alone."""
     m = BridgeMaterial()
     m.base_color[:] = (0.8, 0.8, 0.8)
     m.opacity, m.metallic, m.roughness, m.ior = 1.0, 0.0, 1.0, 1.5
@@ -155,15 +177,18 @@ def bridge_material(**fields):
     for k, v in fields.items():
         if k in ("base_color", "emission"):
             getattr(m, k)[:] = v
+        elif k.endswith("_src"):
+            kind, index = (v, 0) if isinstance(v, int) else v
+            getattr(m, k).kind, getattr(m, k).index = kind, index
         else:
             setattr(m, k, v)
     return m
 
 
 class CameraData(C.Structure):
-    """bridge::CameraData (RenderBridge.h:128-144)."""
+    """bridge::CameraData (RenderBridge.h:128-144), 84 bytes."""
     _fields_ = [
-        ("position", F3), ("direction", F3), ("up", F3), ("orthographic", C.c_int32),
+        ("position", F3), ("direction", F3), ("up", F3), ("orthographic", C.c_uint8), ("pad_", C.c_uint8 * 3),
         ("fovy", C.c_float), ("height", C.c_float), ("aspect", C.c_float),
         ("aperture_radius", C.c_float), ("focus_distance", C.c_float), ("image_region", C.c_float * 4),
         ("clip_near", C.c_float), ("clip_far", C.c_float),
@@ -171,14 +196,66 @@ class CameraData(C.Structure):
 
 
 class RenderSettings(C.Structure):
-    """bridge::RenderSettings (RenderBridge.h:146-170) without the background image."""
+    """bridge::RenderSettings (RenderBridge.h:146-170), 96 bytes; the background
+    image is host memory."""
     _fields_ = [
         ("width", C.c_uint32), ("height", C.c_uint32), ("pixel_samples", C.c_uint32), ("max_depth", C.c_uint32),
-        ("background", C.c_float * 4), ("ambient_color", F3), ("ambient_radiance", C.c_float),
-        ("adaptive_sampling", C.c_int32), ("throughput_clamp", C.c_float),
+        ("background", C.c_float * 4), ("background_image", C.c_void_p), ("background_width", C.c_uint32),
+        ("background_height", C.c_uint32), ("ambient_color", F3), ("ambient_radiance", C.c_float),
+        ("adaptive_sampling", C.c_uint8), ("pad_", C.c_uint8 * 3), ("throughput_clamp", C.c_float),
         ("max_diffuse_bounces", C.c_int32), ("max_glossy_bounces", C.c_int32),
         ("max_transmission_bounces", C.c_int32), ("max_volume_bounces", C.c_int32),
         ("clamp_direct", C.c_float), ("clamp_indirect", C.c_float),
+    ]
+
+
+class LightData(C.Structure):
+    """bridge::LightData (RenderBridge.h:73-85), 56 bytes."""
+    _fields_ = [
+        ("type", C.c_int32), ("position", F3), ("direction", F3), ("color", F3), ("intensity", C.c_float),
+        ("radius", C.c_float), ("spot_angle", C.c_float), ("spot_falloff", C.c_float),
+    ]
+
+
+class SamplerData(C.Structure):
+    """bridge::SamplerData (RenderBridge.h:58-71), the texel vector as pointer + float count."""
+    _fields_ = [
+        ("type", C.c_int32), ("in_attribute", C.c_int32), ("linear", C.c_uint8), ("pad_", C.c_uint8 * 3),
+        ("wrap", C.c_int32 * 3), ("size", C.c_uint32 * 3), ("in_transform", C.c_float * 16),
+        ("in_offset", C.c_float * 4), ("out_transform", C.c_float * 16), ("out_offset", C.c_float * 4),
+        ("texels", C.c_void_p), ("texel_float_count", C.c_uint64),
+    ]
+
+
+class EnvironmentData(C.Structure):
+    """bridge::EnvironmentData (RenderBridge.h:87-97)."""
+    _fields_ = [
+        ("present", C.c_uint8), ("visible", C.c_uint8), ("pad_", C.c_uint8 * 2), ("width", C.c_uint32),
+        ("height", C.c_uint32), ("pixels", C.c_void_p), ("pixel_float_count", C.c_uint64),
+        ("world_to_map", C.c_float * 9),
+    ]
+
+
+class VolumeData(C.Structure):
+    """bridge::VolumeData (RenderBridge.h:99-110)."""
+    _fields_ = [
+        ("world_to_local", C.c_float * 12), ("origin", F3), ("spacing", F3), ("dims", C.c_uint32 * 3),
+        ("values", C.c_void_p), ("value_count", C.c_uint64), ("transfer", C.c_void_p),
+        ("transfer_float_count", C.c_uint64), ("value_range", C.c_float * 2), ("density_scale", C.c_float),
+        ("linear", C.c_uint8), ("pad_", C.c_uint8 * 3),
+    ]
+
+
+class SceneData(C.Structure):
+    """bridge::SceneData (RenderBridge.h:112-125), vectors as pointer + count."""
+    _fields_ = [
+        ("positions", C.c_void_p), ("position_count", C.c_uint64), ("normals", C.c_void_p),
+        ("normal_count", C.c_uint64), ("attributes", C.c_void_p * 5), ("attribute_count", C.c_uint64 * 5),
+        ("indices", C.c_void_p), ("index_count", C.c_uint64), ("material_ids", C.c_void_p),
+        ("material_id_count", C.c_uint64), ("materials", C.POINTER(BridgeMaterial)), ("material_count", C.c_uint32),
+        ("samplers", C.POINTER(SamplerData)), ("sampler_count", C.c_uint32),
+        ("lights", C.POINTER(LightData)), ("light_count", C.c_uint32), ("environment", EnvironmentData),
+        ("volumes", C.POINTER(VolumeData)), ("volume_count", C.c_uint32),
     ]
 
 
@@ -187,8 +264,8 @@ class View(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("params", RenderParams)]
 
 
-EXPECTED_SIZES = {Material: 144, Light: 96, TextureInfo: 16, Volume: 144, Hit: 96, HitCompact: 16, Camera: 104,
-                  Tuning: 32}
+EXPECTED_SIZES = {Material: 200, Light: 92, TextureInfo: 16, Volume: 132, SamplerInfo: 200, Hit: 88, HitCompact: 16,
+                  Camera: 104, Tuning: 32, BridgeMaterial: 132, CameraData: 84, RenderSettings: 96, LightData: 56}
 
 
 def check_layout():
@@ -205,6 +282,26 @@ PHX_MATERIAL_FLAG_REFRACTIVE = 8
 PHX_MATERIAL_FLAG_EMIT_BACK_ONLY = 16
 PHX_ALPHA_BLEND, PHX_ALPHA_OPAQUE, PHX_ALPHA_MASK = 0, 1, 2
 PHX_LIGHT_POINT, PHX_LIGHT_DIRECTIONAL, PHX_LIGHT_SPOT, PHX_LIGHT_AREA = 0, 1, 2, 3
+PHX_SOURCE_CONSTANT, PHX_SOURCE_ATTRIBUTE, PHX_SOURCE_SAMPLER = 0, 1, 2
+PHX_ATTRIBUTE_COLOR, PHX_ATTRIBUTE_WORLD_POSITION, PHX_ATTRIBUTE_WORLD_NORMAL = 4, 5, 6
+PHX_SAMPLER_IMAGE1D, PHX_SAMPLER_IMAGE2D, PHX_SAMPLER_IMAGE3D, PHX_SAMPLER_TRANSFORM = 0, 1, 2, 3
+PHX_WRAP_CLAMP, PHX_WRAP_REPEAT, PHX_WRAP_MIRROR = 0, 1, 2
+
+_IDENTITY4 = (1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0)
+
+
+def sampler_info(type_=PHX_SAMPLER_IMAGE2D, in_attribute=0, linear=1, wrap=(0, 0, 0), offset=0, size=(1, 1, 1),
+                 in_transform=_IDENTITY4, in_offset=(0, 0, 0, 0), out_transform=_IDENTITY4, out_offset=(0, 0, 0, 0)):
+    """A SamplerInfo with the reference's defaults (sampler.h:39-52)."""
+    si = SamplerInfo()
+    si.type, si.in_attribute, si.linear, si.offset = type_, in_attribute, linear, offset
+    si.wrap[:] = wrap
+    si.size[:] = size
+    si.in_transform[:] = in_transform
+    si.in_offset[:] = in_offset
+    si.out_transform[:] = out_transform
+    si.out_offset[:] = out_offset
+    return si
 
 
 def default_material():
@@ -234,6 +331,10 @@ def default_material():
     m.alpha_cutoff = 0.5
     for k in ("base_color_tex", "normal_tex", "roughness_tex", "metallic_tex", "emission_tex", "alpha_tex"):
         setattr(m, k, -1)
+    for k in _SOURCES:
+        getattr(m, k).kind, getattr(m, k).index = PHX_SOURCE_CONSTANT, 0
+    m.base_color_alpha = 0
+    m.roughness_src_squared = 0
     return m
 
 

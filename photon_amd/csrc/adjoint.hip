@@ -72,7 +72,8 @@ PHX_DEV void grad_add(double *lds, uint32_t lds_mats, double *global, uint32_t m
 // Routes one vertex's parameter gradient: base colour to the material or, for
 // a texture-mapped base colour, to the four texels the lookup blends;
 // roughness and metallic to the material while they lie inside the (0, 1)
-// range resolve_material clamps them to.
+// range resolve_material clamps them to.  A parameter that comes from a
+// vertex colour, an attribute or a sampler is not the material's: it gets none.
 PHX_DEV void grad_route(const float g[kGradParams], const AdjHooks &h, const DevScene &s, const AdjointArgs &a,
                         double *lds) {
   const phx_material &raw = s.materials[h.mid];
@@ -89,12 +90,14 @@ PHX_DEV void grad_route(const float g[kGradParams], const AdjHooks &h, const Dev
               unsafeAtomicAdd(&a.grad_tex[size_t(t.idx[j]) * 3 + c], double(g[c] * t.w[j]));
       }
     }
-  } else if (!h.has_color) {
+  } else if (!h.has_color && raw.base_color_src.kind == PHX_SOURCE_CONSTANT) {
     for (int c = 0; c < 3; ++c) grad_add(lds, a.lds_mats, a.grad_mat, h.mid, c, g[c]);
   }
-  if (!(raw.roughness_tex >= 0 && tex) && raw.roughness > 0.f && raw.roughness < 1.f)
+  if (!(raw.roughness_tex >= 0 && tex) && raw.roughness_src.kind == PHX_SOURCE_CONSTANT && raw.roughness > 0.f &&
+      raw.roughness < 1.f)
     grad_add(lds, a.lds_mats, a.grad_mat, h.mid, 3, g[3]);
-  if (!(raw.metallic_tex >= 0 && tex) && raw.metallic > 0.f && raw.metallic < 1.f)
+  if (!(raw.metallic_tex >= 0 && tex) && raw.metallic_src.kind == PHX_SOURCE_CONSTANT && raw.metallic > 0.f &&
+      raw.metallic < 1.f)
     grad_add(lds, a.lds_mats, a.grad_mat, h.mid, 4, g[4]);
 }
 
@@ -103,7 +106,7 @@ PHX_DEV void grad_route(const float g[kGradParams], const AdjHooks &h, const Dev
 // first lds_mats materials, flushed once), continuing paths to the next queue.
 // The work-group steps through tiles of B rays together: one striped fetch
 // and one slot claim per tile, not one per wave.
-template <class P, int B>
+template <class P, int B, int TIER>
 __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, AdjointArgs a) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[ray_count_word(it)];
@@ -136,11 +139,12 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
       g = stv.x;
       const RunItem &r = v.item_at(stv.y >> kItemShift);
       const DevScene &s = v.scene_of(r);
-      const PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], stv);
+      PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], stv);
       const float4 hit = q.hit[i];
       const uint32_t htri = hit_tri(__float_as_uint(hit.w));
       const uint32_t l = g - r.first;
       const uint32_t pixel = l % r.sp.npix;
+      ps.pixel = pixel;
       const uint32_t sample = r.sample_base + l / r.sp.npix;
       AdjHooks h;
       h.nee_on = false;
@@ -148,7 +152,7 @@ __global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, Adjoin
       const float *dp = a.adjoint + size_t(r.pix_base + pixel) * 3;
       h.delta = vec3{dp[0], dp[1], dp[2]} * (1.f / float(r.sp.spp));
       Rng rng = make_rng(pixel, sample, 1u + ps.path_it);
-      shade_path(s, r.sp, ps, htri != 0xffffffffu, htri, hit.x, hit.y, hit.z, rng, out, h);
+      shade_path<AdjHooks, false, TIER>(s, r.sp, ps, htri != 0xffffffffu, htri, hit.x, hit.y, hit.z, rng, out, h);
       const float4 Lr = q.sample_L[g];
       vec3 rem = xyz(Lr) - out.add;
       float gv[kGradParams];
@@ -224,10 +228,13 @@ hipError_t launch_shade_adjoint(const ViewArgs &v, const StreamQueues &q, const 
                                 hipStream_t st) {
   auto go = [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
-    if (ls.trace_block == 128)
-      hipLaunchKernelGGL((k_shade_adjoint<P, 128>), dim3(ls.trace_grid), dim3(128), 0, st, p, q, a);
-    else
-      hipLaunchKernelGGL((k_shade_adjoint<P, 256>), dim3(ls.trace_grid), dim3(256), 0, st, p, q, a);
+    // runs of the lean tier replay with the standard tier's code (one fewer instantiation)
+    const bool full = ls.tier == kTierFull;
+    const dim3 g(ls.trace_grid);
+    if (ls.trace_block == 128 && full) hipLaunchKernelGGL((k_shade_adjoint<P, 128, kTierFull>), g, dim3(128), 0, st, p, q, a);
+    else if (ls.trace_block == 128) hipLaunchKernelGGL((k_shade_adjoint<P, 128, kTierStandard>), g, dim3(128), 0, st, p, q, a);
+    else if (full) hipLaunchKernelGGL((k_shade_adjoint<P, 256, kTierFull>), g, dim3(256), 0, st, p, q, a);
+    else hipLaunchKernelGGL((k_shade_adjoint<P, 256, kTierStandard>), g, dim3(256), 0, st, p, q, a);
     return hipGetLastError();
   };
   switch (v.kind) {

@@ -213,6 +213,23 @@ int phx_bridge_render(phx_context *c, const phx_camera_data *cd, const phx_rende
   p.clamp_indirect_sum = 3.f * std::max(rs->clamp_indirect, 0.f);
   p.bottom_up = 1;
   for (int i = 0; i < 4; ++i) p.image_region[i] = cd->image_region[i];
+  // Background image (RenderBridge.cpp:846-869): a device copy, uploaded again
+  // when the caller's image changes.
+  const size_t bg_n = size_t(rs->background_width) * rs->background_height * 4;
+  if (rs->background_image && bg_n > 0) {
+    if (c->b_bg_host.size() != bg_n || c->b_bg_w != rs->background_width ||
+        std::memcmp(c->b_bg_host.data(), rs->background_image, bg_n * sizeof(float)) != 0) {
+      c->b_bg_host.assign(rs->background_image, rs->background_image + bg_n);
+      c->b_bg_w = rs->background_width;
+      HIP_TRY(c, c->b_bg.reserve(bg_n * sizeof(float)));
+      HIP_TRY(c, hipMemcpy(c->b_bg.p, c->b_bg_host.data(), bg_n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    p.background_image = c->b_bg.as<float>();
+    p.background_width = rs->background_width;
+    p.background_height = rs->background_height;
+  } else {
+    c->b_bg_host.clear();
+  }
   if ((r = phx_render(c, &cam, &p))) return r;
 
   const uint32_t spp = p.samples_per_pixel;

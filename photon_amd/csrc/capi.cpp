@@ -24,10 +24,18 @@ using namespace phx_detail;
 
 float length3(const float *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
 
-// Emissive-triangle test of the render bridge (material.h:97-103).
+// Emissive-triangle test of the render bridge (material_may_emit,
+// material.h:97-103): also true when the emission has a source.
 bool may_emit(const phx_material &m) {
   if (m.emission_strength <= 0.f) return false;
-  return m.emission_tex >= 0 || m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f;
+  return m.emission_src.kind != PHX_SOURCE_CONSTANT || m.emission_tex >= 0 || m.emission[0] > 0.f ||
+         m.emission[1] > 0.f || m.emission[2] > 0.f;
+}
+bool has_source(const phx_material &m) {
+  for (const phx_param_source &p : {m.base_color_src, m.alpha_src, m.metallic_src, m.roughness_src, m.emission_src,
+                                    m.transmission_src, m.normal_src})
+    if (p.kind != PHX_SOURCE_CONSTANT) return true;
+  return false;
 }
 
 // Marginal/conditional CDFs of an equirect map (environment_map.cpp:8-65),
@@ -88,6 +96,74 @@ void cdf_guide(const float *cdf, uint32_t n, uint32_t *guide) {
 bool ray_args_ok(const float *a, const float *b, const float *c, const float *d) { return a && b && c && d; }
 
 }  // namespace
+
+namespace phx_detail {
+
+// pt::Material{} (material.h:35-82).
+phx_material default_material() {
+  phx_material m{};
+  m.base_color[0] = m.base_color[1] = m.base_color[2] = 0.8f;
+  m.roughness = 0.5f;
+  m.ior = 1.5f;
+  m.specular = 0.5f;
+  m.clearcoat_roughness = 0.03f;
+  m.sheen_tint = 0.5f;
+  m.subsurface_color[0] = m.subsurface_color[1] = m.subsurface_color[2] = 1.f;
+  m.subsurface_radius = 1.f;
+  m.alpha = 1.f;
+  m.base_color_tex = m.normal_tex = m.roughness_tex = m.metallic_tex = m.emission_tex = m.alpha_tex = -1;
+  m.alpha_mode = PHX_ALPHA_BLEND;
+  m.alpha_cutoff = 0.5f;
+  return m;
+}
+
+// convert_source / convert_material (RenderBridge.cpp:314-370).
+phx_material convert_bridge_material(const phx_bridge_material &in) {
+  auto source = [](phx_param_source p) {
+    phx_param_source o{};
+    o.kind = p.kind == 1 ? PHX_SOURCE_ATTRIBUTE : (p.kind == 2 ? PHX_SOURCE_SAMPLER : PHX_SOURCE_CONSTANT);
+    o.index = p.index;
+    return o;
+  };
+  phx_material m = default_material();
+  for (int k = 0; k < 3; ++k) {
+    m.base_color[k] = in.base_color[k];
+    m.emission[k] = in.emission[k];
+  }
+  m.metallic = in.metallic;
+  m.roughness = in.roughness * in.roughness;  // the BSDF takes the GGX alpha
+  m.ior = in.ior;
+  m.transmission = in.transmission;
+  m.specular = 0.5f * in.specular;  // 0.5 is the full dielectric reflectance
+  m.clearcoat = in.clearcoat;
+  m.clearcoat_roughness = in.clearcoat_roughness;
+  m.sheen = in.sheen;
+  m.emission_strength = 1.f;
+  m.alpha = in.opacity;
+  m.base_color_src = source(in.base_color_src);
+  m.alpha_src = source(in.opacity_src);
+  m.metallic_src = source(in.metallic_src);
+  m.roughness_src = source(in.roughness_src);
+  m.emission_src = source(in.emission_src);
+  m.transmission_src = source(in.transmission_src);
+  m.normal_src = source(in.normal_src);
+  m.roughness_src_squared = 1;
+  m.base_color_alpha = in.base_color_alpha ? 1 : 0;
+  if (m.alpha_src.kind != PHX_SOURCE_CONSTANT) m.alpha = 1.f;  // a sourced opacity scales the constant
+  m.alpha_mode = in.alpha_mode == 0 ? PHX_ALPHA_BLEND : (in.alpha_mode == 2 ? PHX_ALPHA_MASK : PHX_ALPHA_OPAQUE);
+  m.alpha_cutoff = in.alpha_cutoff;
+  m.thin = in.thin_walled ? 1 : 0;
+  m.flags = PHX_MATERIAL_FLAG_REFRACTIVE;
+  if (in.emitter_only) m.flags |= PHX_MATERIAL_FLAG_EMITTER_ONLY;
+  if (in.hidden_to_camera) m.flags |= PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA;
+  if (in.emission_side == 2)
+    m.flags |= PHX_MATERIAL_FLAG_EMIT_BOTH_SIDES;
+  else if (in.emission_side == 1)
+    m.flags |= PHX_MATERIAL_FLAG_EMIT_BACK_ONLY;
+  return m;
+}
+
+}  // namespace phx_detail
 
 extern "C" {
 
@@ -411,12 +487,58 @@ int phx_set_scene(phx_context *c, const phx_scene_desc *d) {
   s.tex_pixels = c->tex_pixels.as<float>();
   s.tex_infos = c->tex_infos.as<phx_texture_info>();
 
+  // Vertex attributes and samplers of the parameter sources.
+  for (int k = 0; k < PHX_ATTRIBUTE_MESH_COUNT; ++k) {
+    const int32_t off = d->attributes ? d->attribute_offset[k] : -1;
+    if (off >= 0 && (!d->attributes || uint64_t(off) + c->vertex_count > d->attribute_count))
+      return fail(c, PHX_ERR_INVALID, "set_scene: attribute slot outside the attribute records");
+    s.attr_offset[k] = off >= 0 ? off : -1;
+  }
+  if ((r = upload(c, c->attributes, reinterpret_cast<const float4 *>(d->attributes),
+                  d->attributes ? d->attribute_count : 0)))
+    return r;
+  s.attributes = c->attributes.as<float4>();
+  s.sampler_count = 0;
+  if (d->sampler_count && d->samplers) {
+    for (uint32_t i = 0; i < d->sampler_count; ++i) {
+      const phx_sampler_info &si = d->samplers[i];
+      if (si.type > PHX_SAMPLER_TRANSFORM) return fail(c, PHX_ERR_INVALID, "set_scene: unknown sampler type");
+      if (si.type == PHX_SAMPLER_TRANSFORM) continue;
+      uint64_t n = 1;
+      for (int k = 0; k < 3; ++k) {
+        if (si.size[k] == 0 || si.wrap[k] > PHX_WRAP_MIRROR)
+          return fail(c, PHX_ERR_INVALID, "set_scene: sampler size 0 or unknown wrap mode");
+        n *= si.size[k];
+      }
+      if (!d->sampler_texels || uint64_t(si.offset) + n > d->sampler_texel_count)
+        return fail(c, PHX_ERR_INVALID, "set_scene: sampler image outside the sampler texels");
+    }
+    if ((r = upload(c, c->samplers, d->samplers, d->sampler_count))) return r;
+    if ((r = upload(c, c->sampler_texels, reinterpret_cast<const float4 *>(d->sampler_texels),
+                    d->sampler_texels ? d->sampler_texel_count : 0)))
+      return r;
+    s.sampler_count = d->sampler_count;
+  } else {
+    c->samplers.release();
+    c->sampler_texels.release();
+  }
+  s.samplers = c->samplers.as<phx_sampler_info>();
+  s.sampler_texels = c->sampler_texels.as<float4>();
+  s.has_sources = 0;
+  for (const phx_material &m : mats)
+    if (has_source(m)) s.has_sources = 1;
+
   // Volumes.
   s.vol_count = 0;
   s.vol_tf_count = 0;
   if (d->volume_count && d->volumes) {
     for (uint32_t i = 0; i < d->volume_count; ++i) {
       const phx_volume &v = d->volumes[i];
+      // tracking skips a volume without a majorant or table and reads none of its data
+      if (!(v.majorant > 0.f) || v.tf_size == 0) continue;
+      for (int k = 0; k < 3; ++k)
+        if (v.dims[k] == 0 || v.macro_dims[k] == 0)
+          return fail(c, PHX_ERR_INVALID, "set_scene: volume with an empty grid or macro-cell grid");
       const size_t cells = size_t(v.dims[0]) * v.dims[1] * v.dims[2];
       if (v.data_offset + cells > d->volume_value_count || v.tf_offset + v.tf_size > d->volume_tf_count)
         return fail(c, PHX_ERR_INVALID, "set_scene: volume data out of range");
@@ -497,45 +619,7 @@ int phx_trace_occluded(phx_context *c, const float *o, const float *d, const flo
 
 int phx_convert_material(const phx_bridge_material *in, phx_material *out) {
   if (!in || !out) return PHX_ERR_INVALID;
-  // The parameter sources (attributes, samplers) are not part of this
-  // library's material; only constants convert exactly.
-  for (int32_t kind : {in->base_color_src, in->opacity_src, in->metallic_src, in->roughness_src, in->emission_src,
-                       in->transmission_src, in->normal_src})
-    if (kind != 0) return PHX_ERR_UNSUPPORTED;
-  phx_material m{};
-  // pt::Material defaults of the fields convert_material does not set
-  m.sheen_tint = 0.5f;
-  m.specular_tint = 0.f;
-  m.anisotropic = 0.f;
-  m.subsurface = 0.f;
-  m.subsurface_color[0] = m.subsurface_color[1] = m.subsurface_color[2] = 1.f;
-  m.subsurface_radius = 1.f;
-  m.base_color_tex = m.normal_tex = m.roughness_tex = m.metallic_tex = m.emission_tex = m.alpha_tex = -1;
-  for (int k = 0; k < 3; ++k) {
-    m.base_color[k] = in->base_color[k];
-    m.emission[k] = in->emission[k];
-  }
-  m.metallic = in->metallic;
-  m.roughness = in->roughness * in->roughness;
-  m.ior = in->ior;
-  m.transmission = in->transmission;
-  m.specular = 0.5f * in->specular;
-  m.clearcoat = in->clearcoat;
-  m.clearcoat_roughness = in->clearcoat_roughness;
-  m.sheen = in->sheen;
-  m.emission_strength = 1.f;
-  m.alpha = in->opacity;
-  m.alpha_mode = in->alpha_mode == 0 ? PHX_ALPHA_BLEND : (in->alpha_mode == 2 ? PHX_ALPHA_MASK : PHX_ALPHA_OPAQUE);
-  m.alpha_cutoff = in->alpha_cutoff;
-  m.thin = in->thin_walled ? 1 : 0;
-  m.flags = PHX_MATERIAL_FLAG_REFRACTIVE;
-  if (in->emitter_only) m.flags |= PHX_MATERIAL_FLAG_EMITTER_ONLY;
-  if (in->hidden_to_camera) m.flags |= PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA;
-  if (in->emission_side == 2)
-    m.flags |= PHX_MATERIAL_FLAG_EMIT_BOTH_SIDES;
-  else if (in->emission_side == 1)
-    m.flags |= PHX_MATERIAL_FLAG_EMIT_BACK_ONLY;
-  *out = m;
+  *out = convert_bridge_material(*in);
   return PHX_OK;
 }
 

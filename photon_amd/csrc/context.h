@@ -70,7 +70,7 @@ struct phx_context {
   // shading data
   bool have_scene = false;
   DBuf materials, lights, em_ids, em_cdf, em_guide, em_recs, prim_light_pdf, env_pixels, env_marginal, env_cond, env_guide_m, env_guide_c,
-      tex_pixels, tex_infos,
+      tex_pixels, tex_infos, attributes, samplers, sampler_texels,
       volumes, vol_values, vol_tf, majorants;
   phx::DevScene dev{};
   uint32_t tex_pixel_count = 0;  // texels of the atlas
@@ -104,8 +104,10 @@ struct phx_context {
   std::vector<unsigned char> graph_key;
 
   // render bridge (seam B)
-  DBuf b_color, b_alpha, b_depth, b_normal, b_albedo;
+  DBuf b_color, b_alpha, b_depth, b_normal, b_albedo, b_bg;
   uint32_t b_w = 0, b_h = 0, b_samples = 0, b_counter = 0;
+  std::vector<float> b_bg_host;  // the background image last uploaded to b_bg
+  uint32_t b_bg_w = 0;
 
   // reference-structure baseline (proxy.hip): host mesh copy, its median-split
   // BVH (built on first use) and full-width per-slot state
@@ -119,6 +121,9 @@ struct phx_context {
 };
 
 namespace phx_detail {
+
+phx_material default_material();
+phx_material convert_bridge_material(const phx_bridge_material &in);
 
 inline int fail(phx_context *c, int code, const std::string &m) {
   if (c) c->error = m;

@@ -92,6 +92,12 @@ struct DevScene {
   const float *vol_values;
   const float4 *vol_tf;
   const float *majorants;
+  const float4 *attributes;  // per-vertex RGBA records of the attribute slots
+  const phx_sampler_info *samplers;
+  const float4 *sampler_texels;
+  int32_t attr_offset[PHX_ATTRIBUTE_MESH_COUNT];  // first record of each slot, -1: absent
+  uint32_t sampler_count;
+  uint32_t has_sources;  // a material takes a parameter from an attribute or a sampler
   uint32_t material_count;
   uint32_t light_count;
   uint32_t em_count;
@@ -303,28 +309,195 @@ PHX_DEV TexTaps tex_taps(const DevScene &s, int id, float u, float v) {
   return t;
 }
 
-// Material at a hit with atlas textures applied (pathtracer.cpp:186-247).
-// TEX = false leaves the texture lookups out of the code (for scenes without
-// textures, where they never run).
-template <bool TEX = true>
-PHX_DEV phx_material resolve_material(const DevScene &s, uint32_t mid, vec2 uv, bool has_color, vec3 color,
-                                      float &alpha) {
-  phx_material m = s.materials[mid];
+// A point of the scene mesh as shading sees it (SurfaceQuery,
+// pathtracer.cpp:95-108): primitive and material, barycentrics (b1, b2) of
+// the second and third vertex, position, geometric normal, the shading normal
+// (interpolated, flat where degenerate, on the geometric normal's side,
+// :906-910), texture coordinates and vertex colour.
+struct SurfaceHit {
+  vec3 p, ng, sn;
+  vec2 uv;
+  vec3 color;
+  bool has_color;
+  uint32_t prim, mid;
+  float b1, b2;
+};
+
+// ---- attributes and samplers (sampler.h:25-139, pathtracer.cpp:111-132) ------
+PHX_DEV float4 make_rgba(vec3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+// Attribute `slot` at the point: the world position or shading normal, or the
+// per-vertex records of a mesh slot blended by the barycentrics
+// (a0 * (1 - b1 - b2) + a1 * b1 + a2 * b2); (0, 0, 0, 1) for an absent slot.
+PHX_DEV float4 fetch_attribute(const DevScene &s, int slot, const SurfaceHit &q) {
+  if (slot == PHX_ATTRIBUTE_WORLD_POSITION) return make_rgba(q.p, 1.f);
+  if (slot == PHX_ATTRIBUTE_WORLD_NORMAL) return make_rgba(q.sn, 1.f);
+  // the slot's first record by selects: an index computed at run time into the
+  // scene's array would put the scene in scratch memory
+  const int off = slot == 0 ? s.attr_offset[0]
+                : slot == 1 ? s.attr_offset[1]
+                : slot == 2 ? s.attr_offset[2]
+                : slot == 3 ? s.attr_offset[3]
+                : slot == 4 ? s.attr_offset[4] : -1;
+  if (off < 0) return make_float4(0.f, 0.f, 0.f, 1.f);
+  const float4 *rec = s.attributes + off;
+  const float4 a0 = rec[s.indices[q.prim * 3]], a1 = rec[s.indices[q.prim * 3 + 1]], a2 = rec[s.indices[q.prim * 3 + 2]];
+  const float w0 = 1.f - q.b1 - q.b2;
+  return make_float4(a0.x * w0 + a1.x * q.b1 + a2.x * q.b2, a0.y * w0 + a1.y * q.b1 + a2.y * q.b2,
+                     a0.z * w0 + a1.z * q.b1 + a2.z * q.b2, a0.w * w0 + a1.w * q.b1 + a2.w * q.b2);
+}
+// Texel index of a coordinate outside [0, n) under a wrap mode (sampler.h:62-71).
+PHX_DEV int wrap_texel(int i, int n, uint32_t mode) {
+  if (mode == PHX_WRAP_REPEAT) return ((i % n) + n) % n;
+  if (mode == PHX_WRAP_MIRROR) {
+    const int j = ((i % (2 * n)) + 2 * n) % (2 * n);
+    return j < n ? j : 2 * n - 1 - j;
+  }
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+// Row-major 4x4 matrix times a column vector, plus an offset.
+PHX_DEV float4 affine4(const float *m, const float *o, float4 v) {
+  return make_float4(m[0] * v.x + m[1] * v.y + m[2] * v.z + m[3] * v.w + o[0],
+                     m[4] * v.x + m[5] * v.y + m[6] * v.z + m[7] * v.w + o[1],
+                     m[8] * v.x + m[9] * v.y + m[10] * v.z + m[11] * v.w + o[2],
+                     m[12] * v.x + m[13] * v.y + m[14] * v.z + m[15] * v.w + o[3]);
+}
+// SamplerTable::sample (sampler.h:73-138): the input transform, a nearest or
+// multilinear lookup in a 1-3D image (texel rows from coordinate 0), the output
+// transform; (0, 0, 0, 1) for an unknown sampler.
+PHX_DEV float4 sampler_lookup(const DevScene &s, int id, float4 attr) {
+  if (id < 0 || uint32_t(id) >= s.sampler_count) return make_float4(0.f, 0.f, 0.f, 1.f);
+  const phx_sampler_info &si = s.samplers[id];
+  float4 r = affine4(si.in_transform, si.in_offset, attr);
+  if (si.type != PHX_SAMPLER_TRANSFORM) {
+    const int dims = si.type == PHX_SAMPLER_IMAGE1D ? 1 : (si.type == PHX_SAMPLER_IMAGE2D ? 2 : 3);
+    const float coord[3] = {r.x, r.y, r.z};
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    float fr[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d >= dims) continue;
+      const int n = int(si.size[d]);
+      float x = coord[d];
+      if (!(x > -1e6f && x < 1e6f)) x = 0.f;  // keeps the texel finite for degenerate coordinates
+      if (si.linear) {
+        const float f = x * float(n) - 0.5f;
+        const float fl = floorf(f);
+        fr[d] = f - fl;
+        lo[d] = wrap_texel(int(fl), n, si.wrap[d]);
+        hi[d] = wrap_texel(int(fl) + 1, n, si.wrap[d]);
+      } else {
+        lo[d] = hi[d] = wrap_texel(int(floorf(x * float(n))), n, si.wrap[d]);
+      }
+    }
+    const uint32_t sx = si.size[0], sxy = si.size[0] * si.size[1];
+    const float4 *tx = s.sampler_texels + si.offset;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int nz = dims > 2 ? 2 : 1, ny = dims > 1 ? 2 : 1;
+    for (int kz = 0; kz < nz; ++kz) {
+      const float wz = dims > 2 ? (kz ? fr[2] : 1.f - fr[2]) : 1.f;
+      const uint32_t z = uint32_t(kz ? hi[2] : lo[2]);
+      for (int ky = 0; ky < ny; ++ky) {
+        const float wy = dims > 1 ? (ky ? fr[1] : 1.f - fr[1]) : 1.f;
+        const uint32_t row = z * sxy + uint32_t(ky ? hi[1] : lo[1]) * sx;
+        const float4 a = tx[row + uint32_t(lo[0])], b = tx[row + uint32_t(hi[0])];
+        const float w = wy * wz, u0 = 1.f - fr[0], u1 = fr[0];
+        acc.x = acc.x + (a.x * u0 + b.x * u1) * w;
+        acc.y = acc.y + (a.y * u0 + b.y * u1) * w;
+        acc.z = acc.z + (a.z * u0 + b.z * u1) * w;
+        acc.w = acc.w + (a.w * u0 + b.w * u1) * w;
+      }
+    }
+    r = acc;
+  }
+  return affine4(si.out_transform, si.out_offset, r);
+}
+// A non-constant parameter source at the point (pathtracer.cpp:125-133).
+PHX_DEV float4 eval_source(const DevScene &s, phx_param_source src, const SurfaceHit &q) {
+  if (src.kind == PHX_SOURCE_ATTRIBUTE) return fetch_attribute(s, src.index, q);
+  if (src.index < 0 || uint32_t(src.index) >= s.sampler_count) return make_float4(0.f, 0.f, 0.f, 1.f);
+  return sampler_lookup(s, src.index, fetch_attribute(s, s.samplers[src.index].in_attribute, q));
+}
+// Shading normal from a sourced tangent-space normal map (pathtracer.cpp:135-176):
+// the tangent frame is dP/du, dP/dv of the triangle in the coordinates the map
+// is looked up with, made orthogonal to `sn`; `sn` itself where that frame
+// is degenerate.
+PHX_DEV vec3 apply_normal_map(const DevScene &s, phx_param_source src, const SurfaceHit &q, vec3 sn) {
+  const float4 c = eval_source(s, src, q);
+  const vec3 nm{2.f * c.x - 1.f, 2.f * c.y - 1.f, 2.f * c.z - 1.f};
+  int slot = src.index;
+  if (src.kind != PHX_SOURCE_ATTRIBUTE) {
+    if (src.index < 0 || uint32_t(src.index) >= s.sampler_count) return sn;
+    slot = s.samplers[src.index].in_attribute;
+  }
+  SurfaceHit corner = q;
+  corner.b1 = 0.f;
+  corner.b2 = 0.f;
+  const float4 a0 = fetch_attribute(s, slot, corner);
+  corner.b1 = 1.f;
+  const float4 a1 = fetch_attribute(s, slot, corner);
+  corner.b1 = 0.f;
+  corner.b2 = 1.f;
+  const float4 a2 = fetch_attribute(s, slot, corner);
+  const float du1 = a1.x - a0.x, dv1 = a1.y - a0.y, du2 = a2.x - a0.x, dv2 = a2.y - a0.y;
+  const float det = du1 * dv2 - du2 * dv1;
+  if (!(fabsf(det) > 1e-20f)) return sn;
+  const vec3 p0 = ld3(s.positions + size_t(s.indices[q.prim * 3]) * 3);
+  const vec3 e1 = ld3(s.positions + size_t(s.indices[q.prim * 3 + 1]) * 3) - p0;
+  const vec3 e2 = ld3(s.positions + size_t(s.indices[q.prim * 3 + 2]) * 3) - p0;
+  vec3 t = (e1 * dv2 - e2 * dv1) * (1.f / det);
+  vec3 b = (e2 * du1 - e1 * du2) * (1.f / det);
+  t = t - sn * dot(sn, t);
+  b = b - sn * dot(sn, b);
+  if (!(dot(t, t) > 0.f) || !(dot(b, b) > 0.f)) return sn;
+  const vec3 n = normalize(t) * nm.x + normalize(b) * nm.y + sn * nm.z;
+  return dot(n, n) > 0.f ? normalize(n) : sn;
+}
+
+// Material at a surface point with atlas textures and parameter sources
+// applied (resolve_material, pathtracer.cpp:186-247); `alpha` is the
+// probability that a ray interacts with the surface.  TEX = false / SRC =
+// false leave the texture / source lookups out of the code (scenes without
+// textures / sources, where they never run).
+template <bool TEX = true, bool SRC = true>
+PHX_DEV phx_material resolve_material(const DevScene &s, const SurfaceHit &q, float &alpha) {
+  phx_material m = s.materials[q.mid];
   const bool tex = TEX && s.tex_count > 0;
   alpha = m.alpha;
-  if (m.alpha_tex >= 0 && tex) alpha = tex_sample(s, m.alpha_tex, uv.x, uv.y).x;
+  if (m.alpha_tex >= 0 && tex) alpha = tex_sample(s, m.alpha_tex, q.uv.x, q.uv.y).x;
   if (m.base_color_tex >= 0 && tex) {
-    const vec3 c = tex_sample(s, m.base_color_tex, uv.x, uv.y);
+    const vec3 c = tex_sample(s, m.base_color_tex, q.uv.x, q.uv.y);
     m.base_color[0] = c.x;
     m.base_color[1] = c.y;
     m.base_color[2] = c.z;
-  } else if (has_color) {
-    m.base_color[0] = color.x;
-    m.base_color[1] = color.y;
-    m.base_color[2] = color.z;
+  } else if (q.has_color) {
+    m.base_color[0] = q.color.x;
+    m.base_color[1] = q.color.y;
+    m.base_color[2] = q.color.z;
   }
-  if (m.roughness_tex >= 0 && tex) m.roughness = tex_sample(s, m.roughness_tex, uv.x, uv.y).x;
-  if (m.metallic_tex >= 0 && tex) m.metallic = tex_sample(s, m.metallic_tex, uv.x, uv.y).x;
+  if (m.roughness_tex >= 0 && tex) m.roughness = tex_sample(s, m.roughness_tex, q.uv.x, q.uv.y).x;
+  if (m.metallic_tex >= 0 && tex) m.metallic = tex_sample(s, m.metallic_tex, q.uv.x, q.uv.y).x;
+  if (SRC && s.has_sources) {
+    if (m.base_color_src.kind != PHX_SOURCE_CONSTANT) {
+      const float4 c = eval_source(s, m.base_color_src, q);
+      m.base_color[0] = fmaxf(c.x, 0.f);
+      m.base_color[1] = fmaxf(c.y, 0.f);
+      m.base_color[2] = fmaxf(c.z, 0.f);
+      if (m.base_color_alpha) alpha *= c.w;
+    }
+    if (m.alpha_src.kind != PHX_SOURCE_CONSTANT) alpha *= eval_source(s, m.alpha_src, q).x;
+    if (m.metallic_src.kind != PHX_SOURCE_CONSTANT) m.metallic = eval_source(s, m.metallic_src, q).x;
+    if (m.roughness_src.kind != PHX_SOURCE_CONSTANT) {
+      const float r = clamp01(eval_source(s, m.roughness_src, q).x);
+      m.roughness = m.roughness_src_squared ? r * r : r;
+    }
+    if (m.transmission_src.kind != PHX_SOURCE_CONSTANT) m.transmission = eval_source(s, m.transmission_src, q).x;
+    if (m.emission_src.kind != PHX_SOURCE_CONSTANT) {
+      const float4 e = eval_source(s, m.emission_src, q);
+      m.emission[0] = fmaxf(e.x, 0.f);
+      m.emission[1] = fmaxf(e.y, 0.f);
+      m.emission[2] = fmaxf(e.z, 0.f);
+    }
+  }
   m.metallic = clamp01(m.metallic);
   m.roughness = clamp01(m.roughness);
   m.transmission = clamp01(m.transmission);
@@ -334,20 +507,33 @@ PHX_DEV phx_material resolve_material(const DevScene &s, uint32_t mid, vec2 uv, 
   return m;
 }
 PHX_DEV vec3 material_emission(const phx_material &m) { return ld3(m.emission) * m.emission_strength; }
+// material.h:97-103: also true when the emission comes from a source.
 PHX_DEV bool material_may_emit(const phx_material &m) {
   if (m.emission_strength <= 0.f) return false;
-  return m.emission_tex >= 0 || m.emission[0] > 0.f || m.emission[1] > 0.f || m.emission[2] > 0.f;
+  return m.emission_src.kind != PHX_SOURCE_CONSTANT || m.emission_tex >= 0 || m.emission[0] > 0.f ||
+         m.emission[1] > 0.f || m.emission[2] > 0.f;
 }
 PHX_DEV bool emits_on_side(const phx_material &m, bool front) {
   if (m.flags & PHX_MATERIAL_FLAG_EMIT_BOTH_SIDES) return true;
   const bool back_only = (m.flags & PHX_MATERIAL_FLAG_EMIT_BACK_ONLY) != 0;
   return front ? !back_only : back_only;
 }
+// The emitter's radiance depends on the sampled point: its emission or alpha
+// comes from a parameter source.
+PHX_DEV bool emission_varies(const phx_material &m) {
+  return m.emission_src.kind != PHX_SOURCE_CONSTANT || m.alpha_src.kind != PHX_SOURCE_CONSTANT ||
+         (m.base_color_src.kind != PHX_SOURCE_CONSTANT && m.base_color_alpha);
+}
 
+// Bit of the emitter record's side mask: the radiance is resolved per sample.
+constexpr uint32_t kEmitVaries = 4u;
 // The emitter record of emissive-CDF entry i (four float4): the triangle's
-// vertices with its area pdf (pathtracer.cpp:411-503) and the emitter's
-// radiance, emission times alpha of its material resolved at uv (0, 0), with
-// bit 0 / bit 1 of r3.y set when it emits toward its front / back side.
+// vertices with its area pdf (pathtracer.cpp:411-503); the emitter's radiance,
+// emission times alpha of its material resolved at uv (0, 0) as
+// sample_direct does (:325-345), with bit 0 / bit 1 of r3.y set when it emits
+// toward its front / back side; the primitive and material in r3.z, r3.w.
+// When the radiance varies over the triangle (kEmitVaries in r3.y),
+// sample_direct resolves the material at the sampled point instead.
 PHX_DEV void em_record(const DevScene &s, uint32_t i, float4 *out) {
   const uint32_t prim = s.em_ids[i];
   const float pa = s.prim_light_pdf[prim];
@@ -355,14 +541,19 @@ PHX_DEV void em_record(const DevScene &s, uint32_t i, float4 *out) {
   const vec3 v1 = ld3(s.positions + size_t(s.indices[prim * 3 + 1]) * 3);
   const vec3 v2 = ld3(s.positions + size_t(s.indices[prim * 3 + 2]) * 3);
   float alpha = 1.f;
-  const uint32_t mid = s.material_ids ? s.material_ids[prim] : 0u;
-  const phx_material em = resolve_material(s, mid, vec2{0.f, 0.f}, false, vec3{0.f, 0.f, 0.f}, alpha);
+  SurfaceHit q{};
+  q.prim = prim;
+  q.mid = s.material_ids ? s.material_ids[prim] : 0u;
+  q.p = v0;
+  q.ng = q.sn = normalize(cross(v1 - v0, v2 - v0));
+  const phx_material em = resolve_material(s, q, alpha);
   const vec3 L = material_emission(em) * alpha;
-  const uint32_t sides = (emits_on_side(em, true) ? 1u : 0u) | (emits_on_side(em, false) ? 2u : 0u);
+  const uint32_t sides = (emits_on_side(em, true) ? 1u : 0u) | (emits_on_side(em, false) ? 2u : 0u) |
+                         (emission_varies(em) ? kEmitVaries : 0u);
   out[0] = make_float4(v0.x, v0.y, v0.z, pa);
   out[1] = make_float4(v1.x, v1.y, v1.z, L.x);
   out[2] = make_float4(v2.x, v2.y, v2.z, L.y);
-  out[3] = make_float4(L.z, __uint_as_float(sides), 0.f, 0.f);
+  out[3] = make_float4(L.z, __uint_as_float(sides), __uint_as_float(prim), __uint_as_float(q.mid));
 }
 
 // ---- light strategy pick (pathtracer.cpp:270-367) ---------------------------
@@ -371,9 +562,10 @@ struct Direct {
   float pdf, dist;
   bool mis, valid;
 };
-// LIGHTS = false leaves the analytic lights out of the code (for scenes
-// without them, where that branch never runs).
-template <bool LIGHTS = true>
+// LIGHTS = false leaves the analytic lights, SRC = false the per-sample
+// resolve of emitters with sourced radiance out of the code (for scenes
+// without them, where those branches never run).
+template <bool LIGHTS = true, bool SRC = true>
 PHX_DEV Direct sample_direct(const DevScene &s, vec3 p, Rng &rng) {
   Direct ds{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, 0.f, false, false};
   const uint32_t nstrat = s.strategy_count;
@@ -420,9 +612,26 @@ PHX_DEV Direct sample_direct(const DevScene &s, vec3 p, Rng &rng) {
     const vec3 ln = normalize(cross(v1 - v0, v2 - v0));
     const float cl = -dot(wi, ln);
     if (fabsf(cl) < 1e-6f) return ds;
-    if (!(__float_as_uint(r3.y) & (cl > 0.f ? 1u : 2u))) return ds;
+    const uint32_t sides = __float_as_uint(r3.y);
+    vec3 L{r1.w, r2.w, r3.x};
+    if (SRC && (sides & kEmitVaries)) {
+      // the emitter's material at the sampled point (pathtracer.cpp:325-345)
+      SurfaceHit q{};
+      q.prim = __float_as_uint(r3.z);
+      q.mid = __float_as_uint(r3.w);
+      q.b1 = b1;
+      q.b2 = b2;
+      q.p = lp;
+      q.ng = q.sn = ln;
+      float alpha = 1.f;
+      const phx_material em = resolve_material(s, q, alpha);
+      if (!emits_on_side(em, cl > 0.f)) return ds;
+      L = material_emission(em) * alpha;
+    } else if (!(sides & (cl > 0.f ? 1u : 2u))) {
+      return ds;
+    }
     ds.wi = wi;
-    ds.L = vec3{r1.w, r2.w, r3.x};
+    ds.L = L;
     ds.pdf = pa * d2 / fabsf(cl) * pick_pdf;
     ds.dist = dist;
     ds.mis = true;

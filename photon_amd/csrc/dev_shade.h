@@ -25,6 +25,8 @@ struct ShadeParams {
   vec3 ambient;
   float throughput_clamp, clamp_direct, clamp_indirect;
   int32_t max_diffuse, max_glossy, max_transmission, max_volume;
+  const float4 *bg_image;  // RGBA rows from the bottom of the frame, or null
+  uint32_t bg_w, bg_h;
 };
 
 struct PathState {
@@ -38,6 +40,7 @@ struct PathState {
   bool primary;
   float path_dist;
   uint32_t item;  // the path's item (view, spp range) in the run's item table
+  uint32_t pixel; // the path's pixel in its view (set by the caller; not queued)
 };
 
 struct ShadeOut {
@@ -66,13 +69,6 @@ PHX_DEV vec3 clamp_contribution(vec3 c, int light_bounce, float ds, float is) {
 }
 
 // Surface attributes at a hit (mesh_intersector.h:126-178).
-struct SurfaceHit {
-  vec3 p, ng, sn;
-  vec2 uv;
-  vec3 color;
-  bool has_color;
-  uint32_t prim, mid;
-};
 PHX_DEV SurfaceHit surface_at(const DevScene &s, uint32_t tri, float t, float b1, float b2, vec3 ro, vec3 rd) {
   SurfaceHit h;
   const TriRec tr = s.tris[tri];
@@ -101,10 +97,31 @@ PHX_DEV SurfaceHit surface_at(const DevScene &s, uint32_t tri, float t, float b1
     h.has_color = true;
   }
   h.mid = ta.w;  // the material id (0 without material ids), as material_ids[prim]
+  h.b1 = b1;
+  h.b2 = b2;
   // pathtracer.cpp:906-910
   if (!(dot(h.sn, h.sn) > 0.25f)) h.sn = h.ng;
   else if (dot(h.sn, h.ng) < 0.f) h.sn = h.sn * -1.f;
   return h;
+}
+
+// Bilinear lookup in the background image at image-plane position (bs, bt),
+// edge texels clamped (sample_background_image, pathtracer.cpp:370-389).
+PHX_DEV float4 background_at(const ShadeParams &w, float bs, float bt) {
+  const int W = int(w.bg_w), H = int(w.bg_h);
+  const float fx = bs * float(W) - 0.5f, fy = bt * float(H) - 0.5f;
+  const float flx = floorf(fx), fly = floorf(fy);
+  const float tx = fx - flx, ty = fy - fly;
+  auto cl = [](int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); };
+  const int x0 = cl(int(flx), W), x1 = cl(int(flx) + 1, W), y0 = cl(int(fly), H), y1 = cl(int(fly) + 1, H);
+  const float4 p00 = w.bg_image[y0 * W + x0], p01 = w.bg_image[y0 * W + x1];
+  const float4 p10 = w.bg_image[y1 * W + x0], p11 = w.bg_image[y1 * W + x1];
+  const float ux = 1.f - tx, uy = 1.f - ty;
+  const float4 a = make_float4(p00.x * ux + p01.x * tx, p00.y * ux + p01.y * tx, p00.z * ux + p01.z * tx,
+                               p00.w * ux + p01.w * tx);
+  const float4 b = make_float4(p10.x * ux + p11.x * tx, p10.y * ux + p11.y * tx, p10.z * ux + p11.z * tx,
+                               p10.w * ux + p11.w * tx);
+  return make_float4(a.x * uy + b.x * ty, a.y * uy + b.y * ty, a.z * uy + b.z * ty, a.w * uy + b.w * ty);
 }
 
 // Observers of the weights that depend on the material parameters, called by
@@ -117,13 +134,18 @@ struct NoHooks {
   PHX_DEV void bsdf(const phx_material &, const SurfaceHit &, vec3, vec3, vec3, vec3, const BsdfSample &) {}
 };
 
+// Shading tiers: the code a run's scenes and views need, so a kernel carries
+// no branch that could not run in it (a smaller kernel keeps more of the rest
+// in registers).  kTierLean: no volumes, textures, parameter sources or
+// analytic lights, no background image; kTierStandard: volumes, textures and
+// analytic lights; kTierFull: also parameter sources (attributes, samplers,
+// sourced normal maps and emission) and the background image.
+constexpr int kTierLean = 0, kTierStandard = 1, kTierFull = 2;
+
 // VOL_PRE: the volume event was sampled ahead (k_volume) and is given by
 // vol_ev (scatter distance or -1, albedo) and vol_rng (the RNG state after its
 // draws); otherwise it is sampled here.  The draws are the same either way.
-// LEAN: the scene has no volumes, no textures and no analytic lights, and
-// the code for them is left out (it would never run); a smaller shading
-// kernel keeps more of the rest in registers.
-template <class Hooks, bool VOL_PRE = false, bool LEAN = false>
+template <class Hooks, bool VOL_PRE = false, int TIER = kTierFull>
 PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState &ps, bool hr_hit, uint32_t htri,
                         float ht, float b1, float b2, Rng &rng, ShadeOut &out, Hooks &hooks,
                         float4 vol_ev = make_float4(-1.f, 0.f, 0.f, 0.f), uint32_t vol_rng = 0u) {
@@ -144,7 +166,8 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
   out.next = ps;
   out.next.path_it = ps.path_it + 1u;
 
-  if (!LEAN && s.vol_count > 0) {
+  constexpr bool STD = TIER >= kTierStandard, FULL = TIER >= kTierFull;
+  if (STD && s.vol_count > 0) {
     // Scattering inside a volume in front of the hit (:789-857).
     vec3 alb{0.f, 0.f, 0.f};
     float ts;
@@ -165,7 +188,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
       if (depth >= w.max_depth || emission_only) return;
       const vec3 tpv = tp * alb;
       const float phase = 0.25f * kInvPi;
-      const Direct ds = sample_direct<!LEAN>(s, p, rng);
+      const Direct ds = sample_direct<STD, FULL>(s, p, rng);
       if (ds.valid) {
         const float wm = ds.mis ? power_heuristic(ds.pdf, phase) : 1.f;
         out.sc = clamp_contribution(tpv * ds.L * (phase * wm / ds.pdf), int(depth), w.clamp_direct, w.clamp_indirect);
@@ -202,6 +225,12 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
       if (s.has_env && s.env_visible) {
         Lm = env_eval(s, rd);
         out.add_alpha += 1.f;
+      } else if (FULL && w.bg_image) {
+        const uint32_t px = ps.pixel % w.width, py = ps.pixel / w.width;
+        const float bs = (float(px) + 0.5f) / float(w.width), bt = (float(py) + 0.5f) / float(w.height);
+        const float4 bg = background_at(w, bs, w.bottom_up ? bt : 1.f - bt);
+        Lm = vec3{bg.x, bg.y, bg.z} * bg.w;
+        out.add_alpha += bg.w;
       } else {
         Lm = w.bg * w.bg_alpha;
         out.add_alpha += w.bg_alpha;
@@ -224,7 +253,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
   const SurfaceHit sh = surface_at(s, htri, ht, b1, b2, ro, rd);
   const vec3 p = sh.p, ng = sh.ng, wo = -rd;
   float alpha = 1.f;
-  const phx_material mat = resolve_material<!LEAN>(s, sh.mid, sh.uv, sh.has_color, sh.color, alpha);
+  const phx_material mat = resolve_material<STD, FULL>(s, sh, alpha);
   const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0 && primary;
   if (hidden || (alpha < 1.f && rng.next() >= alpha)) {
     // Stochastic transparency (:929-951): continue behind the surface.
@@ -255,13 +284,14 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
     n = n * -1.f;
     sn = sn * -1.f;
   }
-  if (!LEAN && mat.normal_tex >= 0 && s.tex_count > 0) {
+  if (STD && mat.normal_tex >= 0 && s.tex_count > 0) {
     vec3 nm = tex_sample(s, mat.normal_tex, sh.uv.x, sh.uv.y);
     nm = nm * 2.f - vec3{1.f, 1.f, 1.f};
     vec3 ta, bi;
     onb(sn, ta, bi);
     sn = normalize(ta * nm.x + bi * nm.y + sn * nm.z);
   }
+  if (FULL && mat.normal_src.kind != PHX_SOURCE_CONSTANT) sn = apply_normal_map(s, mat.normal_src, sh, sn);
   if (primary) {
     primary = false;
     out.add_alpha += 1.f;
@@ -279,7 +309,7 @@ PHX_DEV void shade_path(const DevScene &s, const ShadeParams &w, const PathState
   if ((mat.flags & PHX_MATERIAL_FLAG_EMITTER_ONLY) != 0 || depth >= w.max_depth || emission_only) return;
   {
     // Next event estimation (:1006-1035).
-    const Direct ds = sample_direct<!LEAN>(s, p, rng);
+    const Direct ds = sample_direct<STD, FULL>(s, p, rng);
     if (ds.valid) {
       float bp;
       const vec3 f = bsdf_eval_pdf(mat, wo, ds.wi, n, sn, bp);

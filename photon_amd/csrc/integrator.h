@@ -148,7 +148,7 @@ struct LoopShape {
   uint32_t refill;       // idle lanes of a traversal wave that trigger a refill (1..64)
   uint32_t groups;       // material groups of sorted shading (0: queue order)
   uint32_t volumes;      // a scene of the run holds volumes: k_volume runs before k_shade
-  uint32_t lean;         // no scene of the run has volumes, textures or analytic lights
+  uint32_t tier;         // shading tier (kTierLean/Standard/Full) the run's scenes and views need
 };
 
 // RenderParams -> the shading constants of one frame (render.cpp).

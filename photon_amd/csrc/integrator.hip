@@ -224,7 +224,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_e
   block_sum<B / 64>(&q.stats[kStatClosest], live);
 }
 
-template <class P, bool SORTED, bool LEAN>
+template <class P, bool SORTED, int TIER>
 __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, uint32_t groups, uint32_t it,
                                            uint32_t count, uint32_t tile) {
   uint32_t i = tile + threadIdx.x;
@@ -271,12 +271,13 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
   auto shade_one = [&](uint32_t item) {
     const RunItem &r = v.item_at(item);
     const DevScene &s = v.scene_of(r);
-    const PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], stv);
-    const float4 hit = q.hit[i];
-    const uint32_t htri = hit_tri(__float_as_uint(hit.w));
     const uint32_t l = g - r.first;
     const uint32_t pixel = l % r.sp.npix;
     const uint32_t s_local = l / r.sp.npix;
+    PathState ps = unpack(q.o[buf][i], q.d[buf][i], q.tp[buf][i], stv);
+    ps.pixel = pixel;
+    const float4 hit = q.hit[i];
+    const uint32_t htri = hit_tri(__float_as_uint(hit.w));
     Rng rng = make_rng(pixel, r.sample_base + s_local, 1u + ps.path_it);
     float4 vol_ev = make_float4(-1.f, 0.f, 0.f, 0.f);
     uint32_t vol_rng = 0u;
@@ -285,7 +286,7 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
       vol_rng = q.vol_rng[i];
     }
     NoHooks hooks;
-    shade_path<NoHooks, true, LEAN>(s, r.sp, ps, htri != 0xffffffffu, htri, hit.x, hit.y, hit.z, rng, out, hooks,
+    shade_path<NoHooks, true, TIER>(s, r.sp, ps, htri != 0xffffffffu, htri, hit.x, hit.y, hit.z, rng, out, hooks,
                                     vol_ev, vol_rng);
     if (out.add.x != 0.f || out.add.y != 0.f || out.add.z != 0.f || out.add_alpha != 0.f) {
       const float4 L = q.sample_L[g];
@@ -319,14 +320,14 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
 // shadow rays go to compacted queues, one atomic per work-group.  The grid covers the
 // path slots and work-groups beyond the queue exit at once: a grid-stride loop
 // over the queue measured slower, since the loop raises the spill traffic.
-template <class P, bool SORTED, bool LEAN>
+template <class P, bool SORTED, int TIER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(P v, StreamQueues q,
                                                                                       uint32_t groups) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[ray_count_word(it)];
   const uint32_t tile = blockIdx.x * 256u;
   if (tile >= count) return;
-  shade_tile<P, SORTED, LEAN>(v, q, groups, it, count, tile);
+  shade_tile<P, SORTED, TIER>(v, q, groups, it, count, tile);
 }
 
 // Any hit over the shadow queue (trace_occluded; persistent waves, per-lane
@@ -482,7 +483,7 @@ __global__ __launch_bounds__(256) void k_resolve(DevScene s, ShadeParams w, Stre
         if (!hit) break;
         const SurfaceHit sh = surface_at(s, h.tri, h.t, h.u, h.v, ps.ro, ps.rd);
         float alpha = 1.f;
-        const phx_material mat = resolve_material(s, sh.mid, sh.uv, sh.has_color, sh.color, alpha);
+        const phx_material mat = resolve_material(s, sh, alpha);
         const bool hidden = (mat.flags & PHX_MATERIAL_FLAG_HIDDEN_TO_CAMERA) != 0;
         if (hidden || (alpha < 1.f && rng.next() >= alpha)) {
           path_dist += h.t;
@@ -509,6 +510,7 @@ __global__ __launch_bounds__(256) void k_resolve(DevScene s, ShadeParams w, Stre
           onb(sn, ta, bi);
           sn = normalize(ta * nm.x + bi * nm.y + sn * nm.z);
         }
+        if (mat.normal_src.kind != PHX_SOURCE_CONSTANT) sn = apply_normal_map(s, mat.normal_src, sh, sn);
         depth = path_dist + h.t;
         nrm = normalize(sn);
         alb = base_of(mat);
@@ -575,16 +577,16 @@ hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopSha
 hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st) {
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
-    if (ls.groups) {
-      if (ls.lean)
-        hipLaunchKernelGGL((k_shade<P, true, true>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, ls.groups);
-      else
-        hipLaunchKernelGGL((k_shade<P, true, false>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, ls.groups);
+    const dim3 g(ls.shade_grid), b(256);
+    const uint32_t groups = ls.groups;
+    if (groups) {
+      if (ls.tier == kTierLean) hipLaunchKernelGGL((k_shade<P, true, kTierLean>), g, b, 0, st, p, q, groups);
+      else if (ls.tier == kTierStandard) hipLaunchKernelGGL((k_shade<P, true, kTierStandard>), g, b, 0, st, p, q, groups);
+      else hipLaunchKernelGGL((k_shade<P, true, kTierFull>), g, b, 0, st, p, q, groups);
     } else {
-      if (ls.lean)
-        hipLaunchKernelGGL((k_shade<P, false, true>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, 0u);
-      else
-        hipLaunchKernelGGL((k_shade<P, false, false>), dim3(ls.shade_grid), dim3(256), 0, st, p, q, 0u);
+      if (ls.tier == kTierLean) hipLaunchKernelGGL((k_shade<P, false, kTierLean>), g, b, 0, st, p, q, 0u);
+      else if (ls.tier == kTierStandard) hipLaunchKernelGGL((k_shade<P, false, kTierStandard>), g, b, 0, st, p, q, 0u);
+      else hipLaunchKernelGGL((k_shade<P, false, kTierFull>), g, b, 0, st, p, q, 0u);
     }
     return hipGetLastError();
   });

@@ -120,6 +120,8 @@ __global__ void k_px_shade(DevScene s, ShadeParams w, ProxyState st, uint32_t it
   p.primary = (u.y & kPrimary) != 0u;
   p.kinds = u.z;
   p.path_dist = __uint_as_float(u.w);
+  p.item = 0;
+  p.pixel = idx;
   Rng rng = make_rng(idx, w.sample_offset + sample_index, 1u + it);
   ShadeOut out;
   shade_path(s, w, p, hr.hit != 0u, hr.prim_id, hr.t, hr.bary[0], hr.bary[1], rng, out);

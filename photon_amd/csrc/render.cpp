@@ -38,6 +38,10 @@ ShadeParams make_shade_params(const phx_camera &cam, const phx_render_params &p,
   w.max_glossy = p.max_glossy_bounces;
   w.max_transmission = p.max_transmission_bounces;
   w.max_volume = p.max_volume_bounces;
+  const bool bg = p.background_image && p.background_width > 0 && p.background_height > 0;
+  w.bg_image = bg ? reinterpret_cast<const float4 *>(p.background_image) : nullptr;
+  w.bg_w = bg ? p.background_width : 0u;
+  w.bg_h = bg ? p.background_height : 0u;
   return w;
 }
 
@@ -98,6 +102,10 @@ int check_params(phx_context *c, const phx_render_params *p, const char *who) {
     return fail(c, PHX_ERR_INVALID, std::string(who) + ": frame larger than 2^26 pixels");
   const uint32_t max_depth = p->max_depth > 0 ? p->max_depth : 1u;
   if (max_depth + 9u > kMaxIterations) return fail(c, PHX_ERR_INVALID, std::string(who) + ": max_depth too large");
+  if (p->background_image && (p->background_width == 0 || p->background_height == 0))
+    return fail(c, PHX_ERR_INVALID, std::string(who) + ": background image without a size");
+  if (p->background_image && (reinterpret_cast<uintptr_t>(p->background_image) & 15u))
+    return fail(c, PHX_ERR_INVALID, std::string(who) + ": background image not 16-byte aligned");
   return PHX_OK;
 }
 
@@ -333,14 +341,18 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   HIP_TRY(c, c->stats_buf.reserve(kStatWords * 8));
   if (!c->pinned) HIP_TRY(c, hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 64, hipHostMallocDefault));
   LoopShape ls = loop_shape(c, cap);
-  ls.lean = 1u;
+  ls.tier = kTierLean;
   uint32_t depth = 0;
   for (const phx_context *sc : scenes) depth = std::max(depth, sc->bvh_depth);
   if (!c->tuning.refill_lanes) ls.refill = depth >= kDeepTree ? kRefillDeep : kRefillShallow;
   for (const phx_context *sc : scenes) {
     ls.volumes |= sc->dev.vol_count > 0 ? 1u : 0u;
-    if (sc->dev.vol_count > 0 || sc->dev.tex_count > 0 || sc->dev.light_count > 0) ls.lean = 0u;
+    if (sc->dev.vol_count > 0 || sc->dev.tex_count > 0 || sc->dev.light_count > 0)
+      ls.tier = std::max<uint32_t>(ls.tier, kTierStandard);
+    if (sc->dev.has_sources) ls.tier = kTierFull;
   }
+  for (const ViewJob &j : views)
+    if (j.sp.bg_image) ls.tier = kTierFull;
   if (ls.volumes) {
     HIP_TRY(c, c->vol_ev.reserve(size_t(cap) * 16));
     HIP_TRY(c, c->vol_rng.reserve(size_t(cap) * 4));

@@ -55,6 +55,9 @@ class Scene:
         self.volume_values = None
         self.volume_tf = None
         self.volume_majorants = None
+        self.attributes = {}        # slot (0-4) -> (vertex_count, 4) float32 RGBA records
+        self.samplers = []          # abi.SamplerInfo
+        self.sampler_texels = None  # (N, 4) float32
         self.ray_epsilon = 0.0
         self.camera = None          # dict(look_from, look_at, vup, vfov, aspect)
         self.name = "scene"
@@ -118,6 +121,28 @@ class Scene:
             d.volume_tf_count = self.volume_tf.size // 4
             d.volume_majorants = _ptr(self.volume_majorants, C.c_float)
             d.volume_majorant_count = self.volume_majorants.size
+        for k in range(5):
+            d.attribute_offset[k] = -1
+        if self.attributes:
+            recs, off = [], 0
+            for slot in sorted(self.attributes):
+                a = np.ascontiguousarray(self.attributes[slot], dtype=np.float32).reshape(-1, 4)
+                d.attribute_offset[slot] = off
+                recs.append(a)
+                off += a.shape[0]
+            allrec = np.ascontiguousarray(np.concatenate(recs), dtype=np.float32)
+            self._keep.append(allrec)
+            d.attributes = _ptr(allrec, C.c_float)
+            d.attribute_count = allrec.shape[0]
+        if self.samplers:
+            sms = (abi.SamplerInfo * len(self.samplers))(*self.samplers)
+            self._keep.append(sms)
+            d.samplers = C.cast(sms, C.POINTER(abi.SamplerInfo))
+            d.sampler_count = len(self.samplers)
+            if self.sampler_texels is not None:
+                self.sampler_texels = np.ascontiguousarray(self.sampler_texels, dtype=np.float32).reshape(-1, 4)
+                d.sampler_texels = _ptr(self.sampler_texels, C.c_float)
+                d.sampler_texel_count = self.sampler_texels.shape[0]
         d.ray_epsilon = self.ray_epsilon
         return d
 
@@ -133,6 +158,7 @@ def diffuse(rgb):
     m.emission_strength = 1.0
     m.alpha_mode = abi.PHX_ALPHA_OPAQUE
     m.flags = abi.PHX_MATERIAL_FLAG_REFRACTIVE
+    m.roughness_src_squared = 1
     return m
 
 

@@ -37,16 +37,44 @@ def test_library_exports_every_declared_function():
 
 def test_struct_layouts_match_header():
     abi.check_layout()
-    # offsets that the kernels rely on
-    assert abi.Material.emission.offset == 64
-    assert abi.Material.flags.offset == 100
+    # offsets of pt::Material / pt::TfVolume (material.h:35-82, tf_volume.h:14-33)
+    assert abi.Material.emission.offset == 56
+    assert abi.Material.base_color_src.offset == 124
+    assert abi.Material.flags.offset == 188
     assert abi.Volume.macro_dims.offset == 120
     assert C.sizeof(abi.SceneDesc) > 0
 
 
+def test_abi_types_equal_the_reference_declarations():
+    # tests/layout/reference_types.cpp restates the reference's boundary types
+    # member by member and static_asserts every offset and size against
+    # include/phx.h: compiling it is the check
+    src = os.path.join(ROOT, "tests", "layout", "reference_types.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_layout_check_catches_a_planted_mistake(tmp_path):
+    # the same check against a header whose HitResult bool is widened (the
+    # round-1 layout) must fail to compile
+    bad = tmp_path / "include"
+    bad.mkdir()
+    text = open(HEADER).read().replace("  uint8_t hit;\n  uint8_t has_interpolated_color;\n  uint8_t pad_[2];",
+                                        "  uint32_t hit;\n  uint32_t has_interpolated_color;\n  uint32_t pad_;")
+    assert "uint32_t has_interpolated_color" in text
+    (bad / "phx.h").write_text(text)
+    layout = tmp_path / "tests" / "layout"
+    layout.mkdir(parents=True)
+    src = open(os.path.join(ROOT, "tests", "layout", "reference_types.cpp")).read()
+    (layout / "reference_types.cpp").write_text(src)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", str(layout / "reference_types.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "phx_hit" in r.stderr
+
+
 def test_backend_name_and_abi_version():
     assert lib().phx_backend_name() == b"hip"
-    assert lib().phx_abi_version() == 2
+    assert lib().phx_abi_version() == 3
 
 
 def test_errors_without_gpu_or_arguments():
@@ -122,7 +150,10 @@ def test_ctypes_layouts_equal_the_c_header(tmp_path):
                "phx_render_params": abi.RenderParams, "phx_hit": abi.Hit, "phx_hit_compact": abi.HitCompact,
                "phx_tuning": abi.Tuning, "phx_stats": abi.Stats, "phx_view": abi.View,
                "phx_camera_data": abi.CameraData, "phx_render_settings": abi.RenderSettings,
-               "phx_bridge_material": abi.BridgeMaterial}
+               "phx_bridge_material": abi.BridgeMaterial, "phx_sampler_info": abi.SamplerInfo,
+               "phx_param_source": abi.ParamSource, "phx_light_data": abi.LightData,
+               "phx_sampler_data": abi.SamplerData, "phx_environment_data": abi.EnvironmentData,
+               "phx_volume_data": abi.VolumeData, "phx_scene_data": abi.SceneData}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "phx.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
@@ -145,6 +176,8 @@ def _fields(m):
     out = {}
     for name, _ in m._fields_:
         v = getattr(m, name)
+        if isinstance(v, C.Structure):
+            v = tuple(_fields(v).items())
         out[name] = tuple(v) if hasattr(v, "__len__") else v
     return out
 
@@ -184,15 +217,23 @@ def test_convert_material_follows_the_bridge():
         assert convert_material(abi.bridge_material(emission_side=side)).flags == abi.PHX_MATERIAL_FLAG_REFRACTIVE | bit
 
 
-def test_convert_material_rejects_parameter_sources():
-    from photon_amd._lib import PhxError
+def test_convert_material_copies_parameter_sources():
+    # convert_source (RenderBridge.cpp:314-322): kind 1 attribute, 2 sampler,
+    # anything else constant; the index is kept.  Sourced roughness is
+    # perceptual (roughness_src_squared = 1); a sourced opacity scales a
+    # neutral constant (alpha = 1).
     from photon_amd.renderer import convert_material
-    for src in ("base_color_src", "opacity_src", "metallic_src", "roughness_src", "emission_src", "transmission_src",
-                "normal_src"):
-        for kind in (1, 2):  # attribute, sampler
-            with pytest.raises(PhxError) as e:
-                convert_material(abi.bridge_material(**{src: kind}))
-            assert e.value.code == -7, (src, kind)  # PHX_ERR_UNSUPPORTED
+    names = {"base_color_src": "base_color_src", "opacity_src": "alpha_src", "metallic_src": "metallic_src",
+             "roughness_src": "roughness_src", "emission_src": "emission_src", "transmission_src": "transmission_src",
+             "normal_src": "normal_src"}
+    for src, dst in names.items():
+        for kind, want in ((1, abi.PHX_SOURCE_ATTRIBUTE), (2, abi.PHX_SOURCE_SAMPLER), (5, abi.PHX_SOURCE_CONSTANT)):
+            m = convert_material(abi.bridge_material(opacity=0.4, **{src: (kind, 3)}))
+            assert (getattr(m, dst).kind, getattr(m, dst).index) == (want, 3), (src, kind)
+            assert m.alpha == (1.0 if (src == "opacity_src" and want != 0) else np.float32(0.4)), (src, kind)
+            assert m.roughness_src_squared == 1
+    assert convert_material(abi.bridge_material(base_color_alpha=1)).base_color_alpha == 1
+    assert convert_material(abi.bridge_material()).base_color_alpha == 0
 
 
 def test_scene_materials_are_converted_bridge_materials():
