@@ -397,6 +397,18 @@ int phx_trace_closest_compact(phx_context *ctx, const float *d_origins, const fl
 int phx_trace_occluded(phx_context *ctx, const float *d_origins, const float *d_directions,
                        const float *d_tmin, const float *d_tmax, uint32_t count,
                        uint32_t *d_occluded, void *stream);
+/* phx_trace_closest without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  The kernels' scratch (chunk counters and
+ * traversal-stack spill) is a static 64 MiB device array per GPU, looked up
+ * once per device by phx_create; the persistent grid is sized so that a BVH's
+ * worst-case spill fits it.  Calls whose kernels overlap on one GPU (other
+ * streams, threads or contexts) share that array and race.  A BVH too deep
+ * for one work-group per CU gives PHX_ERR_OVERFLOW.  Results equal
+ * phx_trace_closest's. */
+int phx_trace_closest_noalloc(phx_context *ctx, const float *d_origins, const float *d_directions,
+                              const float *d_tmin, const float *d_tmax, uint32_t count, phx_hit *d_hits,
+                              void *stream);
 int phx_synchronize(phx_context *ctx);
 
 /* ---- integrator: PathTracer (pathtracer.h:65-73) ------------------------

@@ -38,6 +38,7 @@ def _sig(lib):
         "phx_trace_closest": (C.c_int, [P, P, P, P, P, C.c_uint32, P, P]),
         "phx_trace_closest_compact": (C.c_int, [P, P, P, P, P, C.c_uint32, P, P]),
         "phx_trace_occluded": (C.c_int, [P, P, P, P, P, C.c_uint32, P, P]),
+        "phx_trace_closest_noalloc": (C.c_int, [P, P, P, P, P, C.c_uint32, P, P]),
         "phx_synchronize": (C.c_int, [P]),
         "phx_bsdf_eval": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P]),
         "phx_bsdf_sample": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P, P, P]),

@@ -92,7 +92,7 @@ class HipBackend:
         return n
 
     def trace_closest(self, origins, directions, tmin, tmax, hits, stream=None):
-        """Full HitResult records into `hits` (uint8 tensor of n * 96 bytes)."""
+        """Full HitResult records into `hits` (uint8 tensor of n * 88 bytes)."""
         n = self._rays(origins, directions, tmin, tmax)
         if hits.numel() * hits.element_size() < n * C.sizeof(abi.Hit):
             raise ValueError("hits buffer too small")

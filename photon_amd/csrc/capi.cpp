@@ -177,6 +177,8 @@ int phx_create(phx_context **out, int device) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PHX_ERR_HIP;
   if (device < 0 || device >= n) return PHX_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return PHX_ERR_HIP;
+  // the no-allocation entry point's static scratch on this device (once per device)
+  if (phx_detail::noalloc_setup(device) != PHX_OK) return PHX_ERR_HIP;
   auto *c = new phx_context();
   c->device = device;
   if (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->stream2) != hipSuccess ||

@@ -123,6 +123,8 @@ struct phx_context {
 namespace phx_detail {
 
 phx_material default_material();
+// noalloc.hip: looks up the static traversal scratch of `device` (current).
+int noalloc_setup(int device);
 phx_material convert_bridge_material(const phx_bridge_material &in);
 
 inline int fail(phx_context *c, int code, const std::string &m) {

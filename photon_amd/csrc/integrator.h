@@ -6,6 +6,7 @@
 
 #include "dev_queue.h"
 #include "dev_shade.h"
+#include "dev_trace.h"
 
 namespace phx {
 
@@ -203,16 +204,19 @@ hipError_t launch_grad_out(const double *src, float *dst, uint32_t n, hipStream_
 hipError_t launch_clear_f64(double *p, uint32_t n, hipStream_t st);
 hipError_t launch_set_material_params(phx_material *mats, const float *p, uint32_t count, hipStream_t st);
 
-// Seam A kernels (trace.hip).
+// Seam A kernels (trace.hip): persistent grids of `grid` work-groups of 256
+// with `spill_stride` ints of traversal-stack spill per thread in `spill`
+// and the chunk counters in `fetch` (kFetchSetWords words, zeroed first).
+constexpr uint32_t kFullSpill = kMaxStack - kShortStack;  // spill for any BVH the builder accepts
 hipError_t launch_trace_compact(const DevScene &s, const float *org, const float *dir, const float *tmin,
                                 const float *tmax, uint32_t n, phx_hit_compact *out, uint32_t *fetch, int *spill,
-                                uint32_t grid, hipStream_t st);
+                                uint32_t grid, hipStream_t st, uint32_t spill_stride = kFullSpill);
 hipError_t launch_trace_full(const DevScene &s, const float *org, const float *dir, const float *tmin,
                              const float *tmax, uint32_t n, phx_hit *out, uint32_t *fetch, int *spill, uint32_t grid,
-                             hipStream_t st);
+                             hipStream_t st, uint32_t spill_stride = kFullSpill);
 hipError_t launch_trace_occluded(const DevScene &s, const float *org, const float *dir, const float *tmin,
                                  const float *tmax, uint32_t n, uint32_t *out, uint32_t *fetch, int *spill,
-                                 uint32_t grid, hipStream_t st);
+                                 uint32_t grid, hipStream_t st, uint32_t spill_stride = kFullSpill);
 hipError_t launch_clear_u32(uint32_t *p, uint32_t n, hipStream_t st);
 
 }  // namespace phx

@@ -12,12 +12,15 @@ constexpr uint32_t kRefill = 16;  // idle lanes that make a wave take new rays
 PHX_DEV vec3 ldv(const float *p, uint32_t i) { return {p[i * 3], p[i * 3 + 1], p[i * 3 + 2]}; }
 
 template <int MODE>  // 0: compact closest, 1: full HitResult, 2: occluded flag
+// `spill_stride` ints of stack spill per thread: at least the entries the
+// scene's BVH can push beyond the LDS stack ((kNodeWidth - 1) * depth + 1 -
+// kShortStack).
 __global__ __launch_bounds__(256) void k_trace(DevScene s, const float *org, const float *dir, const float *tmin,
                                                const float *tmax, uint32_t n, void *out, uint32_t *fetch,
-                                               int *spill) {
+                                               int *spill, uint32_t spill_stride) {
   __shared__ int lds[kShortStack * 256];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-  LaneStack st{&lds[threadIdx.x], 256, spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
+  LaneStack st{&lds[threadIdx.x], 256, spill + size_t(gtid) * spill_stride, 0};
   traverse_stream<MODE == 2>(
       st, fetch, n, kChunk, kRefill,
       [&](uint32_t i, vec3 &o, vec3 &d, float &t0, float &t1, const BvhNode *&nodes, const TriRec *&tris) {
@@ -91,20 +94,23 @@ hipError_t launch_clear_u32(uint32_t *p, uint32_t n, hipStream_t st) {
 }
 hipError_t launch_trace_compact(const DevScene &s, const float *org, const float *dir, const float *tmin,
                                 const float *tmax, uint32_t n, phx_hit_compact *out, uint32_t *fetch, int *spill,
-                                uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(256), 0, st, s, org, dir, tmin, tmax, n, (void *)out, fetch, spill);
+                                uint32_t grid, hipStream_t st, uint32_t spill_stride) {
+  hipLaunchKernelGGL(k_trace<0>, dim3(grid), dim3(256), 0, st, s, org, dir, tmin, tmax, n, (void *)out, fetch, spill,
+                     spill_stride);
   return hipGetLastError();
 }
 hipError_t launch_trace_full(const DevScene &s, const float *org, const float *dir, const float *tmin,
                              const float *tmax, uint32_t n, phx_hit *out, uint32_t *fetch, int *spill, uint32_t grid,
-                             hipStream_t st) {
-  hipLaunchKernelGGL(k_trace<1>, dim3(grid), dim3(256), 0, st, s, org, dir, tmin, tmax, n, (void *)out, fetch, spill);
+                             hipStream_t st, uint32_t spill_stride) {
+  hipLaunchKernelGGL(k_trace<1>, dim3(grid), dim3(256), 0, st, s, org, dir, tmin, tmax, n, (void *)out, fetch, spill,
+                     spill_stride);
   return hipGetLastError();
 }
 hipError_t launch_trace_occluded(const DevScene &s, const float *org, const float *dir, const float *tmin,
                                  const float *tmax, uint32_t n, uint32_t *out, uint32_t *fetch, int *spill,
-                                 uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_trace<2>, dim3(grid), dim3(256), 0, st, s, org, dir, tmin, tmax, n, (void *)out, fetch, spill);
+                                 uint32_t grid, hipStream_t st, uint32_t spill_stride) {
+  hipLaunchKernelGGL(k_trace<2>, dim3(grid), dim3(256), 0, st, s, org, dir, tmin, tmax, n, (void *)out, fetch, spill,
+                     spill_stride);
   return hipGetLastError();
 }
 
