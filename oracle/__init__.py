@@ -61,11 +61,17 @@ def lib():
         L.orc_render_detached.restype = C.c_double
         L.orc_render_detached.argtypes = [P, P, P, P, C.c_uint32, P, P, P, C.c_int]
         L.orc_render.argtypes = [P, P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_uint32, C.c_uint32]
+        L.orc_lobe_weights.argtypes = [P, C.c_double, C.c_int, P]
+        L.orc_sample_light.argtypes = [P, P, C.c_uint32, P]
+        L.orc_texture_sample.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_int, P]
+        L.orc_sampler_sample.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_int, P, C.c_int, P]
+        L.orc_surface_resolve.argtypes = [P, C.c_uint32, C.c_double, C.c_double, P, P, P, P, P, P]
         _lib = L
     return _lib
 
 
 def _p(a):
+    """Pointer to an array's data; the caller holds the array for the call."""
     return None if a is None else C.c_void_p(a.ctypes.data)
 
 
@@ -154,16 +160,29 @@ class OracleScene:
     def volume_transmittance(self, org, dir_, tmin, tmax, seeds):
         seeds = _c(seeds, np.uint32)
         out = np.zeros(seeds.size, np.float32)
-        lib().orc_volume_transmittance(self.h, _p(_c(org, np.float32)), _p(_c(dir_, np.float32)), tmin, tmax,
+        org, dir_ = _c(org, np.float32), _c(dir_, np.float32)  # held: the call reads them
+        lib().orc_volume_transmittance(self.h, _p(org), _p(dir_), tmin, tmax,
                                        _p(seeds), seeds.size, _p(out))
         return out
 
     def volume_scatter(self, org, dir_, tmin, tmax, seeds):
         seeds = _c(seeds, np.uint32)
         out = np.zeros(seeds.size, np.float32)
-        lib().orc_volume_scatter(self.h, _p(_c(org, np.float32)), _p(_c(dir_, np.float32)), tmin, tmax, _p(seeds),
+        org, dir_ = _c(org, np.float32), _c(dir_, np.float32)
+        lib().orc_volume_scatter(self.h, _p(org), _p(dir_), tmin, tmax, _p(seeds),
                                  seeds.size, _p(out))
         return out
+
+    def surface_resolve(self, prim, b1, b2, p, sn, uv=(0.0, 0.0)):
+        """resolve_material plus the normal maps at a point of primitive
+        `prim`: (material, alpha, shading normal), in double."""
+        from photon_amd import abi
+        m = abi.Material()
+        alpha = np.zeros(1, np.float64)
+        n = np.zeros(3, np.float64)
+        p, sn, uv = _c(p, np.float64), _c(sn, np.float64), _c(uv, np.float64)
+        lib().orc_surface_resolve(self.h, prim, b1, b2, _p(p), _p(sn), _p(uv), C.addressof(m), _p(alpha), _p(n))
+        return m, float(alpha[0]), n
 
     def render(self, camera, params, threads=0, use_double=False, pixel_range=(0, 0), aovs=False):
         """PathTracer::render on the CPU: returns (color (h,w,3), extras dict, seconds)."""
@@ -273,4 +292,41 @@ def volume_majorants(vol, values, tf):
     n = lib().orc_volume_majorants(C.addressof(vol), _p(values), _p(tf), tf.size // 4, None, 0)
     out = np.zeros(n, np.float32)
     lib().orc_volume_majorants(C.addressof(vol), _p(values), _p(tf), tf.size // 4, _p(out), n)
+    return out
+
+
+def lobe_weights(mat, nov, use_double=True):
+    """disney_lobe_weights at NoV: (wd, ws, wt, wc)."""
+    out = np.zeros(4, np.float64)
+    lib().orc_lobe_weights(C.addressof(mat), nov, int(use_double), _p(out))
+    return out
+
+
+def sample_light(light, p, seed):
+    """sample_light from point p with Rng(seed): (wi, Li, pdf, dist)."""
+    out = np.zeros(8, np.float32)
+    p = _c(p, np.float32)
+    lib().orc_sample_light(C.addressof(light), _p(p), seed, _p(out))
+    return out[:3], out[3:6], float(out[6]), float(out[7])
+
+
+def texture_sample(pixels, infos, tex_id, u, v, use_double=True):
+    """TextureAtlas::sample: pixels (N, 3), infos [(offset, width, height)]."""
+    from photon_amd import abi
+    px = _c(pixels, np.float32).reshape(-1, 3)
+    arr = (abi.TextureInfo * max(len(infos), 1))(*[abi.TextureInfo(o, w, h, 0) for (o, w, h) in infos])
+    out = np.zeros(3, np.float64)
+    lib().orc_texture_sample(_p(px), px.shape[0], C.addressof(arr), len(infos), tex_id, u, v, int(use_double), _p(out))
+    return out
+
+
+def sampler_sample(infos, texels, sampler_id, attr, use_double=True):
+    """SamplerTable::sample of abi.SamplerInfo list over RGBA texels (N, 4)."""
+    from photon_amd import abi
+    arr = (abi.SamplerInfo * max(len(infos), 1))(*infos)
+    tx = None if texels is None else _c(texels, np.float32).reshape(-1, 4)
+    out = np.zeros(4, np.float64)
+    attr = _c(attr, np.float64)
+    lib().orc_sampler_sample(C.addressof(arr), len(infos), _p(tx), 0 if tx is None else tx.shape[0], sampler_id,
+                             _p(attr), int(use_double), _p(out))
     return out

@@ -1500,6 +1500,93 @@ void orc_bsdf_sample(const phx_material *mats, const float *wo, const float *ng,
   }
 }
 
+// disney_lobe_weights (disney_bsdf.h:148-175) at NoV: wd, ws, wt, wc.
+void orc_lobe_weights(const phx_material *m, double nov, int use_double, double *out) {
+  if (use_double) {
+    double w[4];
+    disney_lobe_weights(mat_from<double>(*m), nov, w[0], w[1], w[2], w[3]);
+    std::copy(w, w + 4, out);
+  } else {
+    float w[4];
+    disney_lobe_weights(mat_from<float>(*m), float(nov), w[0], w[1], w[2], w[3]);
+    for (int k = 0; k < 4; ++k) out[k] = w[k];
+  }
+}
+// sample_light (light.h:72-148) from point p with Rng(seed): wi, Li, pdf, dist.
+void orc_sample_light(const phx_light *l, const float *p, uint32_t seed, float *out) {
+  Rng r(seed);
+  const LightSample<float> ls = sample_light<float>(*l, v3<float>(p), r);
+  const float v[8] = {ls.wi.x, ls.wi.y, ls.wi.z, ls.Li.x, ls.Li.y, ls.Li.z, ls.pdf, ls.dist};
+  std::copy(v, v + 8, out);
+}
+// TextureAtlas::sample (texture.h:25-64) of texture `id` at (u, v).
+void orc_texture_sample(const float *pixels, uint32_t pixel_count, const phx_texture_info *infos, uint32_t count, int id,
+                        double u, double v, int use_double, double *out) {
+  Atlas a;
+  a.pixels.assign(pixels, pixels + size_t(pixel_count) * 3);
+  a.infos.assign(infos, infos + count);
+  if (use_double) {
+    const V3<double> c = a.sample<double>(id, u, v);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+  } else {
+    const V3<float> c = a.sample<float>(id, float(u), float(v));
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+  }
+}
+// SamplerTable::sample (sampler.h:73-138) of sampler `id` for attribute value `attr`.
+void orc_sampler_sample(const phx_sampler_info *infos, uint32_t count, const float *texels, uint32_t texel_count, int id,
+                        const double *attr, int use_double, double *out) {
+  Scene s;
+  s.samplers.assign(infos, infos + count);
+  if (texels) s.sampler_texels.assign(texels, texels + size_t(texel_count) * 4);
+  if (use_double) {
+    const V4<double> r = sampler_sample<double>(s, id, V4<double>{attr[0], attr[1], attr[2], attr[3]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+  } else {
+    const V4<float> r = sampler_sample<float>(s, id, V4<float>{float(attr[0]), float(attr[1]), float(attr[2]), float(attr[3])});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+  }
+}
+// resolve_material and the sourced normal map (pathtracer.cpp:111-247, 966-972)
+// at a point of primitive `prim` with barycentrics (b1, b2), position p,
+// shading normal sn and texture coordinates uv: the resolved material, its
+// alpha and the shading normal after the normal maps (the normal_tex map
+// first, then normal_src).
+void orc_surface_resolve(void *sp, uint32_t prim, double b1, double b2, const double *p, const double *sn,
+                         const double *uv, phx_material *mat_out, double *alpha_out, double *normal_out) {
+  const Scene &s = *static_cast<Scene *>(sp);
+  Query<double> q;
+  q.prim = prim;
+  q.mat = s.mesh.mat.empty() ? 0u : s.mesh.mat[prim];
+  q.b1 = b1;
+  q.b2 = b2;
+  q.p = {p[0], p[1], p[2]};
+  q.sn = {sn[0], sn[1], sn[2]};
+  q.uv = {uv[0], uv[1]};
+  double alpha = 1.0;
+  const Mat<double> m = resolve_material<double>(s, q, alpha, nullptr);
+  V3<double> n = q.sn;
+  if (m.normal_tex >= 0 && !s.atlas.infos.empty()) {
+    V3<double> nm = s.atlas.sample<double>(m.normal_tex, q.uv.x, q.uv.y);
+    nm = nm * 2.0 - V3<double>{1.0, 1.0, 1.0};
+    V3<double> ta, bi;
+    onb_from_normal(n, ta, bi);
+    n = normalize(ta * nm.x + bi * nm.y + n * nm.z);
+  }
+  if (m.normal_src.kind != PHX_SOURCE_CONSTANT) n = apply_normal_map<double>(s, m.normal_src, q, n);
+  phx_material o = s.mats[q.mat];
+  for (int k = 0; k < 3; ++k) {
+    o.base_color[k] = float((&m.base_color.x)[k]);
+    o.emission[k] = float((&m.emission.x)[k]);
+  }
+  o.metallic = float(m.metallic);
+  o.roughness = float(m.roughness);
+  o.transmission = float(m.transmission);
+  *mat_out = o;
+  *alpha_out = alpha;
+  normal_out[0] = n.x; normal_out[1] = n.y; normal_out[2] = n.z;
+}
+
 void orc_rng(uint32_t pixel, uint32_t sample, uint32_t dim, uint32_t n, uint32_t *u32_out, float *f32_out) {
   Rng r = make_rng(pixel, sample, dim);
   Rng r2 = r;
