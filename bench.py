@@ -45,28 +45,20 @@ def batched_form(pt_c1, c1, args):
     from photon_amd import abi, scenes
     from photon_amd.backend import BatchRenderer, HipBackend, PathTracer, make_camera
 
-    rng = np.random.default_rng(2024)
-    res = rng.choice([64, 128, 256, 512, 1024], 16)
-    spp = rng.choice([4, 8, 16, 32, 64], 16)
+    draw = scenes.batch_draw(2024, 16)
+    res = [d["res"] for d in draw]
+    spp = [d["spp"] for d in draw]
     views = []
-    for v in range(16):
-        w = h = int(res[v])
-        if v < 8:
+    for d in draw:
+        w = h = d["res"]
+        if d["scene"] == "spheres":
             pt, s = pt_c1, c1
-            ang = rng.uniform(0, 2 * np.pi)
         else:
-            walls = tuple(tuple(rng.uniform(0.05, 0.8, 3)) for _ in range(3))
-            s = scenes.cornell_box(wall_colors=walls)
+            s = scenes.cornell_box(wall_colors=d["walls"])
             pt = PathTracer(HipBackend(device=0))
             pt.set_scene(s)
-            ang = rng.uniform(-0.3, 0.3)
-        eye = np.asarray(s.camera["look_from"], np.float64)
-        at = np.asarray(s.camera["look_at"], np.float64)
-        d = eye - at
-        c, sn = np.cos(ang), np.sin(ang)
-        eye = at + np.array([c * d[0] + sn * d[2], d[1], -sn * d[0] + c * d[2]])
-        cam = make_camera(eye, at, s.camera["vup"], s.camera["vfov"], 1.0)
-        views.append((pt, cam, abi.default_render_params(w, h, int(spp[v]), args.depth)))
+        cam = make_camera(scenes.orbit_eye(s, d["angle"]), s.camera["look_at"], s.camera["vup"], s.camera["vfov"], 1.0)
+        views.append((pt, cam, abi.default_render_params(w, h, d["spp"], args.depth)))
     total = sum(p.width * p.height * p.samples_per_pixel for _, _, p in views)
 
     def plain_pass():
@@ -125,12 +117,7 @@ def volume_form(args):
     pt.set_scene(s)
 
     def cam_at(ang):
-        eye = np.asarray(s.camera["look_from"], np.float64)
-        at = np.asarray(s.camera["look_at"], np.float64)
-        d = eye - at
-        c, sn = np.cos(ang), np.sin(ang)
-        eye = at + np.array([c * d[0] + sn * d[2], d[1], -sn * d[0] + c * d[2]])
-        return make_camera(eye, at, s.camera["vup"], s.camera["vfov"], 1.0)
+        return make_camera(scenes.orbit_eye(s, ang), s.camera["look_at"], s.camera["vup"], s.camera["vfov"], 1.0)
 
     params = abi.default_render_params(1024, 1024, 32, 16)
     pt.camera, pt.params = cam_at(0.0), params
@@ -262,6 +249,16 @@ def main():
     for i in range(args.warmup):
         pt.render(read=False)
         print(f"[bench] warmup {i} {pt.ctx.stats().last_render_ms:.1f} ms", file=sys.stderr, flush=True)
+    # the same frames without per-launch HIP events: what the event records cost
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_steps = max(2, args.steps // 4)
+    for i in range(plain_steps):
+        pt.render(read=False)
+    torch.cuda.synchronize()
+    ms_plain = (time.perf_counter() - t0) * 1e3 / plain_steps
+    # the timed steps record an event pair around every kernel launch, on the
+    # launch's stream: the roofline's kernel time is measured over them
     pt.ctx.set_tuning(time_kernels=1)
     closest = shadow = shade = 0
     ext_ms = shd_ms = sdw_ms = 0.0
@@ -298,7 +295,19 @@ def main():
                 "alg_bytes_per_launch": alg_bytes / max(ext_launches, 1)}
     traffic_file = os.path.join(ROOT, "profiles", "extend_traffic.json")
     if os.path.exists(traffic_file):
-        roofline["traffic"] = json.load(open(traffic_file)).get("hbm_bytes_per_launch")
+        tr = json.load(open(traffic_file))
+        roofline["traffic"] = tr.get("hbm_bytes_per_launch")
+        roofline["traffic_source"] = ("stored: calibrated rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE of k_extend on this "
+                                      "workload (" + tr.get("profile", "profiles/extend_traffic.json") + "), not "
+                                      "measured in this run")
+    # the whole frame against the same roofline: algorithmic bytes of every
+    # kernel (48 B per closest ray, 36 B per shadow ray, 128 B per shading
+    # event, 64 B per sample, 16 B per pixel) plus the scene read once
+    st = pt.ctx.stats()
+    frame_bytes = (48.0 * closest + 36.0 * shadow + 128.0 * shade) / args.steps + 64.0 * npix * args.spp + \
+        16.0 * npix + float(st.scene_bytes)
+    frame = {"alg_bytes": frame_bytes, "achieved": frame_bytes / (ms_per_step * 1e-3) / 1e9, "unit": "GB/s"}
+    frame["frac"] = frame["achieved"] / HBM_PEAK_GBS
 
     # Same frame with the reference's algorithmic structure on this GPU (the
     # stand-in for the reference's Kokkos-HIP build): binary median-split BVH,
@@ -327,11 +336,17 @@ def main():
         o = oracle.OracleScene(scene)
         rows = min(args.cpu_rows, args.height)
         y0 = (args.height - rows) // 2
-        _, _, secs = o.render(cam, pt.params, threads=cores, pixel_range=(y0 * args.width, (y0 + rows) * args.width))
+        ref, _, secs = o.render(cam, pt.params, threads=cores,
+                                pixel_range=(y0 * args.width, (y0 + rows) * args.width))
         cpu_samples = rows * args.width * args.spp
+        # the oracle's rows against the same rows of the GPU frame
+        g, r = res.color[y0:y0 + rows].astype(np.float64), ref[y0:y0 + rows].astype(np.float64)
+        rel_mse = float(np.mean((g - r) ** 2 / (r ** 2 + 1e-2)))
+        equal = float(np.mean(np.abs(g - r) <= 1e-3 * (1 + np.abs(r))))
         cpu = {"value": cpu_samples / secs, "unit": "samples/s", "cores": cores, "kind": "port",
                "sample": f"rows {y0}-{y0 + rows - 1} of the same 1024x1024x64spp frame ({cpu_samples} samples, "
-                         f"{secs:.1f} s)"}
+                         f"{secs:.1f} s)",
+               "parity": {"rel_mse_vs_gpu": rel_mse, "pixels_equal_1e-3": equal}}
 
     batched = batched_form(pt, scene, args) if args.batch_reps > 0 else None
     adjoint = adjoint_form(args) if args.adjoint_reps > 0 else None
@@ -361,6 +376,8 @@ def main():
                                "note": "HIP-event spans on each launch's stream; the shadow pass runs on a second "
                                        "stream beside the next extend and shade launches, so the shade and shadow spans include waiting for CUs"},
         "roofline": roofline,
+        "frame_roofline": frame,
+        "ms_per_step_without_kernel_events": ms_plain,
         "cpu_baseline": cpu,
         "gpu_baseline": gpu_base,
         "batched": batched,

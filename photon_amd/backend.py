@@ -139,6 +139,18 @@ def make_camera(look_from, look_at, vup=(0.0, 1.0, 0.0), vfov=40.0, aspect=1.0, 
     return cam
 
 
+def make_orthographic_camera(look_from, look_at, vup=(0.0, 1.0, 0.0), height=1.0, aspect=1.0):
+    """Camera::make_orthographic (camera.h:57-77), computed by the library."""
+    cam = abi.Camera()
+    f = np.asarray(look_from, np.float32)
+    a = np.asarray(look_at, np.float32)
+    u = np.asarray(vup, np.float32)
+    code = lib().phx_camera_orthographic(C.byref(cam), fptr(f), fptr(a), fptr(u), height, aspect)
+    if code != 0:
+        raise PhxError(code, "phx_camera_orthographic")
+    return cam
+
+
 class BatchRenderer:
     """phx_render_batch: many views (of one or several scenes) in one wavefront
     with shared queues.  A view is (scene, camera, params) where scene is a

@@ -570,3 +570,36 @@ def fog_box(n=256, seed=7, threshold=0.4, scale=20.0, macro=32, albedo=0.9, env=
     s.camera = dict(look_from=(0.0, 0.6, 4.2), look_at=(0.0, -0.1, 0.0), vup=(0.0, 1.0, 0.0), vfov=45.0, aspect=1.0)
     s.name = f"fog{n}"
     return s
+
+
+# --------------------------------------------------------------------------- batched workloads
+def orbit_eye(scene, angle):
+    """The scene's camera position turned by `angle` about the vertical axis
+    through its look-at point."""
+    eye = np.asarray(scene.camera["look_from"], np.float64)
+    at = np.asarray(scene.camera["look_at"], np.float64)
+    d = eye - at
+    c, sn = np.cos(angle), np.sin(angle)
+    return at + np.array([c * d[0] + sn * d[2], d[1], -sn * d[0] + c * d[2]])
+
+
+def batch_draw(seed=2024, views=16):
+    """The batched workload's seeded draw: resolutions from {64..1024}^2 and
+    spp from {4..64}; the first half of the views are the 737K-triangle
+    sphere grid from orbit angles in [0, 2 pi), the second half Cornell boxes
+    with random wall colours from angles in [-0.3, 0.3].  Returns one dict per
+    view: scene ("spheres" or "cornell"), res, spp, angle, walls."""
+    rng = np.random.default_rng(seed)
+    res = rng.choice([64, 128, 256, 512, 1024], views)
+    spp = rng.choice([4, 8, 16, 32, 64], views)
+    out = []
+    for v in range(views):
+        if v < views // 2:
+            out.append(dict(scene="spheres", res=int(res[v]), spp=int(spp[v]), angle=float(rng.uniform(0, 2 * np.pi)),
+                            walls=None))
+        else:
+            walls = tuple(tuple(rng.uniform(0.05, 0.8, 3)) for _ in range(3))
+            out.append(dict(scene="cornell", res=int(res[v]), spp=int(spp[v]), angle=float(rng.uniform(-0.3, 0.3)),
+                            walls=walls))
+    return out
+

@@ -91,6 +91,21 @@ def test_camera_matches_oracle_bitwise():
         assert bytes(a) == bytes(b)
 
 
+def test_orthographic_camera_matches_oracle_bitwise():
+    from oracle.scene_data import _orthographic
+    from photon_amd.backend import make_orthographic_camera
+    for f, at, up, h, asp in (((0, 1, 3.7), (0, 1, 0), (0, 1, 0), 2.4, 1.2), ((1, 2, 3), (-1, 0.5, 0.2), (0, 1, 0), 0.7, 0.5)):
+        a = make_orthographic_camera(f, at, up, h, asp)
+        b = _orthographic(np.array(f, np.float32), np.array(at, np.float32), np.array(up, np.float32), h, asp)
+        assert bytes(a) == bytes(b)
+        # closed form: the view plane is centred on the eye, h tall, h*aspect wide
+        assert a.is_ortho == 1 and a.lens_radius == 0
+        assert np.isclose(np.linalg.norm(a.vertical), h, rtol=1e-6)
+        assert np.isclose(np.linalg.norm(a.horizontal), h * asp, rtol=1e-6)
+        centre = np.array(a.lower_left) + 0.5 * np.array(a.horizontal) + 0.5 * np.array(a.vertical)
+        assert np.allclose(centre, f, atol=1e-6)
+
+
 def test_volume_majorants_match_oracle():
     rng = np.random.default_rng(0)
     for size, scale in ((9, 2.0), (40, 20.0)):
