@@ -6,10 +6,13 @@ One step renders the whole frame: 6x6 displaced icospheres (737,290 triangles),
 per pixel, max depth 8 (adaptive sampling off, no clamps).  Inputs are resident
 in HBM before the timed region.  Prints one JSON line.
 
-  value     samples/s = W * H * spp * steps / wall time
+  value     samples/s = W * H * spp * steps / wall time, the steps run as a
+            caller runs them (no per-kernel events, graph capture on)
   roofline  the closest-hit (extend) kernel: algorithmic bytes = 48 B per
             active closest ray (32 B ray read + 16 B hit written), divided by
-            the kernel's HIP-event time over the timed steps
+            the kernel's HIP-event time, taken in further steps of the same
+            frame that record an event pair around every launch
+  frame_roofline  the whole frame's algorithmic bytes over the step time
   cpu_baseline  the oracle (CPU restatement of the reference integrator) on a
             band of rows of the same frame, on this host's cores
   gpu_baseline  the same frame rendered with the reference's algorithmic
@@ -62,15 +65,21 @@ def batched_form(pt_c1, c1, args):
     total = sum(p.width * p.height * p.samples_per_pixel for _, _, p in views)
 
     def plain_pass():
-        t = time.perf_counter()
+        per = []
         for pt, cam, p in views:
+            t = time.perf_counter()
             pt.camera, pt.params = cam, p
             pt.render(read=False)
-        torch.cuda.synchronize()
-        return time.perf_counter() - t
+            torch.cuda.synchronize()
+            per.append(time.perf_counter() - t)
+        return sum(per), per
 
     plain_pass()  # warm-up: allocations
-    seq = min(plain_pass() for _ in range(args.batch_reps))
+    seq, per_view = min((plain_pass() for _ in range(args.batch_reps)), key=lambda x: x[0])
+    # views of 2^22 samples or more fill the GPU alone (the batch streams them
+    # in runs of their own): their plain time bounds the batch from below
+    solo = [v for v, (_, _, p) in enumerate(views) if p.width * p.height * p.samples_per_pixel >= 1 << 22]
+    solo_ms = sum(per_view[v] for v in solo) * 1e3
     br = BatchRenderer(device=0)
     tuning = None
     if not args.no_autotune:
@@ -94,6 +103,9 @@ def batched_form(pt_c1, c1, args):
     same = bool(np.array_equal(pt.render().color, br.read(small).color))
     return {"views": 16, "samples": total, "ms": tb * 1e3, "samples_per_s": total / tb,
             "sequential_ms": seq * 1e3, "sequential_samples_per_s": total / seq, "speedup": seq / tb,
+            "plain_ms_per_view": [round(x * 1e3, 3) for x in per_view],
+            "gpu_filling_views": solo, "gpu_filling_plain_ms": solo_ms,
+            "speedup_bound": seq * 1e3 / solo_ms if solo_ms > 0 else None,
             "resolutions": [int(x) for x in res], "spp": [int(x) for x in spp], "tuning": tuning,
             "equal_to_plain": same}
 
@@ -249,20 +261,8 @@ def main():
     for i in range(args.warmup):
         pt.render(read=False)
         print(f"[bench] warmup {i} {pt.ctx.stats().last_render_ms:.1f} ms", file=sys.stderr, flush=True)
-    # the same frames without per-launch HIP events: what the event records cost
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plain_steps = max(2, args.steps // 4)
-    for i in range(plain_steps):
-        pt.render(read=False)
-    torch.cuda.synchronize()
-    ms_plain = (time.perf_counter() - t0) * 1e3 / plain_steps
-    # the timed steps record an event pair around every kernel launch, on the
-    # launch's stream: the roofline's kernel time is measured over them
-    pt.ctx.set_tuning(time_kernels=1)
+    # the timed steps: the default path (no per-launch events, the graph on)
     closest = shadow = shade = 0
-    ext_ms = shd_ms = sdw_ms = 0.0
-    ext_launches = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -271,13 +271,28 @@ def main():
         closest += st.closest_rays
         shadow += st.shadow_rays
         shade += st.shade_events
+        print(f"[bench] step {i} {st.last_render_ms:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the kernel split, in separate steps that record an event pair around
+    # every kernel launch on the launch's stream: the roofline's kernel time
+    pt.ctx.set_tuning(time_kernels=1)
+    k_steps = max(2, args.steps // 4)
+    k_closest = 0
+    ext_ms = shd_ms = sdw_ms = 0.0
+    ext_launches = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(k_steps):
+        pt.render(read=False)
+        st = pt.ctx.stats()
+        k_closest += st.closest_rays
         ext_ms += st.extend_ms
         shd_ms += st.shade_ms
         sdw_ms += st.shadow_ms
         ext_launches += st.extend_launches
-        print(f"[bench] step {i} {st.last_render_ms:.1f} ms", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    ms_instrumented = (time.perf_counter() - t0) * 1e3 / k_steps
     pt.ctx.set_tuning(time_kernels=0)
 
     npix = args.width * args.height
@@ -286,7 +301,7 @@ def main():
     res = pt.read_result()
     assert np.all(np.isfinite(res.color)), "non-finite pixels in the benchmark frame"
 
-    alg_bytes = 48.0 * closest  # over all extend launches of the timed steps
+    alg_bytes = 48.0 * k_closest  # over all extend launches of the kernel-timing steps
     achieved = alg_bytes / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
@@ -371,13 +386,13 @@ def main():
                    "parallelism": "single GPU"},
         "mrays_per_s": (closest + shadow) / wall / 1e6,
         "rays": {"closest": closest, "shadow": shadow, "shade_events": shade},
-        "kernel_ms_per_step": {"extend": ext_ms / args.steps, "shade": shd_ms / args.steps,
-                               "shadow": sdw_ms / args.steps,
+        "kernel_ms_per_step": {"extend": ext_ms / k_steps, "shade": shd_ms / k_steps,
+                               "shadow": sdw_ms / k_steps, "steps": k_steps,
                                "note": "HIP-event spans on each launch's stream; the shadow pass runs on a second "
                                        "stream beside the next extend and shade launches, so the shade and shadow spans include waiting for CUs"},
         "roofline": roofline,
         "frame_roofline": frame,
-        "ms_per_step_without_kernel_events": ms_plain,
+        "ms_per_step_with_kernel_events": ms_instrumented,
         "cpu_baseline": cpu,
         "gpu_baseline": gpu_base,
         "batched": batched,

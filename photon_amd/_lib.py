@@ -81,6 +81,16 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with photon_amd.build_lib.build()")
         abi.check_layout()
+        # PyTorch ships its own copy of the HIP runtime under the same soname
+        # (libamdhip64.so.7) but is linked against it by another name, so
+        # loading this library first would put the system runtime in the
+        # process beside torch's, and torch then finds no GPU.  With torch
+        # loaded first the library binds to torch's runtime: one runtime, and
+        # device pointers pass between the two.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = C.CDLL(LIB_PATH)
         _sig(handle)
         _lib = handle

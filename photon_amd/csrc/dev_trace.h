@@ -129,14 +129,12 @@ PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float t
                           fminf(fmaf(q16(fzw, k), Az, Bz), tmax));
     const bool valid = k < int(nchild);
     const bool hit = valid && a <= b * 1.0000004f;
-    int code;
-    if (k < ninner) {
-      code = int(child_base) + k;
-    } else {
-      const uint32_t c = ((leaf_cnt >> (2 * k)) & 3u) + 1u;
-      code = leaf_code(leaf_first, c);
-      if (valid) leaf_first += c;
-    }
+    // both codes and a select: a branch on the lane's ninner would run both
+    // sides under exec masks anyway, with the mask bookkeeping on top
+    const bool inner = k < ninner;
+    const uint32_t c = ((leaf_cnt >> (2 * k)) & 3u) + 1u;
+    const int code = inner ? int(child_base) + k : leaf_code(leaf_first, c);
+    leaf_first += (!inner && valid) ? c : 0u;
     key[k] = hit ? a : INFINITY;
     codes[k] = code;
     cnt += hit ? 1 : 0;

@@ -117,8 +117,19 @@ ViewJob make_job(const phx_context *scene, uint32_t scene_index, const phx_camer
                  std::max<uint32_t>(p.samples_per_pixel, 1), pix_base};
 }
 
+// A view of at least this many samples keeps the GPU's path slots full by
+// itself: in a batch it streams in runs of its own, through the single-view
+// kernels (scene and view in kernel arguments, scalar loads), as phx_render
+// would run it.  Only the smaller views share runs through the item tables,
+// whose per-lane view and scene loads cost 10-18 % on the kernels of views
+// large enough not to need the sharing (the 16-view draw: its five views of
+// 2^22 samples or more took 93.4 ms together in one batch against 83.6 ms as
+// plain renders).
+constexpr uint64_t kSoloSamples = uint64_t(1) << 22;
+
 // Splits the views' samples into runs of at most kSampleBudget samples, each
-// a list of (view, first spp, spp count) items in view order.
+// a list of (view, first spp, spp count) items in view order; views of
+// kSoloSamples or more get runs of their own, split as phx_render splits them.
 std::vector<Run> plan_runs(const std::vector<ViewJob> &views) {
   // PHX_SAMPLE_BUDGET=N (diagnostic): a smaller run budget, to exercise the
   // splitting of large jobs at small sizes.
@@ -128,8 +139,17 @@ std::vector<Run> plan_runs(const std::vector<ViewJob> &views) {
     if (b > 0) budget = std::min<uint64_t>(budget, uint64_t(b));
   }
   std::vector<Run> runs(1);
-  for (uint32_t v = 0; v < views.size(); ++v) {
+  std::vector<uint32_t> order;
+  for (uint32_t v = 0; v < views.size(); ++v)
+    if (views.size() > 1 && uint64_t(views[v].sp.npix) * views[v].spp >= kSoloSamples) order.push_back(v);
+  const size_t nsolo = order.size();
+  for (uint32_t v = 0; v < views.size(); ++v)
+    if (!(views.size() > 1 && uint64_t(views[v].sp.npix) * views[v].spp >= kSoloSamples)) order.push_back(v);
+  for (size_t o = 0; o < order.size(); ++o) {
+    const uint32_t v = order[o];
     const ViewJob &j = views[v];
+    // a solo view, and the first shared view after them, start a new run
+    if ((o < nsolo || o == nsolo) && !runs.back().items.empty()) runs.emplace_back();
     const uint32_t npix = j.sp.npix;
     uint32_t s0 = 0;
     while (s0 < j.spp) {
@@ -340,26 +360,34 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   HIP_TRY(c, c->counters.reserve(kCounterWords * 4));
   HIP_TRY(c, c->stats_buf.reserve(kStatWords * 8));
   if (!c->pinned) HIP_TRY(c, hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 64, hipHostMallocDefault));
-  LoopShape ls = loop_shape(c, cap);
-  ls.tier = kTierLean;
-  uint32_t depth = 0;
-  for (const phx_context *sc : scenes) depth = std::max(depth, sc->bvh_depth);
-  if (!c->tuning.refill_lanes) ls.refill = depth >= kDeepTree ? kRefillDeep : kRefillShallow;
-  for (const phx_context *sc : scenes) {
-    ls.volumes |= sc->dev.vol_count > 0 ? 1u : 0u;
-    if (sc->dev.vol_count > 0 || sc->dev.tex_count > 0 || sc->dev.light_count > 0)
-      ls.tier = std::max<uint32_t>(ls.tier, kTierStandard);
-    if (sc->dev.has_sources) ls.tier = kTierFull;
-  }
-  for (const ViewJob &j : views)
-    if (j.sp.bg_image) ls.tier = kTierFull;
-  if (ls.volumes) {
+  // The launch shape of each run: the refill threshold from the depth of its
+  // scenes' trees, the shading tier and the volume pass from what its scenes
+  // and views hold (a batch's small views do not slow its solo views).
+  const LoopShape ls0 = loop_shape(c, cap);
+  auto run_shape = [&](const Run &run) {
+    LoopShape ls = ls0;
+    ls.tier = kTierLean;
+    uint32_t depth = 0;
+    for (const HostItem &h : run.items) {
+      const phx_context *sc = scenes[h.r.scene];
+      depth = std::max(depth, sc->bvh_depth);
+      ls.volumes |= sc->dev.vol_count > 0 ? 1u : 0u;
+      if (sc->dev.vol_count > 0 || sc->dev.tex_count > 0 || sc->dev.light_count > 0)
+        ls.tier = std::max<uint32_t>(ls.tier, kTierStandard);
+      if (sc->dev.has_sources || h.r.sp.bg_image) ls.tier = kTierFull;
+    }
+    if (!c->tuning.refill_lanes) ls.refill = depth >= kDeepTree ? kRefillDeep : kRefillShallow;
+    return ls;
+  };
+  bool any_volumes = false;
+  for (const phx_context *sc : scenes) any_volumes |= sc->dev.vol_count > 0;
+  if (any_volumes) {
     HIP_TRY(c, c->vol_ev.reserve(size_t(cap) * 16));
     HIP_TRY(c, c->vol_rng.reserve(size_t(cap) * 4));
   }
   // Two spill areas, one per traversal pass: k_shadow runs beside k_extend
   // on the second stream, and a thread of each may spill at the same index.
-  const size_t trace_threads = size_t(ls.trace_grid) * ls.trace_block;
+  const size_t trace_threads = size_t(ls0.trace_grid) * ls0.trace_block;
   int r = ensure_trace_scratch(c, 2 * trace_threads);
   if (r) return r;
 
@@ -399,8 +427,8 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   q.stats = c->stats_buf.as<unsigned long long>();
   q.spill = c->spill.as<int>();
   q.spill_shadow = q.spill + trace_threads * (kMaxStack - kShortStack);
-  q.vol_ev = ls.volumes ? c->vol_ev.as<float4>() : nullptr;
-  q.vol_rng = ls.volumes ? c->vol_rng.as<uint32_t>() : nullptr;
+  q.vol_ev = any_volumes ? c->vol_ev.as<float4>() : nullptr;
+  q.vol_rng = any_volumes ? c->vol_rng.as<uint32_t>() : nullptr;
 
   hipStream_t st = c->stream;
   // PHX_TRACE_COUNTS=N: read the queue length every N iterations and print it.
@@ -420,6 +448,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   std::vector<uint32_t> first(pad);
   for (const Run &run : runs) {
     if (run.items.empty()) continue;
+    const LoopShape ls = run_shape(run);
     ViewArgs va;
     std::memset(&va, 0, sizeof va);
     if (run.items.size() == 1) {

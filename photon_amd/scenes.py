@@ -523,12 +523,12 @@ def colored_noise_texture(size, seed, octaves=3):
 
 # --------------------------------------------------------------------------- volume workload
 def fog_box(n=256, seed=7, threshold=0.4, scale=20.0, macro=32, albedo=0.9, env=True, env_size=(1024, 512),
-            majorants_fn=None):
+            majorants_fn=None, tf_size=256):
     """A 256^3 fBm density grid in an open box: floor, back and left walls,
     one quad area light and the sky environment.  The transfer function maps
     the normalized field to extinction (0 below `threshold`, ramping to 1),
-    density_scale = `scale`; the scattering albedo is `albedo`.
-    `majorants_fn(volume, values, tf)` builds the macro-cell grid (the
+    density_scale = `scale`; the scattering albedo is `albedo`, sampled in
+    `tf_size` entries. `majorants_fn(volume, values, tf)` builds the macro-cell grid (the
     library's phx_volume_majorants is used when it is None)."""
     b = _MeshBuilder()
     mats = [disney((0.6, 0.6, 0.6), 1.0), emitter((30.0, 28.0, 25.0))]
@@ -552,11 +552,11 @@ def fog_box(n=256, seed=7, threshold=0.4, scale=20.0, macro=32, albedo=0.9, env=
     vol.density_scale = scale
     vol.linear = 1
     vol.macro_dims[:] = (macro, macro, macro)
-    x = np.linspace(0, 1, 256)
-    tf = np.zeros((256, 4), np.float32)
+    x = np.linspace(0, 1, tf_size)
+    tf = np.zeros((tf_size, 4), np.float32)
     tf[:, :3] = albedo
     tf[:, 3] = np.clip((x - threshold) / (1 - threshold), 0, 1)
-    vol.tf_size = 256
+    vol.tf_size = tf_size
     if majorants_fn is None:
         from ._lib import volume_majorants
         majorants_fn = volume_majorants

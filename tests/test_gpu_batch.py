@@ -193,17 +193,26 @@ def _overlapping_soup(n, seed):
                         material_ids=mids)
 
 
-def test_forked_bounce_loop_equals_serial(monkeypatch):
+@pytest.mark.parametrize("which", ["soup", "fog"])
+def test_forked_bounce_loop_equals_serial(monkeypatch, which):
     # The any-hit pass runs on a second stream beside the closest-hit and
     # shading passes; on one stream (PHX_SERIAL_BOUNCE=1) the same launches run
-    # in turn.  Both traversal passes spill deep stacks here, each to its own
-    # area, and the images must agree bit for bit.
+    # in turn.  In the soup both traversal passes spill deep stacks, each to
+    # its own area; in the fog box the second stream runs the shadow rays'
+    # ratio tracking (k_shadow_transmit), which scales the shadow
+    # contributions in place.  The images must agree bit for bit.
     from photon_amd.backend import Context, HipBackend, PathTracer, make_camera
-    s = _overlapping_soup(6000, seed=11)
     pt = PathTracer(HipBackend(context=Context(0)))
-    pt.set_scene(s)
-    pt.camera = make_camera((0.3, 0.2, 4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, 1.0)
-    pt.params = abi.default_render_params(96, 96, 8, 8)
+    if which == "soup":
+        s = _overlapping_soup(6000, seed=11)
+        pt.set_scene(s)
+        pt.camera = make_camera((0.3, 0.2, 4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, 1.0)
+        pt.params = abi.default_render_params(96, 96, 8, 8)
+    else:
+        s = scenes.fog_box(n=32, macro=8)
+        pt.set_scene(s)
+        pt.camera = make_camera(s.camera["look_from"], s.camera["look_at"], s.camera["vup"], s.camera["vfov"], 1.0)
+        pt.params = abi.default_render_params(96, 96, 8, 12)
     forked = pt.render()
     assert pt.ctx.stats().shadow_rays > 0
     monkeypatch.setenv("PHX_SERIAL_BOUNCE", "1")

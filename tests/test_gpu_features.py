@@ -167,6 +167,30 @@ def test_fog_volume_bounce_limit_matches_oracle(gpu_ctx):
     _render_pair(gpu_ctx, s, cam, p)
 
 
+@pytest.mark.parametrize("tf_size", [257, 512])
+def test_transfer_function_beyond_lds_matches_oracle_and_batch(gpu_ctx, tf_size):
+    # transfer functions of more than 256 entries stay in global memory (up
+    # to 256 are staged in LDS): the fog box with a 257- and a 512-entry
+    # table, whose albedo varies along the table so that a wrong entry shows,
+    # against the oracle, and as a one-view batch against the plain render
+    from photon_amd.backend import BatchRenderer, Context, make_camera
+    s = scenes.fog_box(n=32, macro=8, env_size=(64, 32), tf_size=tf_size)
+    tf = s.volume_tf.reshape(-1, 4)
+    tf[:, 0] = np.linspace(0.3, 0.95, tf_size)
+    tf[:, 2] = np.linspace(0.95, 0.4, tf_size)
+    s.volume_tf = tf.reshape(-1)
+    s.name = f"fog_tf{tf_size}"
+    cam = make_camera(s.camera["look_from"], s.camera["look_at"], s.camera["vup"], s.camera["vfov"], 1.0)
+    p = abi.default_render_params(40, 40, 4, 12)
+    res, _ = _render_pair(gpu_ctx, s, cam, p)
+    from photon_amd.backend import HipBackend, PathTracer
+    pt = PathTracer(HipBackend(context=Context(0)))
+    pt.set_scene(s)
+    br = BatchRenderer(Context(0))
+    br.render([(pt, cam, p)])
+    np.testing.assert_array_equal(br.read(0).color, res.color)
+
+
 @pytest.mark.parametrize("kind", ["thin_lens", "orthographic", "clipped", "region_bottom_up"])
 def test_cameras_match_oracle(gpu_ctx, kind):
     from photon_amd.backend import make_orthographic_camera
