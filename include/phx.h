@@ -520,6 +520,43 @@ int phx_camera_orthographic(phx_camera *out, const float look_from[3], const flo
 int phx_volume_majorants(phx_volume *vol, const float *values, const float *tf_rgba,
                          uint32_t tf_size, float *out, uint32_t capacity, uint32_t *out_count);
 
+/* ---- display: tone mapping and the denoiser seam --------------------------
+ * The reference's display transforms, applied per pixel on the GPU to a
+ * radiance image: exposure_adjust (c * 2^exposure), then aces_tonemap or
+ * reinhard_tonemap (tone_mapping.h:7-21), then a transfer curve
+ * (gamma_correct, the piecewise sRGB curve of tone_mapping.h:23-28, or
+ * linear_to_srgb, c^(1/2.2) of color.h:9-12), with clamp01 (math.h:86-90)
+ * after the tone map and after the curve when `clamp` is set, as
+ * photon_render's postprocess does (photon_render.cpp:29-35: ACES, clamp,
+ * sRGB curve, clamp).  The 8-bit output is to_u8 (photon_render.cpp:24-27):
+ * clamp01(x) * 255.999, truncated. */
+#define PHX_TONEMAP_NONE 0
+#define PHX_TONEMAP_ACES 1
+#define PHX_TONEMAP_REINHARD 2
+#define PHX_CURVE_LINEAR 0
+#define PHX_CURVE_SRGB 1  /* gamma_correct */
+#define PHX_CURVE_GAMMA22 2 /* linear_to_srgb */
+typedef struct phx_display {
+  int32_t tonemap;  /* PHX_TONEMAP_* */
+  int32_t curve;    /* PHX_CURVE_* */
+  float exposure;   /* stops; 0 leaves the radiance as is */
+  uint8_t clamp;    /* clamp01 after the tone map and after the curve */
+  uint8_t pad_[3];
+} phx_display;
+/* d_rgb: `pixels` pixels of `in_channels` (3 or 4; a fourth channel is
+ * skipped) floats in device memory; writes d_out_rgb (pixels * 3 floats)
+ * and/or d_out_rgb8 (pixels * 3 bytes), either may be NULL.  Runs on
+ * `stream` (NULL: the context's stream), asynchronously. */
+int phx_display_transform(phx_context *ctx, const float *d_rgb, uint32_t in_channels, uint64_t pixels,
+                          const phx_display *display, float *d_out_rgb, uint8_t *d_out_rgb8, void *stream);
+/* Denoiser::is_available / denoise_buffer (denoiser.h:14-31): the reference
+ * wraps Intel Open Image Denoise when it is built with it; this build has no
+ * OIDN, so, as the reference without it, is_available is 0 and
+ * denoise_buffer leaves the colour as it is (color, albedo and normal are
+ * w*h*3 host floats). */
+int phx_denoiser_is_available(void);
+int phx_denoise_buffer(float *color, const float *albedo, const float *normal, uint32_t width, uint32_t height);
+
 /* ---- seam B: photon::bridge::Renderer (RenderBridge.h:175-214) ----------
  * Progressive accumulation over frames (RenderBridge.cpp:828-928): `reset`
  * drops the accumulated samples; sample_offset advances by the frame's spp.

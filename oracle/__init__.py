@@ -66,6 +66,7 @@ def lib():
         L.orc_texture_sample.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_int, P]
         L.orc_sampler_sample.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_int, P, C.c_int, P]
         L.orc_surface_resolve.argtypes = [P, C.c_uint32, C.c_double, C.c_double, P, P, P, P, P, P]
+        L.orc_display.argtypes = [P, C.c_uint32, C.c_uint64, P, P, P]
         _lib = L
     return _lib
 
@@ -330,3 +331,15 @@ def sampler_sample(infos, texels, sampler_id, attr, use_double=True):
     lib().orc_sampler_sample(C.addressof(arr), len(infos), _p(tx), 0 if tx is None else tx.shape[0], sampler_id,
                              _p(attr), int(use_double), _p(out))
     return out
+
+
+def display(rgb, t):
+    """orc_display: the display transform t (an abi.Display) of an (..., 3) or
+    (..., 4) radiance array; returns (float rgb, uint8 rgb)."""
+    rgb = _c(rgb, np.float32)
+    ch = rgb.shape[-1]
+    n = rgb.size // ch
+    out = np.zeros(rgb.shape[:-1] + (3,), np.float32)
+    out8 = np.zeros(rgb.shape[:-1] + (3,), np.uint8)
+    lib().orc_display(_p(rgb), ch, n, C.addressof(t), _p(out), _p(out8))
+    return out, out8

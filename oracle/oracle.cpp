@@ -1731,4 +1731,33 @@ double orc_render(void *sp, const phx_camera *cam, const phx_render_params *rp, 
   return omp_get_wtime() - t0;
 }
 
+// Display transforms (tone_mapping.h:7-28, color.h:9-12) in photon_render's
+// postprocess order (photon_render.cpp:29-35) with its to_u8 (:24-27):
+// exposure factor pow(2, e), tone map, clamp01, transfer curve, clamp01.
+void orc_display(const float *rgb, uint32_t channels, uint64_t n, const phx_display *t, float *out, uint8_t *out8) {
+  const float scale = std::pow(2.f, t->exposure);  // exposure_adjust (tone_mapping.h:17-20)
+  auto c01 = [](float x) { return x < 0.f ? 0.f : (x > 1.f ? 1.f : x); };
+  for (uint64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      float c = rgb[i * channels + k] * scale;
+      if (t->tonemap == PHX_TONEMAP_ACES) {  // tone_mapping.h:7-12
+        const float a = 2.51f, b = 0.03f, cc = 2.43f, d = 0.59f, e = 0.14f;
+        const float num = c * (c * a + b);
+        const float den = c * (c * cc + d) + e;
+        c = num / den;
+      } else if (t->tonemap == PHX_TONEMAP_REINHARD) {  // :14-16
+        c = c / (1.f + c);
+      }
+      if (t->clamp) c = c01(c);
+      if (t->curve == PHX_CURVE_SRGB)  // gamma_correct, :22-28
+        c = c <= 0.0031308f ? 12.92f * c : 1.055f * std::pow(c, 1.f / 2.4f) - 0.055f;
+      else if (t->curve == PHX_CURVE_GAMMA22)  // linear_to_srgb, color.h:9-12
+        c = std::pow(c, 1.f / 2.2f);
+      if (t->clamp) c = c01(c);
+      if (out) out[i * 3 + k] = c;
+      if (out8) out8[i * 3 + k] = static_cast<uint8_t>(c01(c) * 255.999f);
+    }
+  }
+}
+
 }  // extern "C"

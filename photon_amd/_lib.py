@@ -39,6 +39,9 @@ def _sig(lib):
         "phx_trace_closest_compact": (C.c_int, [P, P, P, P, P, C.c_uint32, P, P]),
         "phx_trace_occluded": (C.c_int, [P, P, P, P, P, C.c_uint32, P, P]),
         "phx_trace_closest_noalloc": (C.c_int, [P, P, P, P, P, C.c_uint32, P, P]),
+        "phx_display_transform": (C.c_int, [P, P, C.c_uint32, C.c_uint64, C.POINTER(abi.Display), P, P, P]),
+        "phx_denoiser_is_available": (C.c_int, []),
+        "phx_denoise_buffer": (C.c_int, [P, P, P, C.c_uint32, C.c_uint32]),
         "phx_synchronize": (C.c_int, [P]),
         "phx_bsdf_eval": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P]),
         "phx_bsdf_sample": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P, P, P]),

@@ -264,8 +264,27 @@ class View(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("params", RenderParams)]
 
 
+class Display(C.Structure):
+    """phx_display: exposure, tone map and transfer curve of phx_display_transform."""
+    _fields_ = [("tonemap", C.c_int32), ("curve", C.c_int32), ("exposure", C.c_float), ("clamp", C.c_uint8),
+                ("pad_", C.c_uint8 * 3)]
+
+
+PHX_TONEMAP_NONE, PHX_TONEMAP_ACES, PHX_TONEMAP_REINHARD = 0, 1, 2
+PHX_CURVE_LINEAR, PHX_CURVE_SRGB, PHX_CURVE_GAMMA22 = 0, 1, 2
+
+
+def display(tonemap=PHX_TONEMAP_ACES, curve=PHX_CURVE_SRGB, exposure=0.0, clamp=1):
+    """A Display; the defaults are photon_render's postprocess (ACES, clamp,
+    sRGB curve, clamp: photon_render.cpp:29-35)."""
+    d = Display()
+    d.tonemap, d.curve, d.exposure, d.clamp = tonemap, curve, exposure, clamp
+    return d
+
+
 EXPECTED_SIZES = {Material: 200, Light: 92, TextureInfo: 16, Volume: 132, SamplerInfo: 200, Hit: 88, HitCompact: 16,
-                  Camera: 104, Tuning: 32, BridgeMaterial: 132, CameraData: 84, RenderSettings: 96, LightData: 56}
+                  Camera: 104, Tuning: 32, BridgeMaterial: 132, CameraData: 84, RenderSettings: 96, LightData: 56,
+                  Display: 16}
 
 
 def check_layout():

@@ -37,6 +37,7 @@ class Context:
         if code != 0:
             raise PhxError(code, "phx_create failed (no HIP device?)")
         self.h = h
+        self.device = device
 
     def close(self):
         if getattr(self, "h", None):
@@ -246,3 +247,35 @@ class PathTracer:
         p = C.c_void_p()
         self.ctx.check(lib().phx_result_device_color(self.ctx.h, C.byref(p)))
         return p.value
+
+    def display(self, display=None):
+        """The last frame through phx_display_transform on the GPU (default:
+        photon_render's postprocess, ACES + sRGB curve with clamps): returns
+        (float32 (h, w, 3), uint8 (h, w, 3)) torch tensors on the device."""
+        import torch
+        h, w = self.params.height, self.params.width
+        out = torch.empty((h, w, 3), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        out8 = torch.empty((h, w, 3), dtype=torch.uint8, device=out.device)
+        d = display if display is not None else abi.display()
+        self.ctx.check(lib().phx_display_transform(self.ctx.h, self.device_color(), 3, h * w, C.byref(d),
+                                                   out.data_ptr(), out8.data_ptr(), None))
+        self.ctx.synchronize()
+        return out, out8
+
+
+def display_transform(ctx, rgb, display=None):
+    """phx_display_transform of a (..., 3) or (..., 4) float32 CUDA tensor:
+    returns (float32 (..., 3), uint8 (..., 3)) tensors on the same device."""
+    import torch
+    if not (rgb.is_cuda and rgb.dtype == torch.float32 and rgb.shape[-1] in (3, 4)):
+        raise ValueError("display_transform: a float32 CUDA tensor of 3 or 4 channels")
+    rgb = rgb.contiguous()
+    ch = rgb.shape[-1]
+    n = rgb.numel() // ch
+    out = torch.empty(rgb.shape[:-1] + (3,), dtype=torch.float32, device=rgb.device)
+    out8 = torch.empty(rgb.shape[:-1] + (3,), dtype=torch.uint8, device=rgb.device)
+    d = display if display is not None else abi.display()
+    ctx.check(lib().phx_display_transform(ctx.h, rgb.data_ptr(), ch, n, C.byref(d), out.data_ptr(),
+                                          out8.data_ptr(), None))
+    ctx.synchronize()
+    return out, out8

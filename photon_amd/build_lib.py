@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libphx.so")
-SOURCES = ["adjoint.hip", "capi.cpp", "bridge.hip", "bsdf_batch.hip", "bvh_build.cpp", "integrator.hip", "noalloc.hip", "proxy.hip", "proxy_bvh.cpp", "scene_data.cpp",
+SOURCES = ["adjoint.hip", "capi.cpp", "bridge.hip", "bsdf_batch.hip", "bvh_build.cpp", "display.hip", "integrator.hip", "noalloc.hip", "proxy.hip", "proxy_bvh.cpp", "scene_data.cpp",
            "render.cpp", "trace.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
