@@ -777,58 +777,112 @@ PHX_DEV float4 vol_classify(const DevScene &s, const phx_volume &v, vec3 p) {
   c.w = (a.w * (1.f - fr) + b.w * fr) * v.density_scale;
   return c;
 }
-PHX_DEV float vol_track(const DevScene &s, const phx_volume &v, vec3 org, vec3 dir, float t0, float t1, Rng &rng,
-                        bool scatter, vec3 &albedo, float &tr) {
-  const float o[3] = {org.x, org.y, org.z}, d[3] = {dir.x, dir.y, dir.z};
-  int cell[3], step[3];
-  float t_next[3], t_delta[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int m = int(v.macro_dims[a]);
-    const float extent = v.spacing[a] * float(v.dims[a] > 1 ? v.dims[a] - 1 : 1);
-    const float cpu = float(m) / extent;
-    const float g = (o[a] + d[a] * t0 - v.origin[a]) * cpu;
-    int c = int(g);
-    c = c < 0 ? 0 : (c > m - 1 ? m - 1 : c);
-    cell[a] = c;
-    const float speed = d[a] * cpu;
-    if (speed > 0.f) {
-      step[a] = 1;
-      t_delta[a] = 1.f / speed;
-      t_next[a] = t0 + (float(c + 1) - g) * t_delta[a];
-    } else if (speed < 0.f) {
-      step[a] = -1;
-      t_delta[a] = -1.f / speed;
-      t_next[a] = t0 + (g - float(c)) * t_delta[a];
+// A ray's walk through a volume's macro-cell grid, one cell at a time (a 3-D
+// DDA).  Per axis: `at` is the cell, `dir` the step (+1, -1 or 0), `face` the
+// ray parameter at which the walk leaves the cell through that axis's next
+// face and `stride` the parameter between two such faces.  The arithmetic is
+// the reference's (tf_volume.h:150-186), so the cells and their boundaries
+// fall where its do.  (Three named axes rather than arrays: arrays indexed
+// by the loop variable kept the walk and the volume record in scratch.)
+struct WalkAxis {
+  int at, dir;
+  float face, stride;
+  PHX_DEV void start(uint32_t cells, uint32_t dims, float spacing, float origin, float o, float d, float t0) {
+    const float extent = spacing * float(dims > 1 ? dims - 1 : 1);
+    const float per_unit = float(int(cells)) / extent;  // cells per unit length
+    const float g = (o + d * t0 - origin) * per_unit;
+    const int c = int(g);
+    at = c < 0 ? 0 : (c > int(cells) - 1 ? int(cells) - 1 : c);
+    const float rate = d * per_unit;  // cells per unit of t
+    if (rate > 0.f) {
+      dir = 1;
+      stride = 1.f / rate;
+      face = t0 + (float(at + 1) - g) * stride;
+    } else if (rate < 0.f) {
+      dir = -1;
+      stride = -1.f / rate;
+      face = t0 + (g - float(at)) * stride;
     } else {
-      step[a] = 0;
-      t_delta[a] = 1e30f;
-      t_next[a] = 1e30f;
+      dir = 0;
+      stride = 1e30f;
+      face = 1e30f;
     }
   }
+  // Moves to the neighbouring cell when `go` (by value selects: branches per
+  // axis were merged by the compiler into one update through a selected
+  // address, which put the walk in scratch memory).
+  PHX_DEV void step_if(bool go) {
+    at = go ? at + dir : at;
+    face = go ? face + stride : face;
+  }
+};
+struct MacroWalk {
+  WalkAxis x, y, z;
+  PHX_DEV void start(const phx_volume &v, vec3 o, vec3 d, float t0) {
+    x.start(v.macro_dims[0], v.dims[0], v.spacing[0], v.origin[0], o.x, d.x, t0);
+    y.start(v.macro_dims[1], v.dims[1], v.spacing[1], v.origin[1], o.y, d.y, t0);
+    z.start(v.macro_dims[2], v.dims[2], v.spacing[2], v.origin[2], o.z, d.z, t0);
+  }
+  // the axis whose face comes first (ties go to the later axis, as the
+  // reference's comparisons do)
+  PHX_DEV int next_axis() const { return x.face < y.face ? (x.face < z.face ? 0 : 2) : (y.face < z.face ? 1 : 2); }
+  // that face's parameter (the smallest of the three; no NaN arises here)
+  PHX_DEV float next_face() const { return fminf(fminf(x.face, y.face), z.face); }
+  PHX_DEV uint32_t cell_index(const uint32_t m[3]) const {
+    return (uint32_t(z.at) * m[1] + uint32_t(y.at)) * m[0] + uint32_t(x.at);
+  }
+  // Crosses into the neighbour along `ax` of a grid of m[3] cells; false
+  // (and no move) when that leaves the grid.
+  PHX_DEV bool cross(const uint32_t m[3], int ax) {
+    const bool ox = ax == 0, oy = ax == 1, oz = !ox && !oy;
+    const int next = ox ? x.at + x.dir : (oy ? y.at + y.dir : z.at + z.dir);
+    const int cells = int(ox ? m[0] : (oy ? m[1] : m[2]));
+    if (next < 0 || next >= cells) return false;
+    x.step_if(ox);
+    y.step_if(oy);
+    z.step_if(oz);
+    return true;
+  }
+};
+
+// Bounds of the walk: cells per ray and density lookups per ray
+// (tf_volume.h:188-190).
+constexpr int kMaxWalkCells = 4096, kMaxLookups = 8192;
+
+// Tracking along [t0, t1] of one volume in its local frame, cell by cell
+// with each cell's majorant (tf_volume.h:192-283).  SCATTER: delta tracking,
+// returns the first real collision's distance (its albedo in `albedo`) or
+// -1.  Otherwise ratio tracking: scales `tr` by the transmittance, with
+// Russian roulette below 0.05, and returns -1 (tr = 0 when the roulette ends
+// the ray).  Each cell entered with a majorant draws a fresh free-flight
+// distance; a distance past the cell's far face is dropped at the face.
+template <bool SCATTER>
+PHX_DEV float vol_track(const DevScene &s, const phx_volume &v, vec3 org, vec3 dir, float t0, float t1, Rng &rng,
+                        vec3 &albedo, float &tr) {
+  MacroWalk w;
+  w.start(v, org, dir, t0);
+  const uint32_t cells_of[3] = {v.macro_dims[0], v.macro_dims[1], v.macro_dims[2]};
   float t = t0;
   int lookups = 0;
-  for (int visited = 0; visited < 4096; ++visited) {
-    const int axis = t_next[0] < t_next[1] ? (t_next[0] < t_next[2] ? 0 : 2) : (t_next[1] < t_next[2] ? 1 : 2);
-    const float tna = axis == 0 ? t_next[0] : (axis == 1 ? t_next[1] : t_next[2]);
-    const float t_exit = tna < t1 ? tna : t1;
-    const float bound =
-        s.majorants[v.macro_offset + (uint32_t(cell[2]) * v.macro_dims[1] + uint32_t(cell[1])) * v.macro_dims[0] +
-                    uint32_t(cell[0])];
-    if (bound > 0.f) {
-      const float inv_bound = 1.f / bound;
-      while (lookups < 8192) {
-        t -= logf(1.f - rng.next()) * inv_bound;
-        if (t >= t_exit) break;
+  for (int cells = 0; cells < kMaxWalkCells; ++cells) {
+    const int ax = w.next_axis();
+    const float face = w.next_face();
+    const float seg_end = face < t1 ? face : t1;
+    const float sigma_max = s.majorants[v.macro_offset + w.cell_index(cells_of)];
+    if (sigma_max > 0.f) {
+      const float rcp_max = 1.f / sigma_max;
+      while (lookups < kMaxLookups) {
+        t -= logf(1.f - rng.next()) * rcp_max;
+        if (t >= seg_end) break;
         ++lookups;
         const float4 c = vol_classify(s, v, org + dir * t);
-        if (scatter) {
-          if (rng.next() * bound < c.w) {
+        if (SCATTER) {
+          if (rng.next() * sigma_max < c.w) {  // a real collision
             albedo = {c.x, c.y, c.z};
             return t;
           }
         } else {
-          tr *= 1.f - c.w * inv_bound;
+          tr *= 1.f - c.w * rcp_max;
           if (tr < 0.05f) {
             if (!(tr > 0.f) || rng.next() * 0.05f >= tr) {
               tr = 0.f;
@@ -839,55 +893,42 @@ PHX_DEV float vol_track(const DevScene &s, const phx_volume &v, vec3 org, vec3 d
         }
       }
     }
-    t = t_exit;
-    if (t_exit >= t1 || lookups >= 8192) break;
-    // the step along `axis`, with selects: an index computed at run time into
-    // these arrays (or into v.macro_dims) would put them in scratch memory
-    const bool a0 = axis == 0, a1 = axis == 1;
-    const int ca = (a0 ? cell[0] : (a1 ? cell[1] : cell[2])) + (a0 ? step[0] : (a1 ? step[1] : step[2]));
-    const int ma = int(a0 ? v.macro_dims[0] : (a1 ? v.macro_dims[1] : v.macro_dims[2]));
-    if (ca < 0 || ca >= ma) break;
-    if (a0) {
-      cell[0] = ca;
-      t_next[0] += t_delta[0];
-    } else if (a1) {
-      cell[1] = ca;
-      t_next[1] += t_delta[1];
-    } else {
-      cell[2] = ca;
-      t_next[2] += t_delta[2];
-    }
+    t = seg_end;
+    if (seg_end >= t1 || lookups >= kMaxLookups || !w.cross(cells_of, ax)) break;
   }
   return -1.f;
 }
+// The nearest real collision in front of tmax over all volumes (each one
+// tracked only up to the nearest collision found so far), or -1.
 PHX_DEV float vol_sample_scatter(const DevScene &s, vec3 ro, vec3 rd, float tmin, float tmax, Rng &rng, vec3 &albedo) {
-  float t_event = -1.f, t_limit = tmax, unused = 1.f;
+  float nearest = -1.f, limit = tmax, tr_unused = 1.f;
   for (uint32_t i = 0; i < s.vol_count; ++i) {
     const phx_volume v = s.volumes[i];
     if (v.majorant <= 0.f || v.tf_size == 0) continue;
     const vec3 org = to_local_point(v, ro), dir = to_local_vector(v, rd);
-    float t0 = tmin, t1 = t_limit;
+    float t0 = tmin, t1 = limit;
     if (!vol_clip(v, org, dir, t0, t1)) continue;
-    vec3 ea{0.f, 0.f, 0.f};
-    const float t = vol_track(s, v, org, dir, t0, t1, rng, true, ea, unused);
+    vec3 a{0.f, 0.f, 0.f};
+    const float t = vol_track<true>(s, v, org, dir, t0, t1, rng, a, tr_unused);
     if (t >= 0.f) {
-      t_event = t;
-      t_limit = t;
-      albedo = ea;
+      nearest = t;
+      limit = t;
+      albedo = a;
     }
   }
-  return t_event;
+  return nearest;
 }
+// Transmittance along [tmin, tmax] through every volume.
 PHX_DEV float vol_transmittance(const DevScene &s, vec3 ro, vec3 rd, float tmin, float tmax, Rng &rng) {
   float tr = 1.f;
-  vec3 unused{0.f, 0.f, 0.f};
+  vec3 albedo_unused{0.f, 0.f, 0.f};
   for (uint32_t i = 0; i < s.vol_count; ++i) {
     const phx_volume v = s.volumes[i];
     if (v.majorant <= 0.f || v.tf_size == 0) continue;
     const vec3 org = to_local_point(v, ro), dir = to_local_vector(v, rd);
     float t0 = tmin, t1 = tmax;
     if (!vol_clip(v, org, dir, t0, t1)) continue;
-    vol_track(s, v, org, dir, t0, t1, rng, false, unused, tr);
+    vol_track<false>(s, v, org, dir, t0, t1, rng, albedo_unused, tr);
     if (tr <= 0.f) return 0.f;
   }
   return tr;
