@@ -557,6 +557,34 @@ int phx_display_transform(phx_context *ctx, const float *d_rgb, uint32_t in_chan
 int phx_denoiser_is_available(void);
 int phx_denoise_buffer(float *color, const float *albedo, const float *normal, uint32_t width, uint32_t height);
 
+/* ---- pbrt-v4 scene files ----------------------------------------------------
+ * parse_pbrt_file (pbrt_parser.cpp:233) followed by convert_pbrt_scene
+ * (pbrt_to_photon.cpp:58-503): named materials, triangle meshes, PLY meshes,
+ * spheres, area lights, image textures, an infinite light with an image,
+ * object instances (flattened, within the reference's limits of 500
+ * instances per object and 5e8 triangles; with use_instancing they are only
+ * counted), Include/Import, the camera and the Film/Sampler/Integrator
+ * settings.  Image files: PFM, binary PNM, Radiance HDR and PNG; others load
+ * as missing textures (left out, as the reference does with files it cannot
+ * read).  A missing scene file gives an empty scene, as in the reference:
+ * its mesh is one degenerate triangle that no ray hits.  The handle owns
+ * the converted arrays. */
+typedef struct phx_pbrt_scene phx_pbrt_scene;
+typedef struct phx_pbrt_info {
+  uint32_t width, height, spp, max_depth; /* Film, Sampler, Integrator */
+  uint32_t meshes;    /* meshes placed in the scene (instances flattened) */
+  uint32_t objects;   /* ObjectBegin definitions */
+  uint32_t instances; /* ObjectInstance uses */
+  uint32_t textures;  /* image textures loaded */
+} phx_pbrt_info;
+int phx_pbrt_load(const char *path, int use_instancing, phx_pbrt_scene **out);
+/* The converted scene: `desc` points into the handle's arrays (valid until
+ * phx_pbrt_free; pass it to phx_set_scene), `camera` is
+ * Camera::make_perspective of the file's camera; any output may be NULL. */
+int phx_pbrt_scene_desc(const phx_pbrt_scene *scene, phx_scene_desc *desc, phx_camera *camera,
+                        phx_pbrt_info *info);
+void phx_pbrt_free(phx_pbrt_scene *scene);
+
 /* ---- seam B: photon::bridge::Renderer (RenderBridge.h:175-214) ----------
  * Progressive accumulation over frames (RenderBridge.cpp:828-928): `reset`
  * drops the accumulated samples; sample_offset advances by the frame's spp.

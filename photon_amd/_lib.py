@@ -42,6 +42,10 @@ def _sig(lib):
         "phx_display_transform": (C.c_int, [P, P, C.c_uint32, C.c_uint64, C.POINTER(abi.Display), P, P, P]),
         "phx_denoiser_is_available": (C.c_int, []),
         "phx_denoise_buffer": (C.c_int, [P, P, P, C.c_uint32, C.c_uint32]),
+        "phx_pbrt_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(P)]),
+        "phx_pbrt_scene_desc": (C.c_int, [P, C.POINTER(abi.SceneDesc), C.POINTER(abi.Camera),
+                                          C.POINTER(abi.PbrtInfo)]),
+        "phx_pbrt_free": (None, [P]),
         "phx_synchronize": (C.c_int, [P]),
         "phx_bsdf_eval": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P]),
         "phx_bsdf_sample": (C.c_int, [P, P, P, P, P, P, C.c_uint32, P, P, P, P, P]),

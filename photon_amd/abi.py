@@ -270,6 +270,12 @@ class Display(C.Structure):
                 ("pad_", C.c_uint8 * 3)]
 
 
+class PbrtInfo(C.Structure):
+    """phx_pbrt_info: the pbrt file's Film/Sampler/Integrator settings and counts."""
+    _fields_ = [(n, C.c_uint32) for n in ("width", "height", "spp", "max_depth", "meshes", "objects", "instances",
+                                           "textures")]
+
+
 PHX_TONEMAP_NONE, PHX_TONEMAP_ACES, PHX_TONEMAP_REINHARD = 0, 1, 2
 PHX_CURVE_LINEAR, PHX_CURVE_SRGB, PHX_CURVE_GAMMA22 = 0, 1, 2
 
@@ -284,7 +290,7 @@ def display(tonemap=PHX_TONEMAP_ACES, curve=PHX_CURVE_SRGB, exposure=0.0, clamp=
 
 EXPECTED_SIZES = {Material: 200, Light: 92, TextureInfo: 16, Volume: 132, SamplerInfo: 200, Hit: 88, HitCompact: 16,
                   Camera: 104, Tuning: 32, BridgeMaterial: 132, CameraData: 84, RenderSettings: 96, LightData: 56,
-                  Display: 16}
+                  Display: 16, PbrtInfo: 32}
 
 
 def check_layout():

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libphx.so")
-SOURCES = ["adjoint.hip", "capi.cpp", "bridge.hip", "bsdf_batch.hip", "bvh_build.cpp", "display.hip", "integrator.hip", "noalloc.hip", "proxy.hip", "proxy_bvh.cpp", "scene_data.cpp",
+SOURCES = ["adjoint.hip", "capi.cpp", "bridge.hip", "bsdf_batch.hip", "bvh_build.cpp", "display.hip", "integrator.hip", "noalloc.hip", "pbrt.cpp", "proxy.hip", "proxy_bvh.cpp", "scene_data.cpp",
            "render.cpp", "trace.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -41,7 +41,8 @@ def build(verbose=False, jobs=None):
     if failed:
         raise RuntimeError("building libphx.so failed")
     tmp = OUT + ".tmp"
-    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    # zlib: the PNG reader of the pbrt scene loader
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError("linking libphx.so failed:\n" + r.stdout.decode(errors="replace"))

@@ -59,6 +59,10 @@ class Scene:
         self.samplers = []          # abi.SamplerInfo
         self.sampler_texels = None  # (N, 4) float32
         self.ray_epsilon = 0.0
+        # explicit emissive primitives and their areas (None: derived from the
+        # materials by the library)
+        self.emissive_ids = None
+        self.emissive_areas = None
         self.camera = None          # dict(look_from, look_at, vup, vfov, aspect)
         self.name = "scene"
 
@@ -96,6 +100,12 @@ class Scene:
             self._keep.append(lights)
             d.lights = C.cast(lights, C.POINTER(abi.Light))
         d.light_count = len(self.lights)
+        if self.emissive_ids is not None and len(self.emissive_ids):
+            self.emissive_ids = np.ascontiguousarray(self.emissive_ids, dtype=np.uint32)
+            self.emissive_areas = np.ascontiguousarray(self.emissive_areas, dtype=np.float32)
+            d.emissive_prim_ids = _ptr(self.emissive_ids, C.c_uint32)
+            d.emissive_prim_areas = _ptr(self.emissive_areas, C.c_float)
+            d.emissive_count = self.emissive_ids.size
         if self.texture_infos:
             infos = (abi.TextureInfo * len(self.texture_infos))(
                 *[abi.TextureInfo(o, w, h, 0) for (o, w, h) in self.texture_infos])

@@ -55,8 +55,8 @@ def test_abi_types_equal_the_reference_declarations():
 
 
 def test_layout_check_catches_a_planted_mistake(tmp_path):
-    # the same check against a header whose HitResult bool is widened (the
-    # round-1 layout) must fail to compile
+    # the same check against a header whose HitResult bools are widened to
+    # 32 bits must fail to compile
     bad = tmp_path / "include"
     bad.mkdir()
     text = open(HEADER).read().replace("  uint8_t hit;\n  uint8_t has_interpolated_color;\n  uint8_t pad_[2];",
@@ -168,7 +168,8 @@ def test_ctypes_layouts_equal_the_c_header(tmp_path):
                "phx_bridge_material": abi.BridgeMaterial, "phx_sampler_info": abi.SamplerInfo,
                "phx_param_source": abi.ParamSource, "phx_light_data": abi.LightData,
                "phx_sampler_data": abi.SamplerData, "phx_environment_data": abi.EnvironmentData,
-               "phx_volume_data": abi.VolumeData, "phx_scene_data": abi.SceneData}
+               "phx_volume_data": abi.VolumeData, "phx_scene_data": abi.SceneData, "phx_display": abi.Display,
+               "phx_pbrt_info": abi.PbrtInfo}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "phx.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
