@@ -82,7 +82,7 @@ def batched_form(pt_c1, c1, args):
     solo_ms = sum(per_view[v] for v in solo) * 1e3
     br = BatchRenderer(device=0)
     tuning = None
-    if not args.no_autotune:
+    if args.autotune:
         t, ms = br.autotune(views, repeats=1)
         tuning = {f: getattr(t, f) for f, _ in t._fields_}
         print(f"[bench] autotuned batch shape {tuning}: {ms:.1f} ms", file=sys.stderr, flush=True)
@@ -217,7 +217,9 @@ def main():
                     help="timed frames of the reference-structure GPU baseline (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=3,
                     help="timed repetitions of the batched 16-view launch and of its 16 plain launches (0: skip)")
-    ap.add_argument("--no-autotune", action="store_true", help="batched form with the default launch shape")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batched form with the launch shape phx_autotune picks (default: the library's default "
+                         "shape, which sets the refill threshold per run from its tree's depth)")
     ap.add_argument("--adjoint-reps", type=int, default=3, help="timed primal+adjoint passes (0: skip)")
     ap.add_argument("--volume-reps", type=int, default=2, help="timed frames of the volume workload (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",

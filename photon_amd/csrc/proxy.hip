@@ -7,7 +7,7 @@
 //   - one sample of every pixel per pass; each pass runs max_depth + 9
 //     iterations of trace_closest, pt_shade, trace_occluded and
 //     pt_nee_accum_pack over all W*H slots, finished rays included;
-//   - trace_closest writes the full 96-byte hit record (HitResult) per slot;
+//   - trace_closest writes the full 88-byte hit record (HitResult) per slot;
 //   - the host reads the active count back after every iteration.
 // Shading is the same device code as the streaming backend (shade_path), so
 // both render the same image; only the structure differs.
