@@ -202,6 +202,36 @@ def adjoint_form(args):
             "params": int(params.numel() + tex.numel())}
 
 
+def adaptive_form(pt, args, plain_ms):
+    """The plain workload's view with adaptive sampling on (pathtracer.cpp:
+    676-723; the workload itself is quoted with it off): the frame time, the
+    samples the pixels took and the rate of taken samples."""
+    import numpy as np
+    import torch
+
+    from photon_amd import abi
+    keep = pt.params
+    p = abi.default_render_params(args.width, args.height, args.spp, args.depth)
+    p.adaptive_sampling = 1
+    pt.params = p
+    pt.render(read=False)  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.adaptive_reps):
+        pt.render(read=False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / args.adaptive_reps
+    counts = pt.sample_counts().astype(np.int64)
+    pt.params = keep
+    taken = int(counts.sum())
+    total = args.width * args.height * args.spp
+    print(f"[bench] adaptive: {ms:.1f} ms, {taken / total:.3f} of the samples taken", file=sys.stderr, flush=True)
+    return {"ms_per_frame": ms, "reps": args.adaptive_reps, "plain_ms_per_frame": plain_ms,
+            "samples_taken": taken, "fraction_taken": taken / total,
+            "pixels_stopped_at_16": float(np.mean(counts == 16)),
+            "taken_samples_per_s": taken / (ms / 1e3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +252,8 @@ def main():
                          "shape, which sets the refill threshold per run from its tree's depth)")
     ap.add_argument("--adjoint-reps", type=int, default=3, help="timed primal+adjoint passes (0: skip)")
     ap.add_argument("--volume-reps", type=int, default=2, help="timed frames of the volume workload (0: skip)")
+    ap.add_argument("--adaptive-reps", type=int, default=2,
+                    help="timed frames of the plain view with adaptive sampling on (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -365,6 +397,7 @@ def main():
                          f"{secs:.1f} s)",
                "parity": {"rel_mse_vs_gpu": rel_mse, "pixels_equal_1e-3": equal}}
 
+    adaptive = adaptive_form(pt, args, ms_per_step) if args.adaptive_reps > 0 else None
     batched = batched_form(pt, scene, args) if args.batch_reps > 0 else None
     adjoint = adjoint_form(args) if args.adjoint_reps > 0 else None
     volume = volume_form(args) if args.volume_reps > 0 else None
@@ -400,6 +433,7 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "volume": volume,
+        "adaptive": adaptive,
         "bvh_build_s": build_s,
     }
     print(json.dumps(line))

@@ -286,7 +286,7 @@ typedef struct phx_render_params {
   float background_alpha;
   float ambient_color[3];
   float ambient_radiance;
-  int32_t adaptive_sampling; /* must be 0: the variance-based early exit is not offered */
+  int32_t adaptive_sampling; /* variance-based early exit (pathtracer.cpp:676-723); phx_render only */
   float throughput_clamp;
   int32_t max_diffuse_bounces;
   int32_t max_glossy_bounces;
@@ -423,6 +423,12 @@ int phx_read_result(phx_context *ctx, float *color_rgb, float *alpha, float *dep
                     float *albedo_rgb);
 /* Device pointer of the resolved color (h*w*3 floats), valid until the next render. */
 int phx_result_device_color(phx_context *ctx, const float **d_color);
+/* Samples each pixel took (h*w uint32, rows as rendered, host or device
+ * memory) in the last frame, when phx_render sampled it adaptively; else
+ * PHX_ERR_NO_FRAME.  No counterpart in the reference, whose per-pixel counts
+ * (PathTracer::m_pixel_samples, pathtracer.h:108) are private: it lets a
+ * caller see where the variance-based early exit stopped. */
+int phx_read_sample_counts(phx_context *ctx, uint32_t *counts);
 
 /* ---- batched views ---------------------------------------------------------
  * Renders many views, of one or several scenes, in one wavefront: all views

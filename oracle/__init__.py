@@ -60,7 +60,7 @@ def lib():
         L.orc_bsdf_eval_grad_fd.argtypes = [P, P, P, P, P, C.c_uint32, C.c_double, P]
         L.orc_render_detached.restype = C.c_double
         L.orc_render_detached.argtypes = [P, P, P, P, C.c_uint32, P, P, P, C.c_int]
-        L.orc_render.argtypes = [P, P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_uint32, C.c_uint32]
+        L.orc_render.argtypes = [P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, C.c_uint32, C.c_uint32]
         L.orc_lobe_weights.argtypes = [P, C.c_double, C.c_int, P]
         L.orc_sample_light.argtypes = [P, P, C.c_uint32, P]
         L.orc_texture_sample.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_int, P]
@@ -185,17 +185,21 @@ class OracleScene:
         lib().orc_surface_resolve(self.h, prim, b1, b2, _p(p), _p(sn), _p(uv), C.addressof(m), _p(alpha), _p(n))
         return m, float(alpha[0]), n
 
-    def render(self, camera, params, threads=0, use_double=False, pixel_range=(0, 0), aovs=False):
-        """PathTracer::render on the CPU: returns (color (h,w,3), extras dict, seconds)."""
+    def render(self, camera, params, threads=0, use_double=False, pixel_range=(0, 0), aovs=False, nsamp=False):
+        """PathTracer::render on the CPU: returns (color (h,w,3), extras dict, seconds).
+        extras holds the AOVs with aovs=True and each pixel's sample count
+        ("nsamp", (h,w) uint32) with nsamp=True."""
         h, w = params.height, params.width
         color = np.zeros((h, w, 3), np.float32)
         extras = {}
         if aovs:
             extras = {"alpha": np.zeros((h, w), np.float32), "depth": np.zeros((h, w), np.float32),
                       "normal": np.zeros((h, w, 3), np.float32), "albedo": np.zeros((h, w, 3), np.float32)}
+        if nsamp:
+            extras["nsamp"] = np.zeros((h, w), np.uint32)
         secs = lib().orc_render(self.h, C.addressof(camera), C.addressof(params), _p(color),
                                 _p(extras.get("alpha")), _p(extras.get("depth")), _p(extras.get("normal")),
-                                _p(extras.get("albedo")), int(threads), int(use_double), int(pixel_range[0]),
+                                _p(extras.get("albedo")), _p(extras.get("nsamp")), int(threads), int(use_double), int(pixel_range[0]),
                                 int(pixel_range[1]))
         return color, extras, secs
 

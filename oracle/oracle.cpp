@@ -766,8 +766,22 @@ static void render_pixel(const Scene &s, const Params &P, uint32_t idx, PixelOut
   const V3<T> ambient = v3<T>(rp.ambient_color) * T(rp.ambient_radiance);
   const uint32_t x = idx % w, y = idx / w;
   out = PixelOut<T>{};
+  V3<T> accum_sq{};
   for (uint32_t sidx = 0; sidx < rp.samples_per_pixel; ++sidx) {
     const uint32_t sample = sidx + rp.sample_offset;
+    // adaptive sampling (pathtracer.cpp:676-723): from sample 16 on, a pixel
+    // with 16 or more samples stops once its largest channel variance over
+    // its squared luminance is below 0.01 (its sums no longer change, so it
+    // stays stopped)
+    if (rp.adaptive_sampling && sidx >= 16u && out.ns >= 16u) {
+      const T inv = T(1) / T(out.ns);
+      const V3<T> mean = out.accum * inv, sq = accum_sq * inv;
+      const V3<T> var = sq - V3<T>{mean.x * mean.x, mean.y * mean.y, mean.z * mean.z};
+      const T max_var = fmaxT(var.x, fmaxT(var.y, var.z));
+      const T lum = fmaxT(T(0.2126f) * mean.x + T(0.7152f) * mean.y + T(0.0722f) * mean.z, T(1e-4f));
+      if (max_var / (lum * lum) < T(0.01f)) break;
+    }
+    const V3<T> sample_start = out.accum;
     // pt_init_sample: pathtracer.cpp:727-759
     Rng crng = make_rng(idx, sample, 0u);
     T u = (T(x) + draw<T>(crng)) / T(w);
@@ -1001,6 +1015,11 @@ static void render_pixel(const Scene &s, const Params &P, uint32_t idx, PixelOut
           if (c.x == c.x && c.y == c.y && c.z == c.z) out.accum = out.accum + c;
         }
       }
+    }
+    // pt_update_variance (pathtracer.cpp:1129-1139)
+    if (rp.adaptive_sampling) {
+      const V3<T> sv = out.accum - sample_start;
+      accum_sq = accum_sq + V3<T>{sv.x * sv.x, sv.y * sv.y, sv.z * sv.z};
     }
   }
 }
@@ -1686,11 +1705,13 @@ double orc_render_detached(void *sp, const phx_camera *cam, const phx_render_par
 
 // PathTracer::render (pathtracer.cpp:564-1159) on the CPU with `threads`
 // OpenMP threads (0: all).  Outputs are W*H rows as rendered (row y = idx / W):
-// color rgb, alpha, depth, normal, albedo (any may be NULL).  `pixel_begin`,
+// color rgb, alpha, depth, normal, albedo, the samples each pixel took (any
+// may be NULL).  `pixel_begin`,
 // `pixel_end` restrict the work to a pixel range (bounded CPU baselines).
 // Returns the wall time in seconds.
 double orc_render(void *sp, const phx_camera *cam, const phx_render_params *rp, float *color, float *alpha, float *depth,
-                  float *normal, float *albedo, int threads, int use_double, uint32_t pixel_begin, uint32_t pixel_end) {
+                  float *normal, float *albedo, uint32_t *nsamp, int threads, int use_double, uint32_t pixel_begin,
+                  uint32_t pixel_end) {
   Scene *s = static_cast<Scene *>(sp);
   Params P{*rp, *cam};
   const uint32_t npix = rp->width * rp->height;
@@ -1701,6 +1722,7 @@ double orc_render(void *sp, const phx_camera *cam, const phx_render_params *rp, 
   for (int64_t i = int64_t(pixel_begin); i < int64_t(pixel_end); ++i) {
     const uint32_t idx = uint32_t(i);
     float c[3], a, d, nn[3], al[3];
+    uint32_t ns;
     if (use_double) {
       PixelOut<double> o;
       render_pixel<double>(*s, P, idx, o, nullptr);
@@ -1711,6 +1733,7 @@ double orc_render(void *sp, const phx_camera *cam, const phx_render_params *rp, 
       d = float(o.depth);
       nn[0] = float(o.normal.x); nn[1] = float(o.normal.y); nn[2] = float(o.normal.z);
       al[0] = float(o.albedo.x); al[1] = float(o.albedo.y); al[2] = float(o.albedo.z);
+      ns = o.ns;
     } else {
       PixelOut<float> o;
       render_pixel<float>(*s, P, idx, o, nullptr);
@@ -1721,12 +1744,14 @@ double orc_render(void *sp, const phx_camera *cam, const phx_render_params *rp, 
       d = o.depth;
       nn[0] = o.normal.x; nn[1] = o.normal.y; nn[2] = o.normal.z;
       al[0] = o.albedo.x; al[1] = o.albedo.y; al[2] = o.albedo.z;
+      ns = o.ns;
     }
     if (color) { color[idx * 3] = c[0]; color[idx * 3 + 1] = c[1]; color[idx * 3 + 2] = c[2]; }
     if (alpha) alpha[idx] = a;
     if (depth) depth[idx] = d;
     if (normal) { normal[idx * 3] = nn[0]; normal[idx * 3 + 1] = nn[1]; normal[idx * 3 + 2] = nn[2]; }
     if (albedo) { albedo[idx * 3] = al[0]; albedo[idx * 3 + 1] = al[1]; albedo[idx * 3 + 2] = al[2]; }
+    if (nsamp) nsamp[idx] = ns;
   }
   return omp_get_wtime() - t0;
 }

@@ -61,6 +61,7 @@ def _sig(lib):
         "phx_autotune": (C.c_int, [P, C.POINTER(abi.View), C.c_uint32, C.c_uint32, C.POINTER(abi.Tuning),
                                    C.POINTER(C.c_double)]),
         "phx_result_device_color": (C.c_int, [P, C.POINTER(P)]),
+        "phx_read_sample_counts": (C.c_int, [P, P]),
         "phx_camera_perspective": (C.c_int, [C.POINTER(abi.Camera), PF, PF, PF, C.c_float, C.c_float, C.c_float,
                                              C.c_float]),
         "phx_camera_orthographic": (C.c_int, [C.POINTER(abi.Camera), PF, PF, PF, C.c_float, C.c_float]),

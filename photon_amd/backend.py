@@ -243,6 +243,14 @@ class PathTracer:
                                              fptr(albedo)))
         return RenderResult(color, alpha, depth, normal, albedo)
 
+    def sample_counts(self):
+        """Samples each pixel took in the last, adaptively sampled frame
+        ((h, w) uint32, rows as rendered)."""
+        h, w = self.params.height, self.params.width
+        counts = np.empty((h, w), np.uint32)
+        self.ctx.check(lib().phx_read_sample_counts(self.ctx.h, C.c_void_p(counts.ctypes.data)))
+        return counts
+
     def device_color(self):
         p = C.c_void_p()
         self.ctx.check(lib().phx_result_device_color(self.ctx.h, C.byref(p)))

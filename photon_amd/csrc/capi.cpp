@@ -715,4 +715,15 @@ int phx_read_result(phx_context *c, float *color, float *alpha, float *depth, fl
   return PHX_OK;
 }
 
+int phx_read_sample_counts(phx_context *c, uint32_t *counts) {
+  if (!c || !counts) return PHX_ERR_INVALID;
+  if (!c->have_result) return fail(c, PHX_ERR_NO_FRAME, "no frame has been rendered");
+  if (!c->f_counted)
+    return fail(c, PHX_ERR_NO_FRAME, "the frame was not adaptively sampled: every pixel took samples_per_pixel");
+  const size_t n = size_t(c->f_w) * c->f_h;
+  HIP_TRY(c, hipMemcpyAsync(counts, c->f_nsamp.p, n * 4, hipMemcpyDefault, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PHX_OK;
+}
+
 }  // extern "C"

@@ -89,6 +89,10 @@ struct phx_context {
 
   // resolved frame (RenderResult) and its accumulator
   DBuf f_accum, f_color, f_alpha, f_depth, f_normal, f_albedo;
+  // adaptive sampling: per pixel, the sum of squared sample values, the
+  // samples taken and whether the pixel takes the next one
+  DBuf f_accum_sq, f_nsamp, f_active;
+  bool f_counted = false;  // the frame was adaptively sampled: f_nsamp holds its counts
   uint32_t f_w = 0, f_h = 0;
   bool have_result = false;
 

@@ -10,6 +10,10 @@
 namespace phx {
 
 constexpr uint32_t kEmissionOnly = 0x80000000u;
+// The queue entry holds no path: its pixel stopped sampling (adaptive
+// sampling), so nothing is traced, shaded or added for it.  Bit 7 of the
+// bounce-kind word, which the 7-bit counters leave free.
+constexpr uint32_t kNoPath = 0x80u;
 
 struct ShadeParams {
   phx_camera cam;

@@ -123,6 +123,9 @@ struct StreamQueues {
   unsigned long long *stats;
   int *spill;        // per-thread traversal stack spill
   int *spill_shadow; // the same for k_shadow, which runs beside k_extend
+  // Adaptive sampling: per pixel of the frame (pix_base + pixel), nonzero
+  // while the pixel takes samples; null when every sample is taken.
+  const uint32_t *pix_active;
   // Per ray of the current queue, from k_volume (runs whose scenes hold
   // volumes): the volume event (scatter distance or -1, albedo) and the RNG
   // state after its draws.
@@ -133,9 +136,10 @@ struct StreamQueues {
 struct FrameBuffers {
   float4 *accum;  // (radiance sum, alpha sum)
   float *color, *alpha, *depth, *normal, *albedo;
+  uint32_t *nsamp = nullptr;  // samples taken per pixel (adaptive sampling); null: the view's spp
   FrameBuffers at(uint32_t pix) const {
     return {accum + pix, color + size_t(pix) * 3, alpha + pix, depth + pix, normal + size_t(pix) * 3,
-            albedo + size_t(pix) * 3};
+            albedo + size_t(pix) * 3, nsamp ? nsamp + pix : nullptr};
   }
 };
 
@@ -181,6 +185,14 @@ hipError_t launch_shadow(const ViewArgs &v, const StreamQueues &q, const LoopSha
 hipError_t launch_nee(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 hipError_t launch_volume(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 // Sums the item's samples of each pixel into accum (first: overwrite).
+// Adaptive sampling (pathtracer.cpp:700-723, 1129-1139): the reduce of one
+// run of samples [s0, s0 + nspp) of a view that also keeps each pixel's sum
+// of squared sample values and sample count, applies the stopping rule
+// before each of those samples and decides whether the pixel takes sample
+// s0 + nspp.  `active` is the mask the run's regeneration used.
+hipError_t launch_adaptive_init(uint32_t npix, uint32_t *active, hipStream_t st);
+hipError_t launch_reduce_adaptive(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum,
+                                  float4 *accum_sq, uint32_t *nsamp, uint32_t *active, uint32_t s0, hipStream_t st);
 hipError_t launch_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum, bool first,
                          hipStream_t st);
 hipError_t launch_aov_init(uint32_t npix, uint32_t *aov_s, hipStream_t st);

@@ -50,8 +50,15 @@ __global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahe
     const RunItem &r = v.item_at(item);
     const uint32_t l = g - r.first;
     PathState p;
-    camera_path(r.sp, l % r.sp.npix, r.sample_base + l / r.sp.npix, p);
+    const uint32_t pixel = l % r.sp.npix;
+    camera_path(r.sp, pixel, r.sample_base + l / r.sp.npix, p);
     p.item = item;
+    if (q.pix_active && q.pix_active[r.pix_base + pixel] == 0u) {
+      // the pixel stopped sampling: an entry that traces and shades nothing
+      p.tmin = 1.f;
+      p.tmax = 0.f;
+      p.kinds = kNoPath;
+    }
     const uint32_t slot = used + i;
     q.o[buf][slot] = f4(p.ro, p.tmin);
     q.d[buf][slot] = f4(p.rd, p.tmax);
@@ -269,6 +276,7 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
   // item wave-uniform, so the view and scene would load through scalar loads,
   // measured slower: the loop raised the spill traffic.)
   auto shade_one = [&](uint32_t item) {
+    if (stv.z & kNoPath) return;  // a sample its pixel did not take
     const RunItem &r = v.item_at(item);
     const DevScene &s = v.scene_of(r);
     const uint32_t l = g - r.first;
@@ -432,6 +440,72 @@ __global__ __launch_bounds__(256) void k_reduce(uint32_t npix, uint32_t nspp, co
   accum[pixel] = acc;
 }
 
+// Adaptive sampling: before sample s >= 16 a pixel with at least 16 samples
+// stops when its largest per-channel variance over its squared luminance is
+// below 0.01 (pathtracer.cpp:676-723); the squared sample values are summed
+// after each sample from the difference of the running sums (:1129-1139).
+constexpr uint32_t kAdaptiveMinSamples = 16;
+constexpr float kAdaptiveThreshold = 0.01f;
+__device__ __forceinline__ bool adaptive_stop(uint32_t s, uint32_t ns, const float4 &acc, const float4 &sq) {
+  if (s < kAdaptiveMinSamples || ns < kAdaptiveMinSamples) return false;
+  const float inv = 1.f / float(ns);
+  const float mx = acc.x * inv, my = acc.y * inv, mz = acc.z * inv;
+  const float vx = sq.x * inv - mx * mx, vy = sq.y * inv - my * my, vz = sq.z * inv - mz * mz;
+  const float max_var = fmaxf(vx, fmaxf(vy, vz));
+  const float lum = fmaxf(0.2126f * mx + 0.7152f * my + 0.0722f * mz, 1e-4f);
+  return max_var / (lum * lum) < kAdaptiveThreshold;
+}
+__global__ void k_adaptive_init(uint32_t npix, uint32_t *active) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < npix) active[p] = 1u;
+}
+// Adds samples s0 .. s0+nspp-1 of the pixels that took them, in order, each
+// after the stopping check for its index; a pixel that stops keeps none of
+// the later ones.  `active` then says whether it takes sample s0+nspp.
+__global__ __launch_bounds__(256) void k_reduce_adaptive(uint32_t npix, uint32_t nspp, const float4 *sample_L,
+                                                         float4 *accum, float4 *accum_sq, uint32_t *nsamp,
+                                                         uint32_t *active, uint32_t s0) {
+  const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pixel >= npix) return;
+  const bool first = s0 == 0;
+  float4 acc = first ? make_float4(0.f, 0.f, 0.f, 0.f) : accum[pixel];
+  float4 sq = first ? make_float4(0.f, 0.f, 0.f, 0.f) : accum_sq[pixel];
+  uint32_t ns = first ? 0u : nsamp[pixel];
+  bool go = active[pixel] != 0u;
+  for (uint32_t s = 0; s < nspp && go; ++s) {
+    if (adaptive_stop(s0 + s, ns, acc, sq)) {
+      go = false;
+      break;
+    }
+    const float4 L = sample_L[size_t(s) * npix + pixel];
+    const float4 before = acc;
+    acc.x += L.x;
+    acc.y += L.y;
+    acc.z += L.z;
+    acc.w += L.w;
+    const float dx = acc.x - before.x, dy = acc.y - before.y, dz = acc.z - before.z;
+    sq.x += dx * dx;
+    sq.y += dy * dy;
+    sq.z += dz * dz;
+    ++ns;
+  }
+  accum[pixel] = acc;
+  accum_sq[pixel] = sq;
+  nsamp[pixel] = ns;
+  if (go && adaptive_stop(s0 + nspp, ns, acc, sq)) go = false;
+  active[pixel] = go ? 1u : 0u;
+}
+hipError_t launch_adaptive_init(uint32_t npix, uint32_t *active, hipStream_t st) {
+  hipLaunchKernelGGL(k_adaptive_init, dim3((npix + 255) / 256), dim3(256), 0, st, npix, active);
+  return hipGetLastError();
+}
+hipError_t launch_reduce_adaptive(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum,
+                                  float4 *accum_sq, uint32_t *nsamp, uint32_t *active, uint32_t s0, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_adaptive, dim3((npix + 255) / 256), dim3(256), 0, st, npix, nspp, sample_L, accum,
+                     accum_sq, nsamp, active, s0);
+  return hipGetLastError();
+}
+
 __global__ void k_aov_init(uint32_t npix, uint32_t *aov_s) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < npix) aov_s[p] = 0xffffffffu;
@@ -454,10 +528,11 @@ __global__ __launch_bounds__(256) void k_resolve(DevScene s, ShadeParams w, Stre
     const uint32_t p = base + lane;
     if (p >= w.npix) continue;
     const float4 a = f.accum[p];
-    f.color[p * 3] = a.x * inv;
-    f.color[p * 3 + 1] = a.y * inv;
-    f.color[p * 3 + 2] = a.z * inv;
-    f.alpha[p] = a.w * inv;
+    const float ip = f.nsamp ? (f.nsamp[p] > 0u ? 1.f / float(f.nsamp[p]) : 0.f) : inv;
+    f.color[p * 3] = a.x * ip;
+    f.color[p * 3 + 1] = a.y * ip;
+    f.color[p * 3 + 2] = a.z * ip;
+    f.alpha[p] = a.w * ip;
     float depth = 1e30f;
     vec3 nrm{0.f, 0.f, 0.f}, alb{0.f, 0.f, 0.f};
     const uint32_t s_first = q.aov_s[p];

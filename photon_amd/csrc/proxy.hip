@@ -326,5 +326,6 @@ extern "C" int phx_render_reference_structure(phx_context *c, const phx_camera *
   c->stats.extend_launches = c->stats.shade_launches = c->stats.shadow_launches = 0;
   c->stats.extend_ms = c->stats.shade_ms = c->stats.shadow_ms = 0.0;
   c->have_result = true;
+  c->f_counted = false;
   return PHX_OK;
 }

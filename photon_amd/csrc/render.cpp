@@ -82,6 +82,7 @@ struct ViewJob {
   ShadeParams sp;
   uint32_t spp;
   uint32_t pix_base;
+  bool adaptive;  // RenderParams::adaptive_sampling
 };
 
 // Item of a run on the host side: the run item plus what the reduce needs.
@@ -96,8 +97,6 @@ struct Run {
 
 int check_params(phx_context *c, const phx_render_params *p, const char *who) {
   if (p->width == 0 || p->height == 0) return fail(c, PHX_ERR_INVALID, std::string(who) + ": empty frame");
-  if (p->adaptive_sampling)
-    return fail(c, PHX_ERR_UNSUPPORTED, std::string(who) + ": adaptive sampling is not offered");
   if (uint64_t(p->width) * p->height > (uint64_t(1) << 26))
     return fail(c, PHX_ERR_INVALID, std::string(who) + ": frame larger than 2^26 pixels");
   const uint32_t max_depth = p->max_depth > 0 ? p->max_depth : 1u;
@@ -114,7 +113,7 @@ ViewJob make_job(const phx_context *scene, uint32_t scene_index, const phx_camer
   const uint32_t max_depth = p.max_depth > 0 ? p.max_depth : 1u;
   const uint32_t max_it = max_depth + 1u + 8u;  // pathtracer.cpp:682
   return ViewJob{scene, scene_index, make_shade_params(cam, p, max_depth, max_it),
-                 std::max<uint32_t>(p.samples_per_pixel, 1), pix_base};
+                 std::max<uint32_t>(p.samples_per_pixel, 1), pix_base, p.adaptive_sampling != 0};
 }
 
 // A view of at least this many samples keeps the GPU's path slots full by
@@ -127,17 +126,74 @@ ViewJob make_job(const phx_context *scene, uint32_t scene_index, const phx_camer
 // plain renders).
 constexpr uint64_t kSoloSamples = uint64_t(1) << 22;
 
-// Splits the views' samples into runs of at most kSampleBudget samples, each
-// a list of (view, first spp, spp count) items in view order; views of
-// kSoloSamples or more get runs of their own, split as phx_render splits them.
-std::vector<Run> plan_runs(const std::vector<ViewJob> &views) {
-  // PHX_SAMPLE_BUDGET=N (diagnostic): a smaller run budget, to exercise the
-  // splitting of large jobs at small sizes.
+// Samples one run may hold: kSampleBudget, or less under PHX_SAMPLE_BUDGET=N
+// (diagnostic), to exercise the splitting of large jobs at small sizes.
+uint64_t run_budget() {
   uint64_t budget = kSampleBudget;
   if (const char *e = std::getenv("PHX_SAMPLE_BUDGET")) {
     const long long b = std::atoll(e);
     if (b > 0) budget = std::min<uint64_t>(budget, uint64_t(b));
   }
+  return budget;
+}
+
+// Splits the views' samples into runs of at most run_budget() samples, each
+// a list of (view, first spp, spp count) items in view order; views of
+// kSoloSamples or more get runs of their own, split as phx_render splits them.
+std::vector<Run> plan_runs(const std::vector<ViewJob> &views);
+
+// Runs of an adaptively sampled view (pathtracer.cpp:700-723).  A pixel's
+// decision to take sample s >= 16 needs only its own samples before s, so the
+// first 16 samples stream as usual and the later ones in groups of
+// kAdaptiveGroup sample indices, whose regeneration skips the pixels that
+// stopped before the group.  The reduce applies the stopping rule sample by
+// sample inside a group and drops the samples after a pixel's stop: the
+// result is the reference's for any group size; a pixel that stops inside a
+// group costs the rest of the group's samples in work only.  (The reference
+// decides for every pixel between two sample indices, one barrier each.)
+// The rule tests the variance of single samples, which does not shrink as
+// samples add up, so a pixel's estimate settles early: pixels stop at 16 or
+// take every sample, and stops in between are rare (4 of 2304 pixels of a
+// 48x48 Cornell view at depth 1).  One group for all later samples is then
+// the fastest: on the 737K-triangle view at 1024x1024x64 spp (3.2 % of the
+// pixels stop, all at 16) groups of 1, 8, 16 and 48 took 155.7, 55.4, 48.4 and
+// 44.7 ms against 42.7 ms for the plain frame.
+constexpr uint32_t kAdaptiveGroup = 0xffffffffu;
+std::vector<Run> plan_adaptive_runs(const ViewJob &j) {
+  ViewJob head = j;
+  head.spp = std::min<uint32_t>(j.spp, 16u);
+  head.adaptive = false;
+  std::vector<Run> runs = plan_runs({head});
+  // PHX_ADAPTIVE_GROUP=N (diagnostic): another group size, to time the
+  // trade between fewer runs and samples past the stops
+  uint64_t group = kAdaptiveGroup;
+  if (const char *e = std::getenv("PHX_ADAPTIVE_GROUP")) {
+    const long long g = std::atoll(e);
+    if (g > 0) group = uint64_t(g);
+  }
+  group = std::max<uint64_t>(1, std::min<uint64_t>(group, run_budget() / j.sp.npix));
+  for (uint32_t s = head.spp; s < j.spp;) {
+    const uint32_t n = uint32_t(std::min<uint64_t>(group, j.spp - s));
+    Run run;
+    HostItem h{};
+    h.r.sp = j.sp;
+    h.r.scene = j.scene_index;
+    h.r.first = 0;
+    h.r.sample_base = j.sp.sample_offset + s;
+    h.r.s0 = s;
+    h.r.pix_base = j.pix_base;
+    h.r.view = 0;
+    h.nspp = n;
+    run.items.push_back(h);
+    run.total = j.sp.npix * n;
+    runs.push_back(run);
+    s += n;
+  }
+  return runs;
+}
+
+std::vector<Run> plan_runs(const std::vector<ViewJob> &views) {
+  const uint64_t budget = run_budget();
   std::vector<Run> runs(1);
   std::vector<uint32_t> order;
   for (uint32_t v = 0; v < views.size(); ++v)
@@ -332,7 +388,9 @@ int ensure_graph(phx_context *c, const ViewArgs &va, const StreamQueues &q, cons
 int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::vector<const phx_context *> &scenes,
                  const FrameBuffers &fb, uint32_t *aov, uint32_t total_pixels, bool use_graph,
                  const AdjointArgs *adj = nullptr) {
-  const std::vector<Run> runs = plan_runs(views);
+  // adaptive sampling: one view (phx_render), the per-pixel state in fb
+  const bool adaptive = views.size() == 1 && views[0].adaptive && !adj && fb.nsamp;
+  const std::vector<Run> runs = adaptive ? plan_adaptive_runs(views[0]) : plan_runs(views);
   uint32_t max_total = 0;
   size_t max_items = 1;
   for (const Run &r : runs) {
@@ -429,6 +487,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   q.spill_shadow = q.spill + trace_threads * (kMaxStack - kShortStack);
   q.vol_ev = any_volumes ? c->vol_ev.as<float4>() : nullptr;
   q.vol_rng = any_volumes ? c->vol_rng.as<uint32_t>() : nullptr;
+  q.pix_active = adaptive ? c->f_active.as<uint32_t>() : nullptr;
 
   hipStream_t st = c->stream;
   // PHX_TRACE_COUNTS=N: read the queue length every N iterations and print it.
@@ -444,6 +503,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
 
   HIP_TRY(c, launch_clear_u32(reinterpret_cast<uint32_t *>(q.stats), kStatWords * 2, st));
   HIP_TRY(c, launch_aov_init(total_pixels, aov, st));
+  if (adaptive) HIP_TRY(c, launch_adaptive_init(total_pixels, c->f_active.as<uint32_t>(), st));
   std::vector<RunItem> items(pad);
   std::vector<uint32_t> first(pad);
   for (const Run &run : runs) {
@@ -513,8 +573,15 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
       if (!replay) HIP_TRY(c, trace_pending_shadow(va, q, ls, tm.on ? &tm : nullptr, st));
     }
     if (adj) continue;
-    for (const HostItem &h : run.items)
-      HIP_TRY(c, launch_reduce(h.r.sp.npix, h.nspp, q.sample_L + h.r.first, fb.accum + h.r.pix_base, h.r.s0 == 0, st));
+    for (const HostItem &h : run.items) {
+      if (adaptive)
+        HIP_TRY(c, launch_reduce_adaptive(h.r.sp.npix, h.nspp, q.sample_L + h.r.first, fb.accum + h.r.pix_base,
+                                          c->f_accum_sq.as<float4>() + h.r.pix_base, fb.nsamp + h.r.pix_base,
+                                          c->f_active.as<uint32_t>() + h.r.pix_base, h.r.s0, st));
+      else
+        HIP_TRY(c, launch_reduce(h.r.sp.npix, h.nspp, q.sample_L + h.r.first, fb.accum + h.r.pix_base, h.r.s0 == 0,
+                                 st));
+    }
   }
   for (const ViewJob &j : views) {
     if (adj) break;
@@ -582,13 +649,21 @@ extern "C" int phx_render(phx_context *c, const phx_camera *cam, const phx_rende
   }
   HIP_TRY(c, c->aov_s.reserve(size_t(npix) * 4));
   c->have_result = false;
+  c->f_counted = false;
   const std::vector<ViewJob> views{make_job(c, 0, *cam, *p, 0)};
-  const FrameBuffers fb{c->f_accum.as<float4>(), c->f_color.as<float>(), c->f_alpha.as<float>(),
-                        c->f_depth.as<float>(), c->f_normal.as<float>(), c->f_albedo.as<float>()};
+  FrameBuffers fb{c->f_accum.as<float4>(), c->f_color.as<float>(), c->f_alpha.as<float>(),
+                  c->f_depth.as<float>(), c->f_normal.as<float>(), c->f_albedo.as<float>()};
+  if (p->adaptive_sampling) {
+    HIP_TRY(c, c->f_accum_sq.reserve(size_t(npix) * 16));
+    HIP_TRY(c, c->f_nsamp.reserve(size_t(npix) * 4));
+    HIP_TRY(c, c->f_active.reserve(size_t(npix) * 4));
+    fb.nsamp = c->f_nsamp.as<uint32_t>();
+  }
   if ((r = stream_views(c, views, {c}, fb, c->aov_s.as<uint32_t>(), npix, false))) return r;
   c->stats.last_render_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->have_result = true;
+  c->f_counted = p->adaptive_sampling != 0;
   return PHX_OK;
 }
 
@@ -607,6 +682,8 @@ extern "C" int phx_render_batch(phx_context *c, const phx_view *views, uint32_t 
     if (!w.scene->have_scene) return fail(c, PHX_ERR_NO_SCENE, "render_batch: a view's scene is not set");
     if (w.scene->device != c->device) return fail(c, PHX_ERR_INVALID, "render_batch: scene on another device");
     if ((r = check_params(c, &w.params, "render_batch"))) return r;
+    if (w.params.adaptive_sampling)
+      return fail(c, PHX_ERR_UNSUPPORTED, "render_batch: adaptive sampling is offered by phx_render only");
     uint32_t si = 0;
     while (si < scenes.size() && scenes[si] != w.scene) ++si;
     if (si == scenes.size()) scenes.push_back(w.scene);
@@ -716,6 +793,8 @@ extern "C" int phx_adjoint(phx_context *c, const phx_camera *cam, const phx_rend
   if (!c->have_scene) return fail(c, PHX_ERR_NO_SCENE, "adjoint before set_scene");
   int r = check_params(c, p, "adjoint");
   if (r) return r;
+  if (p->adaptive_sampling)
+    return fail(c, PHX_ERR_UNSUPPORTED, "adjoint: adaptive sampling is not differentiated (its sample counts are not)");
   HIP_TRY(c, hipSetDevice(c->device));
   const auto t0 = std::chrono::steady_clock::now();
   const uint32_t npix = p->width * p->height;

@@ -360,11 +360,12 @@ def test_errors_are_reported(gpu_ctx):
     pt.camera = abi.Camera()
     with pytest.raises(PhxError, match="before set_scene"):
         pt.render()
-    p = abi.default_render_params(8, 8, 1, 2)
-    p.adaptive_sampling = 1
     pt.set_scene(scenes.cornell_box())
-    pt.params = p
-    with pytest.raises(PhxError, match="adaptive"):
+    pt.params = abi.default_render_params(8, 8, 1, 2)
+    with pytest.raises(PhxError, match="no frame"):
+        pt.sample_counts()
+    pt.params.width = 0
+    with pytest.raises(PhxError, match="empty frame"):
         pt.render()
     ctx.close()
 

@@ -671,3 +671,68 @@ def test_ratio_tracking_draws_replayed():
         assert got[i] == pytest.approx(tr, rel=1e-6), seed
     # the mean is exp(-sigma * 2) (ratio tracking is unbiased)
     assert got.mean() == pytest.approx(math.exp(-1.0), rel=0.03)
+
+
+def _adaptive_cornell(w, h, spp, depth):
+    # the Cornell box seen through a 70 degree field of view: the room fills
+    # the middle of the image, the rest sees the black background
+    s = scenes.cornell_box()
+    c = s.camera
+    cam = oracle.camera_perspective(c["look_from"], c["look_at"], c["vup"], 70.0, w / h)
+    p = abi.default_render_params(w, h, spp, depth)
+    return oracle.OracleScene(s), cam, p
+
+
+def test_adaptive_sampling_stops_where_the_variance_rule_says():
+    # pathtracer.cpp:676-723 and :1129-1139.  Three checks that do not use the
+    # oracle's own loop: (1) a pixel that sees only the constant background has
+    # zero variance, so it stops at exactly the 16 minimum samples; (2) the
+    # stopping sample of every pixel, re-derived in float64 from its sample
+    # values (one-sample renders at sample_offset k, independent of the
+    # adaptive loop), agrees for all but marginal pixels; (3) a pixel that
+    # stopped after n samples equals, bit for bit, a plain render of n samples
+    # (the same sample streams summed in the same order).
+    # (depth 1: direct light only, where a few pixels sit near the threshold)
+    w, h, spp, depth = 48, 48, 48, 1
+    o, cam, p = _adaptive_cornell(w, h, spp, depth)
+    p.adaptive_sampling = 1
+    col, ex, _ = o.render(cam, p, aovs=True, nsamp=True)
+    ns = ex["nsamp"].astype(np.int64)
+    miss = ex["depth"] >= 1e29
+    assert 0.1 < miss.mean() < 0.9
+    assert np.all(ns[miss] == 16)
+    assert ns.min() == 16 and ns.max() <= spp
+    assert (ns == spp).any()  # noisy pixels take every sample
+    assert ((ns > 16) & (ns < spp)).any()  # the check runs before every later sample
+
+    vals = []
+    for k in range(spp):
+        q = abi.default_render_params(w, h, 1, depth)
+        q.sample_offset = k
+        vals.append(o.render(cam, q)[0].astype(np.float64))
+    vals = np.stack(vals)  # (spp, h, w, 3): each sample's radiance
+    want = np.full((h, w), spp, np.int64)
+    stopped = np.zeros((h, w), bool)
+    ratio = np.zeros((spp, h, w))
+    for s in range(16, spp):
+        mean = vals[:s].sum(0) / s
+        sq = (vals[:s] ** 2).sum(0) / s
+        max_var = (sq - mean * mean).max(-1)
+        lum = np.maximum(0.2126 * mean[..., 0] + 0.7152 * mean[..., 1] + 0.0722 * mean[..., 2], 1e-4)
+        ratio[s] = max_var / (lum * lum)
+        stop = ~stopped & (ratio[s] < 0.01)
+        want[stop] = s
+        stopped |= stop
+    assert np.mean(ns == want) > 0.99
+    # a pixel may decide otherwise only where its ratio sits at the threshold
+    # (float32 sums against float64 ones) at the first sample they part at
+    yy, xx = np.nonzero(ns != want)
+    first = np.minimum(ns, want)[yy, xx]
+    assert np.all(np.abs(ratio[first, yy, xx] / 0.01 - 1) < 0.02), ratio[first, yy, xx]
+    assert ((want > 16) & (want < spp)).any()  # some pixels stop in between
+
+    for n in np.unique(ns):
+        q = abi.default_render_params(w, h, int(n), depth)
+        plain = o.render(cam, q)[0]
+        m = ns == n
+        assert np.array_equal(col[m], plain[m]), n
