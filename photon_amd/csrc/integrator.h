@@ -154,6 +154,7 @@ struct LoopShape {
   uint32_t groups;       // material groups of sorted shading (0: queue order)
   uint32_t volumes;      // a scene of the run holds volumes: k_volume runs before k_shade
   uint32_t tier;         // shading tier (kTierLean/Standard/Full) the run's scenes and views need
+  uint32_t tiles;        // camera paths dealt in 8x8 pixel tiles instead of row order
 };
 
 // RenderParams -> the shading constants of one frame (render.cpp).

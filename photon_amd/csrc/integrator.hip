@@ -31,10 +31,39 @@ __global__ void k_run_start(StreamQueues q, uint32_t total, uint32_t capacity) {
     q.counters[i] = i == kTotal ? total : (i == kCapacity ? capacity : 0u);
 }
 
+// The sample that regeneration slot g (of the slots [lo, hi) dealt out in
+// one launch) starts, with samples dealt in 8x8 pixel tiles: the samples of
+// each 8-row band of one sample index of a view that the launch deals out
+// whole go in tile order, so 64 consecutive slots (one traversal wave) hold
+// an 8x8 block of pixels instead of a 64x1 strip; bands dealt in part keep
+// row order.  A bijection on [lo, hi) that stays within each band, so every
+// sample is still started once, as the same (pixel, sample index).
+__device__ __forceinline__ uint32_t tiled_sample(uint32_t g, uint32_t lo, uint32_t hi, const RunItem &r) {
+  const uint32_t w = r.sp.width, npix = r.sp.npix;
+  const uint32_t p = (g - r.first) % npix;
+  const uint32_t band_len = 8u * w;
+  const uint32_t band = p / band_len;
+  if ((band + 1u) * 8u > r.sp.height) return g;
+  const uint32_t base = g - (p - band * band_len);
+  if (base < lo || base + band_len > hi) return g;
+  const uint32_t o = g - base;
+  const uint32_t full = (w >> 3) * 64u;  // positions in whole 8x8 tiles
+  uint32_t x, y;
+  if (o < full) {
+    x = (o >> 6) * 8u + (o & 7u);
+    y = (o >> 3) & 7u;
+  } else {  // the last w % 8 columns of the band, row by row
+    const uint32_t rem = w & 7u, k = o - full;
+    y = k / rem;
+    x = (w & ~7u) + k % rem;
+  }
+  return base + y * w + x;
+}
+
 // Fills the free slots of a queue with new camera paths (pt_init_sample).
 // ahead = 0: the current iteration's queue; 1: the next one.
 template <class P>
-__global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahead, uint32_t zero_L) {
+__global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahead, uint32_t zero_L, uint32_t tiles) {
   const uint32_t it = q.counters[kIter] + ahead;
   const uint32_t used = q.counters[ray_count_word(it)];
   const uint32_t next = q.counters[kNextSample];
@@ -45,7 +74,8 @@ __global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahe
   if (tid == 0) q.counters[kRegenTake + ring(it)] = take;
   const int buf = it & 1;
   for (uint32_t i = tid; i < take; i += gridDim.x * blockDim.x) {
-    const uint32_t g = next + i;
+    uint32_t g = next + i;
+    if (tiles) g = tiled_sample(g, next, next + take, v.item_at(v.find(g)));
     const uint32_t item = v.find(g);
     const RunItem &r = v.item_at(item);
     const uint32_t l = g - r.first;
@@ -622,7 +652,7 @@ hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead
                         hipStream_t st, bool zero_L) {
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
-    hipLaunchKernelGGL(k_regen<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q, ahead, zero_L ? 1u : 0u);
+    hipLaunchKernelGGL(k_regen<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q, ahead, zero_L ? 1u : 0u, ls.tiles);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // the forward pass (zero_L) counts its shading events, the adjoint replay does not

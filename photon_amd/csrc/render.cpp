@@ -435,6 +435,8 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
       if (sc->dev.has_sources || h.r.sp.bg_image) ls.tier = kTierFull;
     }
     if (!c->tuning.refill_lanes) ls.refill = depth >= kDeepTree ? kRefillDeep : kRefillShallow;
+    ls.tiles = 1;
+    if (const char *e = std::getenv("PHX_TILE_ORDER")) ls.tiles = std::atoi(e) != 0;
     return ls;
   };
   bool any_volumes = false;
