@@ -97,3 +97,36 @@ def test_adaptive_is_offered_by_phx_render_only(gpu_ctx):
         ctx.check(lib().phx_adjoint(ctx.h, C.byref(cam), C.byref(p), C.c_void_p(g.data_ptr()),
                                     C.c_void_p(gm.data_ptr()), None))
     ctx.close()
+
+
+def test_adaptive_fog_box_matches_oracle_and_plain_renders(gpu_ctx):
+    # a volume scene: the stopped pixels' empty queue entries also pass
+    # through the volume-event pass (their empty interval skips the tracking)
+    from photon_amd.backend import HipBackend, PathTracer, make_camera
+    w, h, spp, depth = 32, 32, 40, 4
+    s = scenes.fog_box(n=32, macro=8, env_size=(64, 32))
+    c = s.camera
+    cam = make_camera(c["look_from"], c["look_at"], c["vup"], 70.0, w / h)
+    pt = PathTracer(HipBackend(context=gpu_ctx))
+    pt.set_scene(s)
+    pt.camera = cam
+    p = abi.default_render_params(w, h, spp, depth)
+    p.adaptive_sampling = 1
+    pt.params = p
+    print(f"adaptive fog box {w}x{h} {spp} spp depth {depth}", flush=True)
+    res = pt.render()
+    ns = pt.sample_counts().astype(np.int64)
+    ref, ex, _ = oracle.OracleScene(s).render(cam, p, nsamp=True)
+    agree = float(np.mean(ns == ex["nsamp"]))
+    err = rel_mse(res.color, ref)
+    hist = dict(zip(*[a.tolist() for a in np.unique(ns, return_counts=True)]))
+    print(f"counts equal {agree:.4f}, relMSE {err:.3e}, counts {hist}", flush=True)
+    assert ns.min() == 16 and ns.max() <= spp
+    assert (ns == 16).mean() > 0.2 and (ns == spp).mean() > 0.2
+    assert agree > 0.98
+    assert err < 1e-3
+    for n in np.unique(ns):
+        pt.params = abi.default_render_params(w, h, int(n), depth)
+        plain = pt.render()
+        sel = ns == n
+        assert np.array_equal(res.color[sel], plain.color[sel]), n
