@@ -547,6 +547,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
       HIP_TRY(c, launch_run_start(q, run.total, std::min(cap, run.total), st));
       HIP_TRY(c, launch_regen(va, q, 0, ls, st, !replay));
       uint32_t it = 0;
+      LoopShape lsb = ls;  // the launch shape of the next batch of iterations
       while (true) {
         if (replay) {
           HIP_TRY(c, launch_replay_iterations(va, q, *adj, ls, batch, st));
@@ -555,7 +556,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
           HIP_TRY(c, hipGraphLaunch(c->graph, st));
           it += kIterBatch;
         } else {
-          HIP_TRY(c, launch_iterations(va, q, ls, batch, tm.on ? &tm : nullptr, st, fork));
+          HIP_TRY(c, launch_iterations(va, q, lsb, batch, tm.on ? &tm : nullptr, st, fork));
           it += batch;
         }
         // the iteration counter, the samples started, then that iteration's queue
@@ -569,6 +570,12 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
           std::fprintf(stderr, "[phx] %s iteration %u: queue %u, started %u of %u samples\n",
                        replay ? "replay" : "render", it, c->pinned[2], c->pinned[1], run.total);
         if (c->pinned[2] == 0 && c->pinned[1] >= run.total) break;
+        // Once every sample has started, no path is added: the queues of the
+        // iterations to come hold at most the one just read, and the shading
+        // grid (one thread per queue entry) shrinks to it.  A grid over all
+        // path slots costs about 28 us to launch even over an empty queue.
+        if (c->pinned[1] >= run.total)
+          lsb.shade_grid = std::max<uint32_t>(1, std::min(lsb.shade_grid, (c->pinned[2] + 255u) / 256u));
         if (it >= kIterationGuard)
           return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");
       }
