@@ -173,9 +173,11 @@ hipError_t launch_em_records(const DevScene &s, float4 *out, hipStream_t st);
 hipError_t launch_run_start(const StreamQueues &q, uint32_t total, uint32_t capacity, hipStream_t st);
 // ahead = 0: fill the queue of the current iteration (start of a run);
 // ahead = 1: fill the next iteration's queue and advance the iteration.
-// zero_L = false keeps each new sample's stored radiance (the adjoint replay).
+// forward = false (the adjoint replay) leaves the shading events uncounted.
+// A new sample's stored radiance is not touched: its first shading event
+// writes it.
 hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead, const LoopShape &ls,
-                        hipStream_t st, bool zero_L = true);
+                        hipStream_t st, bool forward = true);
 hipError_t launch_extend(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);
 // With ls.groups > 0 the rays of each work-group are shaded in material-group order.
 hipError_t launch_shade(const ViewArgs &v, const StreamQueues &q, const LoopShape &ls, hipStream_t st);

@@ -63,7 +63,7 @@ __device__ __forceinline__ uint32_t tiled_sample(uint32_t g, uint32_t lo, uint32
 // Fills the free slots of a queue with new camera paths (pt_init_sample).
 // ahead = 0: the current iteration's queue; 1: the next one.
 template <class P>
-__global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahead, uint32_t zero_L, uint32_t tiles) {
+__global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahead, uint32_t tiles) {
   const uint32_t it = q.counters[kIter] + ahead;
   const uint32_t used = q.counters[ray_count_word(it)];
   const uint32_t next = q.counters[kNextSample];
@@ -94,7 +94,6 @@ __global__ __launch_bounds__(256) void k_regen(P v, StreamQueues q, uint32_t ahe
     q.d[buf][slot] = f4(p.rd, p.tmax);
     q.tp[buf][slot] = f4(p.tp, p.prev_pdf);
     q.st[buf][slot] = make_uint4(g, pack_flags(p), p.kinds, __float_as_uint(p.path_dist));
-    if (zero_L) q.sample_L[g] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -323,10 +322,15 @@ __device__ __forceinline__ void shade_tile(const P &v, const StreamQueues &q, ui
       vol_ev = q.vol_ev[i];
       vol_rng = q.vol_rng[i];
     }
+    // a path's first event starts its sample's radiance (regeneration leaves
+    // the slot as it was); its shadow rays add to it in later iterations
+    const bool first_event = ps.path_it == 0u;
     NoHooks hooks;
     shade_path<NoHooks, true, TIER>(s, r.sp, ps, htri != 0xffffffffu, htri, hit.x, hit.y, hit.z, rng, out, hooks,
                                     vol_ev, vol_rng);
-    if (out.add.x != 0.f || out.add.y != 0.f || out.add.z != 0.f || out.add_alpha != 0.f) {
+    if (first_event) {
+      q.sample_L[g] = make_float4(out.add.x, out.add.y, out.add.z, out.add_alpha);
+    } else if (out.add.x != 0.f || out.add.y != 0.f || out.add.z != 0.f || out.add_alpha != 0.f) {
       const float4 L = q.sample_L[g];
       q.sample_L[g] = make_float4(L.x + out.add.x, L.y + out.add.y, L.z + out.add.z, L.w + out.add_alpha);
     }
@@ -649,14 +653,14 @@ hipError_t launch_run_start(const StreamQueues &q, uint32_t total, uint32_t capa
   return hipGetLastError();
 }
 hipError_t launch_regen(const ViewArgs &v, const StreamQueues &q, uint32_t ahead, const LoopShape &ls,
-                        hipStream_t st, bool zero_L) {
+                        hipStream_t st, bool forward) {
   return with_view(v, [&](const auto &p) {
     using P = std::decay_t<decltype(p)>;
-    hipLaunchKernelGGL(k_regen<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q, ahead, zero_L ? 1u : 0u, ls.tiles);
+    hipLaunchKernelGGL(k_regen<P>, dim3(ls.regen_grid), dim3(256), 0, st, p, q, ahead, ls.tiles);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // the forward pass (zero_L) counts its shading events, the adjoint replay does not
-    hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, q, ahead, zero_L ? 1u : 0u);
+    // the forward pass counts its shading events, the adjoint replay does not
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(64), 0, st, q, ahead, forward ? 1u : 0u);
     return hipGetLastError();
   });
 }
