@@ -268,7 +268,12 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   c->bvh_depth = bvh.depth;
   int r;
   if ((r = upload(c, c->nodes, bvh.nodes.data(), bvh.nodes.size()))) return r;
+  // One zero record past the last triangle: the traversal fetches a node's
+  // 80 bytes at a triangle record too (its 48 bytes and the 32 after them).
+  bvh.tris.emplace_back();
+  std::memset(&bvh.tris.back(), 0, sizeof(TriRec));
   if ((r = upload(c, c->tris, bvh.tris.data(), bvh.tris.size()))) return r;
+  bvh.tris.pop_back();
   if ((r = upload(c, c->indices, d->indices, size_t(d->triangle_count) * 3))) return r;
   if ((r = upload(c, c->positions, d->positions, size_t(d->vertex_count) * 3))) return r;
   if ((r = upload(c, c->normals, d->normals, d->normals ? size_t(d->vertex_count) * 3 : 0))) return r;

@@ -55,10 +55,8 @@ struct LaneStack {
 };
 
 // Reference Moller-Trumbore (triangle_intersect.h:18-52) on a leaf record.
-PHX_DEV bool tri_test(const TriRec &r, vec3 o, vec3 d, float tmin, float tmax, float &t, float &u, float &v) {
-  const vec3 v0{r.v0.x, r.v0.y, r.v0.z};
-  const vec3 e1{r.e1.x, r.e1.y, r.e1.z};
-  const vec3 e2{r.e2.x, r.e2.y, r.e2.z};
+PHX_DEV bool tri_test(vec3 v0, vec3 e1, vec3 e2, vec3 o, vec3 d, float tmin, float tmax, float &t, float &u,
+                      float &v) {
   const vec3 p = cross(d, e2);
   const float det = dot(e1, p);
   if (fabsf(det) < 1e-30f) return false;
@@ -72,17 +70,29 @@ PHX_DEV bool tri_test(const TriRec &r, vec3 o, vec3 d, float tmin, float tmax, f
   t = dot(e2, q) * inv;
   return !(t < tmin || t > tmax);
 }
-
-PHX_DEV void cswap(float &ka, int &va, float &kb, int &vb) {
-  // selects, not a branch: one compare and four v_cndmask per exchange
-  const bool s = kb < ka;
-  const float k0 = s ? kb : ka, k1 = s ? ka : kb;
-  const int v0 = s ? vb : va, v1 = s ? va : vb;
-  ka = k0;
-  kb = k1;
-  va = v0;
-  vb = v1;
+PHX_DEV bool tri_test(const TriRec &r, vec3 o, vec3 d, float tmin, float tmax, float &t, float &u, float &v) {
+  return tri_test(vec3{r.v0.x, r.v0.y, r.v0.z}, vec3{r.e1.x, r.e1.y, r.e1.z}, vec3{r.e2.x, r.e2.y, r.e2.z}, o, d,
+                  tmin, tmax, t, u, v);
 }
+
+// The header fields of a node that turn a child slot into its stack entry.
+struct NodeView {
+  uint32_t child_base, tri_base, leaf_cnt;
+  int ninner;
+  // stack entry of child slot k: inner slots are nodes child_base + k; leaf
+  // slot k starts after the triangles of the leaf slots before it (the 2-bit
+  // counts of inner and unused slots are zero)
+  PHX_DEV int code(uint32_t k) const {
+    if (int(k) < ninner) return int(child_base + k);
+    const uint32_t below = leaf_cnt & ((1u << (2u * k)) - 1u);
+    const uint32_t first = tri_base + (k - uint32_t(ninner)) + __popc(below & 0x5555u) + 2u * __popc(below & 0xaaaau);
+    return int(~((first << 3) | ((leaf_cnt >> (2u * k)) & 3u)));
+  }
+};
+// Packed child key: the entry distance's bits (clamped at 0, so the order of
+// non-negative floats is that of their bits) with the slot in the low 3 bits;
+// missed children hold kMissKey | slot, above every entered one.
+constexpr uint32_t kMissKey = 0xfffffff8u;
 
 // Slab test of a node's children against their quantised boxes.  A plane at
 // org + q * 2^e is met at t = (org + q * 2^e - o) * inv = q * A + B with
@@ -90,26 +100,19 @@ PHX_DEV void cswap(float &ka, int &va, float &kb, int &vb) {
 // plane; the near and far planes of each axis are picked once per node from
 // the sign of the direction.  Entry distances are clamped to [tmin, tmax]; the
 // exit is widened by a few ulps so rounding never culls a box the exact test
-// would enter.  Returns the number of children entered; codes[0..count) are
-// their stack entries, nearest first.
+// would enter.  `w` holds the node's 80 bytes (five 16-byte words).  Returns
+// the number of children entered and their packed keys in pk, nearest first
+// when SORT, in slot order otherwise.
 template <bool SORT>
-PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float tmin, float tmax,
-                       int codes[kNodeWidth], float key[kNodeWidth]) {
-  // the node as five 16-byte loads, unpacked in registers
-  const uint4 *src = reinterpret_cast<const uint4 *>(nodes + node);
-  uint4 w[5];
-#pragma unroll
-  for (int i = 0; i < 5; ++i) w[i] = src[i];
+PHX_DEV int visit_node(const uint4 w[5], vec3 o, vec3 inv, float tmin, float tmax, NodeView &nv,
+                       uint32_t pk[kNodeWidth]) {
   const vec3 org{__uint_as_float(w[0].x), __uint_as_float(w[0].y), __uint_as_float(w[0].z)};
   const int ex = int(int8_t(w[0].w & 0xffu)), ey = int(int8_t((w[0].w >> 8) & 0xffu)),
             ez = int(int8_t((w[0].w >> 16) & 0xffu));
-  const int ninner = int(w[0].w >> 24);
-  const uint32_t child_base = w[1].x, tri_base = w[1].y;
-  const uint32_t leaf_cnt = w[1].z & 0xffffu, nchild = (w[1].z >> 16) & 0xffu;
+  nv = NodeView{w[1].x, w[1].y, w[1].z & 0xffffu, int(w[0].w >> 24)};
+  const uint32_t nchild = (w[1].z >> 16) & 0xffu;
   const float Ax = ldexpf(inv.x, ex), Ay = ldexpf(inv.y, ey), Az = ldexpf(inv.z, ez);
   const float Bx = (org.x - o.x) * inv.x, By = (org.y - o.y) * inv.y, Bz = (org.z - o.z) * inv.z;
-  // 8 bytes per axis and bound: lo x = w2.xy, lo y = w2.zw, lo z = w3.xy,
-  // hi x = w3.zw, hi y = w4.xy, hi z = w4.zw
   const uint2 lx{w[2].x, w[2].y}, ly{w[2].z, w[2].w}, lz{w[3].x, w[3].y};
   const uint2 hx{w[3].z, w[3].w}, hy{w[4].x, w[4].y}, hz{w[4].z, w[4].w};
   const uint2 nxw = inv.x >= 0.f ? lx : hx, fxw = inv.x >= 0.f ? hx : lx;
@@ -119,7 +122,6 @@ PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float t
     const uint32_t d = k < 4 ? v.x : v.y;
     return float((d >> (8 * (k & 3))) & 0xffu);
   };
-  uint32_t leaf_first = tri_base;
   int cnt = 0;
 #pragma unroll
   for (int k = 0; k < kNodeWidth; ++k) {
@@ -127,79 +129,73 @@ PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float t
                           fmaxf(fmaf(q16(nzw, k), Az, Bz), tmin));
     const float b = fminf(fminf(fmaf(q16(fxw, k), Ax, Bx), fmaf(q16(fyw, k), Ay, By)),
                           fminf(fmaf(q16(fzw, k), Az, Bz), tmax));
-    const bool valid = k < int(nchild);
-    const bool hit = valid && a <= b * 1.0000004f;
-    // both codes and a select: a branch on the lane's ninner would run both
-    // sides under exec masks anyway, with the mask bookkeeping on top
-    const bool inner = k < ninner;
-    const uint32_t c = ((leaf_cnt >> (2 * k)) & 3u) + 1u;
-    const int code = inner ? int(child_base) + k : leaf_code(leaf_first, c);
-    leaf_first += (!inner && valid) ? c : 0u;
-    key[k] = hit ? a : INFINITY;
-    codes[k] = code;
+    const bool hit = k < int(nchild) && a <= b * 1.0000004f;
+    pk[k] = (hit ? (__float_as_uint(fmaxf(a, 0.f)) & ~7u) : kMissKey) | uint32_t(k);
     cnt += hit ? 1 : 0;
   }
-  if (!SORT) return cnt;  // any hit: children stay in slot order (descend_unsorted)
-  // Batcher's odd-even merge network on 8 (key, code) pairs.
-  cswap(key[0], codes[0], key[1], codes[1]);
-  cswap(key[2], codes[2], key[3], codes[3]);
-  cswap(key[4], codes[4], key[5], codes[5]);
-  cswap(key[6], codes[6], key[7], codes[7]);
-  cswap(key[0], codes[0], key[2], codes[2]);
-  cswap(key[1], codes[1], key[3], codes[3]);
-  cswap(key[4], codes[4], key[6], codes[6]);
-  cswap(key[5], codes[5], key[7], codes[7]);
-  cswap(key[1], codes[1], key[2], codes[2]);
-  cswap(key[5], codes[5], key[6], codes[6]);
-  cswap(key[0], codes[0], key[4], codes[4]);
-  cswap(key[1], codes[1], key[5], codes[5]);
-  cswap(key[2], codes[2], key[6], codes[6]);
-  cswap(key[3], codes[3], key[7], codes[7]);
-  cswap(key[2], codes[2], key[4], codes[4]);
-  cswap(key[3], codes[3], key[5], codes[5]);
-  cswap(key[1], codes[1], key[2], codes[2]);
-  cswap(key[3], codes[3], key[4], codes[4]);
-  cswap(key[5], codes[5], key[6], codes[6]);
+  if (SORT) {
+    // Batcher's odd-even merge network on the packed keys: a min and a max
+    // per exchange
+    auto cx = [](uint32_t &x, uint32_t &y) {
+      const uint32_t lo = min(x, y), hi = max(x, y);
+      x = lo;
+      y = hi;
+    };
+    cx(pk[0], pk[1]); cx(pk[2], pk[3]); cx(pk[4], pk[5]); cx(pk[6], pk[7]);
+    cx(pk[0], pk[2]); cx(pk[1], pk[3]); cx(pk[4], pk[6]); cx(pk[5], pk[7]);
+    cx(pk[1], pk[2]); cx(pk[5], pk[6]);
+    cx(pk[0], pk[4]); cx(pk[1], pk[5]); cx(pk[2], pk[6]); cx(pk[3], pk[7]);
+    cx(pk[2], pk[4]); cx(pk[3], pk[5]);
+    cx(pk[1], pk[2]); cx(pk[3], pk[4]); cx(pk[5], pk[6]);
+  }
   return cnt;
 }
+template <bool SORT>
+PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float tmin, float tmax, NodeView &nv,
+                       uint32_t pk[kNodeWidth]) {
+  // the node as five 16-byte loads, unpacked in registers
+  const uint4 *src = reinterpret_cast<const uint4 *>(nodes + node);
+  uint4 w[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) w[i] = src[i];
+  return visit_node<SORT>(w, o, inv, tmin, tmax, nv, pk);
+}
 
-// Pushes codes[1..cnt) farthest first; the nearest becomes the next node.
-PHX_DEV int descend(LaneStack &st, const int codes[kNodeWidth], int cnt) {
+// Pushes the entered children but the nearest, farthest first (pk sorted);
+// the nearest becomes the next node.
+PHX_DEV int descend(LaneStack &st, const NodeView &nv, const uint32_t pk[kNodeWidth], int cnt) {
 #pragma unroll
   for (int k = kNodeWidth - 1; k >= 1; --k)
-    if (k < cnt) st.push(codes[k]);
-  return codes[0];
+    if (k < cnt) st.push(nv.code(pk[k] & 7u));
+  return nv.code(pk[0] & 7u);
 }
-// Children in slot order with key INFINITY for the missed ones: the nearest
-// entered child becomes the next node, the others are pushed in slot order.
-PHX_DEV int descend_unsorted(LaneStack &st, const int codes[kNodeWidth], const float key[kNodeWidth]) {
-  float bk = INFINITY;
-  int bc = 0, bi = -1;
+// Keys in slot order: the nearest entered child becomes the next node, the
+// other entered ones are pushed in slot order.
+PHX_DEV int descend_unsorted(LaneStack &st, const NodeView &nv, const uint32_t pk[kNodeWidth]) {
+  uint32_t best = pk[0];
 #pragma unroll
-  for (int k = 0; k < kNodeWidth; ++k) {
-    const bool better = key[k] < bk;
-    bk = better ? key[k] : bk;
-    bc = better ? codes[k] : bc;
-    bi = better ? k : bi;
-  }
+  for (int k = 1; k < kNodeWidth; ++k) best = min(best, pk[k]);
 #pragma unroll
   for (int k = kNodeWidth - 1; k >= 0; --k)
-    if (key[k] != INFINITY && k != bi) st.push(codes[k]);
-  return bc;
+    if (pk[k] < kMissKey && pk[k] != best) st.push(nv.code(uint32_t(k)));
+  return nv.code(best & 7u);
 }
 template <bool SORT>
-PHX_DEV int descend_any(LaneStack &st, const int codes[kNodeWidth], const float key[kNodeWidth], int cnt) {
-  return SORT ? descend(st, codes, cnt) : descend_unsorted(st, codes, key);
+PHX_DEV int descend_any(LaneStack &st, const NodeView &nv, const uint32_t pk[kNodeWidth], int cnt) {
+  return SORT ? descend(st, nv, pk, cnt) : descend_unsorted(st, nv, pk);
 }
 
 // Triangle `k` of the leaf order against the ray: updates the closest hit h
 // (tie rule: equal t goes to the larger primitive id); returns true when it
 // becomes the hit.
-PHX_DEV bool visit_tri(const TriRec *tris, uint32_t k, vec3 o, vec3 d, float tmin, float &best, TraceHit &h) {
-  const TriRec r = tris[k];
+// `w` holds the record's 48 bytes: v0 with the primitive id, e1, e2.
+PHX_DEV bool visit_tri(const uint4 w[3], uint32_t k, vec3 o, vec3 d, float tmin, float &best, TraceHit &h) {
+  const vec3 v0{__uint_as_float(w[0].x), __uint_as_float(w[0].y), __uint_as_float(w[0].z)};
+  const vec3 e1{__uint_as_float(w[1].x), __uint_as_float(w[1].y), __uint_as_float(w[1].z)};
+  const vec3 e2{__uint_as_float(w[2].x), __uint_as_float(w[2].y), __uint_as_float(w[2].z)};
   float t, u, v;
-  if (!tri_test(r, o, d, tmin, best, t, u, v)) return false;
-  const uint32_t prim = __float_as_uint(r.v0.w);
+  if (!tri_test(v0, e1, e2, o, d, tmin, best, t, u, v)) return false;
+  const uint32_t prim = w[0].w;
   if (t < best || prim > h.prim || h.tri == 0xffffffffu) {
     best = t;
     h.t = t;
@@ -210,6 +206,11 @@ PHX_DEV bool visit_tri(const TriRec *tris, uint32_t k, vec3 o, vec3 d, float tmi
     return true;
   }
   return false;
+}
+PHX_DEV bool visit_tri(const TriRec *tris, uint32_t k, vec3 o, vec3 d, float tmin, float &best, TraceHit &h) {
+  const uint4 *src = reinterpret_cast<const uint4 *>(tris + k);
+  const uint4 w[3] = {src[0], src[1], src[2]};
+  return visit_tri(w, k, o, d, tmin, best, h);
 }
 // The triangles of a leaf entry against the ray; returns true on a hit when
 // ANY (any hit ends the ray).
@@ -236,11 +237,11 @@ PHX_DEV bool traverse(const DevScene &s, vec3 o, vec3 d, float tmin, float tmax,
   st.sp = 0;
   while (true) {
     if (node >= 0) {
-      int codes[kNodeWidth];
-      float key[kNodeWidth];
-      const int cnt = visit_node<!ANY>(s.nodes, node, o, inv, tmin, best, codes, key);
+      NodeView nv;
+      uint32_t pk[kNodeWidth];
+      const int cnt = visit_node<!ANY>(s.nodes, node, o, inv, tmin, best, nv, pk);
       if (cnt > 0) {
-        node = descend_any<!ANY>(st, codes, key, cnt);
+        node = descend_any<!ANY>(st, nv, pk, cnt);
         continue;
       }
     } else if (visit_leaf<ANY>(s.tris, node, o, d, tmin, best, h)) {
@@ -264,12 +265,15 @@ PHX_DEV bool traverse(const DevScene &s, vec3 o, vec3 d, float tmin, float tmax,
 constexpr uint32_t kNoRay = 0xffffffffu;
 
 // load(i, o, d, tmin, tmax, nodes, tris) reads ray i and the acceleration
-// structure it is traced against; done(i, hit, h) consumes its result.  Every
-// thread of the work-group must call it (it synchronises the work-group once,
-// at the start).
-template <bool ANY, class Load, class Done>
+// structure it is traced against; done(i, hit, h) consumes its result.  With
+// ONE_ACCEL every ray is traced against `accel_nodes` / `accel_tris` (the
+// launch's one scene: the pointers stay wave-uniform, in scalar registers)
+// and load's last two outputs are ignored.  Every thread of the work-group
+// must call it (it synchronises the work-group once, at the start).
+template <bool ANY, bool ONE_ACCEL = false, class Load, class Done>
 PHX_DEV void traverse_stream(LaneStack &st, uint32_t *set, uint32_t count, uint32_t chunk, uint32_t refill_min,
-                             Load load, Done done) {
+                             Load load, Done done, const BvhNode *accel_nodes = nullptr,
+                             const TriRec *accel_tris = nullptr) {
   const uint32_t lane = __lane_id();
   uint32_t next = 0, end = 0;  // the wave's current chunk [next, end)
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -283,8 +287,8 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *set, uint32_t count, uint3
   vec3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 1.f}, inv{0.f, 0.f, 0.f};
   float tmin = 0.f, best = 0.f;
   int node = 0;
-  const BvhNode *nodes = nullptr;
-  const TriRec *tris = nullptr;
+  const BvhNode *nodes = accel_nodes;
+  const TriRec *tris = accel_tris;
   TraceHit h;
   h.tri = h.prim = 0xffffffffu;
   h.t = h.u = h.v = 0.f;
@@ -303,7 +307,13 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *set, uint32_t count, uint3
         else if (nb != kNoRay && rank - avail < nbe - nb) i = nb + (rank - avail);
         if (i != kNoRay) {
           float tmax;
-          load(i, o, d, tmin, tmax, nodes, tris);
+          const BvhNode *ray_nodes = nullptr;
+          const TriRec *ray_tris = nullptr;
+          load(i, o, d, tmin, tmax, ray_nodes, ray_tris);
+          if (!ONE_ACCEL) {
+            nodes = ray_nodes;
+            tris = ray_tris;
+          }
           h.tri = h.prim = 0xffffffffu;
           h.t = tmax;
           h.u = h.v = 0.f;
@@ -331,21 +341,34 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *set, uint32_t count, uint3
     if (idx != kNoRay) {
       bool finished = false;
       bool pop = true;
+      // One fetch per iteration for node and triangle lanes alike, issued
+      // before either test: a wave whose lanes hold both waits on memory once
+      // per iteration, not once per kind.  A triangle lane reads its 48-B
+      // record and the 32 B after it (the triangle array is padded past its
+      // last record).
+      const uint32_t code = ~uint32_t(node);
+      const uint32_t first = code >> 3;
+      const uint4 *src = node >= 0 ? reinterpret_cast<const uint4 *>(nodes + node)
+                                   : reinterpret_cast<const uint4 *>(tris + first);
+      uint4 w[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) w[i] = src[i];
+      asm volatile("" ::"v"(w[3].x), "v"(w[3].y), "v"(w[3].z), "v"(w[3].w), "v"(w[4].x), "v"(w[4].y), "v"(w[4].z),
+                   "v"(w[4].w));
       if (node >= 0) {
-        int codes[kNodeWidth];
-        float key[kNodeWidth];
-        const int cnt = visit_node<!ANY>(nodes, node, o, inv, tmin, best, codes, key);
+        NodeView nv;
+        uint32_t pk[kNodeWidth];
+        const int cnt = visit_node<!ANY>(w, o, inv, tmin, best, nv, pk);
         if (cnt > 0) {
-          node = descend_any<!ANY>(st, codes, key, cnt);
+          node = descend_any<!ANY>(st, nv, pk, cnt);
           pop = false;
         }
       } else {
         // one triangle per iteration: the leaf entry shrinks in place until
         // its last triangle, so a wave whose lanes hold leaves of different
         // sizes pays one triangle test per iteration, not the largest leaf
-        const uint32_t code = ~uint32_t(node);
-        const uint32_t first = code >> 3, count = (code & 7u) + 1u;
-        finished = visit_tri(tris, first, o, d, tmin, best, h) && ANY;
+        const uint32_t count = (code & 7u) + 1u;
+        finished = visit_tri(w, first, o, d, tmin, best, h) && ANY;
         if (!finished && count > 1) {
           node = leaf_code(first + 1u, count - 1u);
           pop = false;

@@ -210,6 +210,21 @@ PHX_DEV void ray_accel(const P &v, uint32_t item, const BvhNode *&nodes, const T
   }
 }
 
+// Launches whose rays all trace one scene (one item, or items of one scene)
+// pass its structure to the traversal once, as wave-uniform pointers.
+template <class P>
+constexpr bool kOneAccel = std::is_same<P, OneItem>::value || std::is_same<P, ItemTable<true>>::value;
+template <class P>
+PHX_DEV const BvhNode *one_accel_nodes(const P &v) {
+  if constexpr (kOneAccel<P>) return v.scene.nodes;
+  else return nullptr;
+}
+template <class P>
+PHX_DEV const TriRec *one_accel_tris(const P &v) {
+  if constexpr (kOneAccel<P>) return v.scene.tris;
+  else return nullptr;
+}
+
 // Material id of primitive `prim` in the scene of the ray's item.
 template <class P>
 PHX_DEV uint32_t ray_material(const P &v, uint32_t item, uint32_t prim) {
@@ -240,7 +255,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_e
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], B, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
   uint32_t live = 0;
-  traverse_stream<false>(
+  traverse_stream<false, kOneAccel<P>>(
       st, &q.counters[fetch_set(kFetchExtend, it)], count, chunk, refill,
       [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const BvhNode *&nodes, const TriRec *&tris) {
         const float4 a = q.o[buf][i], b = q.d[buf][i];
@@ -256,7 +271,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_e
         // one group: the shade kernel only parts hits from misses, no tag needed
         if (TAG && hit) w |= (ray_material(v, q.st[buf][i].y >> kItemShift, h.prim) % groups) << 28;
         q.hit[i] = make_float4(h.t, h.u, h.v, __uint_as_float(w));
-      });
+      },
+      one_accel_nodes(v), one_accel_tris(v));
   block_sum<B / 64>(&q.stats[kStatClosest], live);
 }
 
@@ -389,7 +405,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_s
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], B, q.spill_shadow + size_t(gtid) * (kMaxStack - kShortStack), 0};
   uint32_t live = 0;
-  traverse_stream<true>(
+  traverse_stream<true, kOneAccel<P>>(
       st, &q.counters[fetch_set(kFetchShadow, it)], count, chunk, refill,
       [&](uint32_t i, vec3 &o, vec3 &d, float &tmin, float &tmax, const BvhNode *&nodes, const TriRec *&tris) {
         const float4 a = sh_o[i], b = sh_d[i];
@@ -403,7 +419,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(6))) void k_s
       [&](uint32_t i, bool occ, const TraceHit &) {
         // occluded entries lose their sample id; k_nee_accumulate adds the rest
         if (occ) sh_c[i].w = __uint_as_float(kNoRay);
-      });
+      },
+      one_accel_nodes(v), one_accel_tris(v));
   block_sum<B / 64>(&q.stats[kStatShadow], live);
 }
 

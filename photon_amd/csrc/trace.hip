@@ -21,7 +21,7 @@ __global__ __launch_bounds__(256) void k_trace(DevScene s, const float *org, con
   __shared__ int lds[kShortStack * 256];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], 256, spill + size_t(gtid) * spill_stride, 0};
-  traverse_stream<MODE == 2>(
+  traverse_stream<MODE == 2, true>(
       st, fetch, n, kChunk, kRefill,
       [&](uint32_t i, vec3 &o, vec3 &d, float &t0, float &t1, const BvhNode *&nodes, const TriRec *&tris) {
         o = ldv(org, i);
@@ -80,7 +80,8 @@ __global__ __launch_bounds__(256) void k_trace(DevScene s, const float *org, con
           }
           static_cast<phx_hit *>(out)[i] = r;
         }
-      });
+      },
+      s.nodes, s.tris);
 }
 
 __global__ void k_clear_u32(uint32_t *p, uint32_t n) {
