@@ -77,15 +77,18 @@ PHX_DEV bool tri_test(const TriRec &r, vec3 o, vec3 d, float tmin, float tmax, f
 
 // The header fields of a node that turn a child slot into its stack entry.
 struct NodeView {
-  uint32_t child_base, tri_base, leaf_cnt;
-  int ninner;
+  uint32_t child_base;
+  uint32_t leaf_base;  // tri_base - ninner
+  uint32_t leaf_cnt;
+  uint32_t ninner;
   // stack entry of child slot k: inner slots are nodes child_base + k; leaf
   // slot k starts after the triangles of the leaf slots before it (the 2-bit
-  // counts of inner and unused slots are zero)
+  // counts of inner and unused slots are zero, so the sum of the 2-bit fields
+  // below slot k counts the leaf slots' extra triangles)
   PHX_DEV int code(uint32_t k) const {
-    if (int(k) < ninner) return int(child_base + k);
+    if (k < ninner) return int(child_base + k);
     const uint32_t below = leaf_cnt & ((1u << (2u * k)) - 1u);
-    const uint32_t first = tri_base + (k - uint32_t(ninner)) + __popc(below & 0x5555u) + 2u * __popc(below & 0xaaaau);
+    const uint32_t first = leaf_base + k + __popc(below) + __popc(below & 0xaaaau);
     return int(~((first << 3) | ((leaf_cnt >> (2u * k)) & 3u)));
   }
 };
@@ -100,16 +103,16 @@ constexpr uint32_t kMissKey = 0xfffffff8u;
 // plane; the near and far planes of each axis are picked once per node from
 // the sign of the direction.  Entry distances are clamped to [tmin, tmax]; the
 // exit is widened by a few ulps so rounding never culls a box the exact test
-// would enter.  `w` holds the node's 80 bytes (five 16-byte words).  Returns
-// the number of children entered and their packed keys in pk, nearest first
-// when SORT, in slot order otherwise.
+// would enter.  `w` holds the node's 80 bytes (five 16-byte words).  The
+// children's packed keys go to pk, nearest first when SORT, in slot order
+// otherwise.
 template <bool SORT>
-PHX_DEV int visit_node(const uint4 w[5], vec3 o, vec3 inv, float tmin, float tmax, NodeView &nv,
+PHX_DEV void visit_node(const uint4 w[5], vec3 o, vec3 inv, float tmin, float tmax, NodeView &nv,
                        uint32_t pk[kNodeWidth]) {
   const vec3 org{__uint_as_float(w[0].x), __uint_as_float(w[0].y), __uint_as_float(w[0].z)};
   const int ex = int(int8_t(w[0].w & 0xffu)), ey = int(int8_t((w[0].w >> 8) & 0xffu)),
             ez = int(int8_t((w[0].w >> 16) & 0xffu));
-  nv = NodeView{w[1].x, w[1].y, w[1].z & 0xffffu, int(w[0].w >> 24)};
+  nv = NodeView{w[1].x, w[1].y - (w[0].w >> 24), w[1].z & 0xffffu, w[0].w >> 24};
   const uint32_t nchild = (w[1].z >> 16) & 0xffu;
   const float Ax = ldexpf(inv.x, ex), Ay = ldexpf(inv.y, ey), Az = ldexpf(inv.z, ez);
   const float Bx = (org.x - o.x) * inv.x, By = (org.y - o.y) * inv.y, Bz = (org.z - o.z) * inv.z;
@@ -122,7 +125,6 @@ PHX_DEV int visit_node(const uint4 w[5], vec3 o, vec3 inv, float tmin, float tma
     const uint32_t d = k < 4 ? v.x : v.y;
     return float((d >> (8 * (k & 3))) & 0xffu);
   };
-  int cnt = 0;
 #pragma unroll
   for (int k = 0; k < kNodeWidth; ++k) {
     const float a = fmaxf(fmaxf(fmaf(q16(nxw, k), Ax, Bx), fmaf(q16(nyw, k), Ay, By)),
@@ -130,8 +132,8 @@ PHX_DEV int visit_node(const uint4 w[5], vec3 o, vec3 inv, float tmin, float tma
     const float b = fminf(fminf(fmaf(q16(fxw, k), Ax, Bx), fmaf(q16(fyw, k), Ay, By)),
                           fminf(fmaf(q16(fzw, k), Az, Bz), tmax));
     const bool hit = k < int(nchild) && a <= b * 1.0000004f;
-    pk[k] = (hit ? (__float_as_uint(fmaxf(a, 0.f)) & ~7u) : kMissKey) | uint32_t(k);
-    cnt += hit ? 1 : 0;
+    // a negative entry distance (tmin < 0) is a negative int: clamped to 0
+    pk[k] = (hit ? (uint32_t(max(int(__float_as_uint(a)), 0)) & ~7u) : kMissKey) | uint32_t(k);
   }
   if (SORT) {
     // Batcher's odd-even merge network on the packed keys: a min and a max
@@ -148,41 +150,46 @@ PHX_DEV int visit_node(const uint4 w[5], vec3 o, vec3 inv, float tmin, float tma
     cx(pk[2], pk[4]); cx(pk[3], pk[5]);
     cx(pk[1], pk[2]); cx(pk[3], pk[4]); cx(pk[5], pk[6]);
   }
-  return cnt;
 }
 template <bool SORT>
-PHX_DEV int visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float tmin, float tmax, NodeView &nv,
+PHX_DEV void visit_node(const BvhNode *nodes, int node, vec3 o, vec3 inv, float tmin, float tmax, NodeView &nv,
                        uint32_t pk[kNodeWidth]) {
   // the node as five 16-byte loads, unpacked in registers
   const uint4 *src = reinterpret_cast<const uint4 *>(nodes + node);
   uint4 w[5];
 #pragma unroll
   for (int i = 0; i < 5; ++i) w[i] = src[i];
-  return visit_node<SORT>(w, o, inv, tmin, tmax, nv, pk);
+  visit_node<SORT>(w, o, inv, tmin, tmax, nv, pk);
 }
 
 // Pushes the entered children but the nearest, farthest first (pk sorted);
 // the nearest becomes the next node.
-PHX_DEV int descend(LaneStack &st, const NodeView &nv, const uint32_t pk[kNodeWidth], int cnt) {
+PHX_DEV int descend(LaneStack &st, const NodeView &nv, const uint32_t pk[kNodeWidth]) {
 #pragma unroll
   for (int k = kNodeWidth - 1; k >= 1; --k)
-    if (k < cnt) st.push(nv.code(pk[k] & 7u));
+    if (pk[k] < kMissKey) st.push(nv.code(pk[k] & 7u));
   return nv.code(pk[0] & 7u);
 }
 // Keys in slot order: the nearest entered child becomes the next node, the
 // other entered ones are pushed in slot order.
-PHX_DEV int descend_unsorted(LaneStack &st, const NodeView &nv, const uint32_t pk[kNodeWidth]) {
-  uint32_t best = pk[0];
-#pragma unroll
-  for (int k = 1; k < kNodeWidth; ++k) best = min(best, pk[k]);
+PHX_DEV int descend_unsorted(LaneStack &st, const NodeView &nv, const uint32_t pk[kNodeWidth], uint32_t best) {
 #pragma unroll
   for (int k = kNodeWidth - 1; k >= 0; --k)
     if (pk[k] < kMissKey && pk[k] != best) st.push(nv.code(uint32_t(k)));
   return nv.code(best & 7u);
 }
+// The nearest entered child's key: kMissKey or above when none was entered.
 template <bool SORT>
-PHX_DEV int descend_any(LaneStack &st, const NodeView &nv, const uint32_t pk[kNodeWidth], int cnt) {
-  return SORT ? descend(st, nv, pk, cnt) : descend_unsorted(st, nv, pk);
+PHX_DEV uint32_t nearest_key(const uint32_t pk[kNodeWidth]) {
+  if (SORT) return pk[0];
+  uint32_t best = pk[0];
+#pragma unroll
+  for (int k = 1; k < kNodeWidth; ++k) best = min(best, pk[k]);
+  return best;
+}
+template <bool SORT>
+PHX_DEV int descend_any(LaneStack &st, const NodeView &nv, const uint32_t pk[kNodeWidth], uint32_t best) {
+  return SORT ? descend(st, nv, pk) : descend_unsorted(st, nv, pk, best);
 }
 
 // Triangle `k` of the leaf order against the ray: updates the closest hit h
@@ -239,9 +246,10 @@ PHX_DEV bool traverse(const DevScene &s, vec3 o, vec3 d, float tmin, float tmax,
     if (node >= 0) {
       NodeView nv;
       uint32_t pk[kNodeWidth];
-      const int cnt = visit_node<!ANY>(s.nodes, node, o, inv, tmin, best, nv, pk);
-      if (cnt > 0) {
-        node = descend_any<!ANY>(st, nv, pk, cnt);
+      visit_node<!ANY>(s.nodes, node, o, inv, tmin, best, nv, pk);
+      const uint32_t near = nearest_key<!ANY>(pk);
+      if (near < kMissKey) {
+        node = descend_any<!ANY>(st, nv, pk, near);
         continue;
       }
     } else if (visit_leaf<ANY>(s.tris, node, o, d, tmin, best, h)) {
@@ -358,9 +366,10 @@ PHX_DEV void traverse_stream(LaneStack &st, uint32_t *set, uint32_t count, uint3
       if (node >= 0) {
         NodeView nv;
         uint32_t pk[kNodeWidth];
-        const int cnt = visit_node<!ANY>(w, o, inv, tmin, best, nv, pk);
-        if (cnt > 0) {
-          node = descend_any<!ANY>(st, nv, pk, cnt);
+        visit_node<!ANY>(w, o, inv, tmin, best, nv, pk);
+        const uint32_t near = nearest_key<!ANY>(pk);
+        if (near < kMissKey) {
+          node = descend_any<!ANY>(st, nv, pk, near);
           pop = false;
         }
       } else {
