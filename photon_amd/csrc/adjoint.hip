@@ -105,9 +105,12 @@ PHX_DEV void grad_route(const float g[kGradParams], const AdjHooks &h, const Dev
 // inline NEE visibility, gradient accumulation (per work-group in LDS for the
 // first lds_mats materials, flushed once), continuing paths to the next queue.
 // The work-group steps through tiles of B rays together: one striped fetch
-// and one slot claim per tile, not one per wave.
+// and one slot claim per tile, not one per wave.  Two waves per SIMD (256
+// registers): uncapped, the standard tier took 256 registers and 3 AGPRs,
+// one wave per SIMD, and its pass was 26 % slower than with a 12-byte spill.
 template <class P, int B, int TIER>
-__global__ __launch_bounds__(B) void k_shade_adjoint(P v, StreamQueues q, AdjointArgs a) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(2))) void k_shade_adjoint(P v, StreamQueues q,
+                                                                                             AdjointArgs a) {
   const uint32_t it = q.counters[kIter];
   const uint32_t count = q.counters[ray_count_word(it)];
   const uint32_t units = fetch_units(count, B);
