@@ -547,7 +547,8 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
       HIP_TRY(c, launch_run_start(q, run.total, std::min(cap, run.total), st));
       HIP_TRY(c, launch_regen(va, q, 0, ls, st, !replay));
       uint32_t it = 0;
-      LoopShape lsb = ls;  // the launch shape of the next batch of iterations
+      LoopShape lsb = ls;   // the launch shape of the next batch of iterations
+      uint32_t nb = batch;  // iterations of the next batch (launched one by one)
       while (true) {
         if (replay) {
           HIP_TRY(c, launch_replay_iterations(va, q, *adj, ls, batch, st));
@@ -556,8 +557,8 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
           HIP_TRY(c, hipGraphLaunch(c->graph, st));
           it += kIterBatch;
         } else {
-          HIP_TRY(c, launch_iterations(va, q, lsb, batch, tm.on ? &tm : nullptr, st, fork));
-          it += batch;
+          HIP_TRY(c, launch_iterations(va, q, lsb, nb, tm.on ? &tm : nullptr, st, fork));
+          it += nb;
         }
         // the iteration counter, the samples started, then that iteration's queue
         HIP_TRY(c, hipMemcpyAsync(&c->pinned[0], q.counters + kIter, 4, hipMemcpyDeviceToHost, st));
@@ -574,8 +575,13 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
         // iterations to come hold at most the one just read, and the shading
         // grid (one thread per queue entry) shrinks to it.  A grid over all
         // path slots costs about 28 us to launch even over an empty queue.
-        if (c->pinned[1] >= run.total)
+        // The drain that follows takes a few iterations, each bound by its
+        // slowest rays; batches of half the size launch fewer iterations over
+        // an empty queue at its end (each costs ~60 us) for one more read.
+        if (c->pinned[1] >= run.total) {
           lsb.shade_grid = std::max<uint32_t>(1, std::min(lsb.shade_grid, (c->pinned[2] + 255u) / 256u));
+          nb = std::max<uint32_t>(1, batch / 2);
+        }
         if (it >= kIterationGuard)
           return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");
       }
