@@ -199,8 +199,11 @@ hipError_t launch_reduce_adaptive(uint32_t npix, uint32_t nspp, const float4 *sa
 hipError_t launch_reduce(uint32_t npix, uint32_t nspp, const float4 *sample_L, float4 *accum, bool first,
                          hipStream_t st);
 hipError_t launch_aov_init(uint32_t npix, uint32_t *aov_s, hipStream_t st);
+// `hits`: room for one hit record per pixel of the view (scenes without
+// volumes): the AOV rays are then traced ahead by the streaming traversal;
+// null: the resolve traces them itself.  `fetch`: kFetchSetWords words.
 hipError_t launch_resolve(const DevScene &s, const ShadeParams &sp, const StreamQueues &q, const FrameBuffers &f,
-                          uint32_t ns, uint32_t *fetch, uint32_t grid, hipStream_t st);
+                          uint32_t ns, uint32_t *fetch, uint32_t grid, hipStream_t st, float4 *hits = nullptr);
 
 // Path-replay adjoint (adjoint.hip).  Gradients are accumulated in double:
 // grad_mat[material * 5 + k] for k = base r, g, b, roughness, metallic, and

@@ -562,11 +562,47 @@ __global__ void k_aov_init(uint32_t npix, uint32_t *aov_s) {
   if (p < npix) aov_s[p] = 0xffffffffu;
 }
 
+// The AOV samples' camera rays traced ahead of the resolve (scenes without
+// volumes): pixel p's first sample with a camera-ray event (aov_s) is traced
+// again by the persistent closest-hit traversal, its hit record stored in
+// hits[p] (t, b1, b2, leaf triangle; 0xffffffff: a miss or no such sample).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_aov_trace(DevScene s, ShadeParams w,
+                                                                                          StreamQueues q,
+                                                                                          uint32_t *fetch,
+                                                                                          float4 *hits) {
+  __shared__ int lds[kShortStack * 256];
+  const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+  LaneStack st{&lds[threadIdx.x], 256, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
+  traverse_stream<false, true>(
+      st, fetch, w.npix, kChunk, 16u,
+      [&](uint32_t p, vec3 &o, vec3 &d, float &tmin, float &tmax, const BvhNode *&, const TriRec *&) {
+        const uint32_t s_first = q.aov_s[p];
+        if (s_first == 0xffffffffu) {
+          o = vec3{0.f, 0.f, 0.f};
+          d = vec3{0.f, 0.f, 1.f};
+          tmin = 1.f;
+          tmax = 0.f;
+          return;
+        }
+        PathState ps;
+        camera_path(w, p, w.sample_offset + s_first, ps);
+        o = ps.ro;
+        d = ps.rd;
+        tmin = ps.tmin;
+        tmax = ps.tmax;
+      },
+      [&](uint32_t p, bool hit, const TraceHit &h) {
+        hits[p] = make_float4(h.t, h.u, h.v, __uint_as_float(hit ? h.tri : 0xffffffffu));
+      },
+      s.nodes, s.tris);
+}
+
 // pt_resolve plus the AOVs: the first sample whose camera ray ended at a
 // surface or a volume event is traced again up to that event
-// (pathtracer.cpp:800-809, 974-983) to read depth, normal and albedo.
+// (pathtracer.cpp:800-809, 974-983) to read depth, normal and albedo.  With
+// `hits` (k_aov_trace ran) its first trace is read from there.
 __global__ __launch_bounds__(256) void k_resolve(DevScene s, ShadeParams w, StreamQueues q, FrameBuffers f, uint32_t ns,
-                                                 uint32_t *fetch) {
+                                                 uint32_t *fetch, const float4 *hits) {
   __shared__ int lds[kShortStack * 256];
   const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
   LaneStack st{&lds[threadIdx.x], 256, q.spill + size_t(gtid) * (kMaxStack - kShortStack), 0};
@@ -594,7 +630,17 @@ __global__ __launch_bounds__(256) void k_resolve(DevScene s, ShadeParams w, Stre
       float path_dist = 0.f;
       for (uint32_t it = 0; it < w.max_it; ++it) {
         TraceHit h;
-        const bool hit = traverse<false>(s, ps.ro, ps.rd, ps.tmin, ps.tmax, st, h);
+        bool hit;
+        if (it == 0 && hits) {
+          const float4 hv = hits[p];
+          h.t = hv.x;
+          h.u = hv.y;
+          h.v = hv.z;
+          h.tri = __float_as_uint(hv.w);
+          hit = h.tri != 0xffffffffu;
+        } else {
+          hit = traverse<false>(s, ps.ro, ps.rd, ps.tmin, ps.tmax, st, h);
+        }
         Rng rng = make_rng(p, sample, 1u + it);
         if (s.vol_count > 0) {
           vec3 va{0.f, 0.f, 0.f};
@@ -758,8 +804,17 @@ hipError_t launch_aov_init(uint32_t npix, uint32_t *aov_s, hipStream_t st) {
   return hipGetLastError();
 }
 hipError_t launch_resolve(const DevScene &s, const ShadeParams &sp, const StreamQueues &q, const FrameBuffers &f,
-                          uint32_t ns, uint32_t *fetch, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, st, s, sp, q, f, ns, fetch);
+                          uint32_t ns, uint32_t *fetch, uint32_t grid, hipStream_t st, float4 *hits) {
+  if (hits) {
+    // the AOV rays first, through the streaming traversal (fetch: its stripe
+    // counters, then the resolve's one counter)
+    hipError_t e = launch_clear_u32(fetch, kFetchSetWords, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_aov_trace, dim3(grid), dim3(256), 0, st, s, sp, q, fetch, hits);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_clear_u32(fetch, 1, st)) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, st, s, sp, q, f, ns, fetch, static_cast<const float4 *>(hits));
   return hipGetLastError();
 }
 

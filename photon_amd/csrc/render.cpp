@@ -603,8 +603,10 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
     StreamQueues qv = q;
     qv.aov_s = aov + j.pix_base;
     HIP_TRY(c, launch_clear_u32(c->fetch.as<uint32_t>(), 1, st));
+    // the hit records of the path slots are free now: one per pixel, if they fit
+    float4 *hits = j.scene->dev.vol_count == 0 && j.sp.npix <= cap ? q.hit : nullptr;
     HIP_TRY(c, launch_resolve(j.scene->dev, j.sp, qv, fb.at(j.pix_base), j.spp, c->fetch.as<uint32_t>(),
-                              c->trace_grid, st));
+                              c->trace_grid, st, hits));
   }
   unsigned long long hs[kStatWords];
   HIP_TRY(c, hipMemcpyAsync(hs, q.stats, sizeof hs, hipMemcpyDeviceToHost, st));
