@@ -274,6 +274,9 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   std::memset(&bvh.tris.back(), 0, sizeof(TriRec));
   if ((r = upload(c, c->tris, bvh.tris.data(), bvh.tris.size()))) return r;
   bvh.tris.pop_back();
+  HIP_TRY(c, c->tri_ng.reserve(bvh.tris.size() * sizeof(float4)));
+  HIP_TRY(c, launch_tri_normals(c->tris.as<TriRec>(), c->tri_ng.as<float4>(), uint32_t(bvh.tris.size()), c->stream));
+  c->scene_bytes += bvh.tris.size() * sizeof(float4);
   if ((r = upload(c, c->indices, d->indices, size_t(d->triangle_count) * 3))) return r;
   if ((r = upload(c, c->positions, d->positions, size_t(d->vertex_count) * 3))) return r;
   if ((r = upload(c, c->normals, d->normals, d->normals ? size_t(d->vertex_count) * 3 : 0))) return r;
@@ -321,6 +324,7 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   s.nodes = c->nodes.as<BvhNode>();
   s.tris = c->tris.as<TriRec>();
   s.tri_attr = c->tri_attr.as<uint4>();
+  s.tri_ng = c->tri_ng.as<float4>();
   s.indices = c->indices.as<uint32_t>();
   s.positions = c->positions.as<float>();
   s.normals = c->normals.as<float>();
@@ -329,7 +333,7 @@ int phx_build_accel(phx_context *c, const phx_scene_desc *d) {
   s.material_ids = c->material_ids.as<uint32_t>();
   s.triangle_count = d->triangle_count;
   c->stats.bvh_nodes = bvh.nodes.size();
-  c->stats.bvh_bytes = bvh.nodes.size() * sizeof(BvhNode) + bvh.tris.size() * (sizeof(TriRec) + sizeof(uint4));
+  c->stats.bvh_bytes = bvh.nodes.size() * sizeof(BvhNode) + bvh.tris.size() * (sizeof(TriRec) + 2 * sizeof(uint4));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->have_accel = true;
   return PHX_OK;

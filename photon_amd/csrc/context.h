@@ -61,7 +61,7 @@ struct phx_context {
 
   // geometry (seam A)
   bool have_accel = false;
-  DBuf nodes, tris, tri_attr, indices, positions, normals, texcoords, colors, material_ids;
+  DBuf nodes, tris, tri_attr, tri_ng, indices, positions, normals, texcoords, colors, material_ids;
   uint32_t triangle_count = 0, vertex_count = 0;
   float bounds_lo[3]{}, bounds_hi[3]{};
   uint32_t bvh_depth = 0;

@@ -61,6 +61,10 @@ struct alignas(16) TriRec {
   float4 e1;
   float4 e2;
 };
+// The unflipped geometric normal of a triangle record (mesh_intersector.h:139).
+PHX_DEV vec3 tri_normal(const TriRec &r) {
+  return normalize(cross(vec3{r.e1.x, r.e1.y, r.e1.z}, vec3{r.e2.x, r.e2.y, r.e2.z}));
+}
 
 struct DevScene {
   const BvhNode *nodes;
@@ -68,6 +72,10 @@ struct DevScene {
   // In leaf order beside tris: the triangle's vertex indices and material id,
   // so a hit's attributes load one dependent step after the hit, not two.
   const uint4 *tri_attr;
+  // In leaf order beside tris: the geometric normal normalize(cross(e1, e2))
+  // and the primitive id bits, so shading reads 16 bytes instead of the 48-byte
+  // record and skips the normalisation (null: computed from tris).
+  const float4 *tri_ng;
   const uint32_t *indices;
   const float *positions;
   const float *normals;

@@ -75,11 +75,17 @@ PHX_DEV vec3 clamp_contribution(vec3 c, int light_bounce, float ds, float is) {
 // Surface attributes at a hit (mesh_intersector.h:126-178).
 PHX_DEV SurfaceHit surface_at(const DevScene &s, uint32_t tri, float t, float b1, float b2, vec3 ro, vec3 rd) {
   SurfaceHit h;
-  const TriRec tr = s.tris[tri];
   const uint4 ta = s.tri_attr[tri];
-  h.prim = __float_as_uint(tr.v0.w);
+  if (s.tri_ng) {
+    const float4 nr = s.tri_ng[tri];
+    h.prim = __float_as_uint(nr.w);
+    h.ng = vec3{nr.x, nr.y, nr.z};
+  } else {
+    const TriRec tr = s.tris[tri];
+    h.prim = __float_as_uint(tr.v0.w);
+    h.ng = tri_normal(tr);
+  }
   h.p = ro + rd * t;
-  h.ng = normalize(cross(vec3{tr.e1.x, tr.e1.y, tr.e1.z}, vec3{tr.e2.x, tr.e2.y, tr.e2.z}));
   const uint32_t i0 = ta.x, i1 = ta.y, i2 = ta.z;
   const float w0 = 1.f - b1 - b2;
   h.sn = h.ng;

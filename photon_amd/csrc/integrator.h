@@ -236,5 +236,7 @@ hipError_t launch_trace_occluded(const DevScene &s, const float *org, const floa
                                  const float *tmax, uint32_t n, uint32_t *out, uint32_t *fetch, int *spill,
                                  uint32_t grid, hipStream_t st, uint32_t spill_stride = kFullSpill);
 hipError_t launch_clear_u32(uint32_t *p, uint32_t n, hipStream_t st);
+// out[i] = (normalize(cross(e1, e2)), primitive id bits) of tris[i], i < n.
+hipError_t launch_tri_normals(const TriRec *tris, float4 *out, uint32_t n, hipStream_t st);
 
 }  // namespace phx

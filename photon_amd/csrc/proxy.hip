@@ -244,6 +244,7 @@ extern "C" int phx_render_reference_structure(phx_context *c, const phx_camera *
   DevScene ds = c->dev;
   ds.tris = c->px_tris.as<TriRec>();
   ds.tri_attr = c->px_tri_attr.as<uint4>();
+  ds.tri_ng = nullptr;  // the normals come from the primitive-order records
 
   // per-slot state (pathtracer.cpp:505-562)
   const size_t n = npix;

@@ -84,11 +84,25 @@ __global__ __launch_bounds__(256) void k_trace(DevScene s, const float *org, con
       s.nodes, s.tris);
 }
 
+// The scene's per-triangle normal records (DevScene::tri_ng), in leaf order.
+__global__ void k_tri_normals(const TriRec *tris, float4 *out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const TriRec r = tris[i];
+  const vec3 ng = tri_normal(r);
+  out[i] = make_float4(ng.x, ng.y, ng.z, r.v0.w);
+}
+
 __global__ void k_clear_u32(uint32_t *p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
 }
 
+hipError_t launch_tri_normals(const TriRec *tris, float4 *out, uint32_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tri_normals, dim3((n + 255u) / 256u), dim3(256), 0, st, tris, out, n);
+  return hipGetLastError();
+}
 hipError_t launch_clear_u32(uint32_t *p, uint32_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_clear_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
   return hipGetLastError();
