@@ -191,6 +191,10 @@ int phx_create(phx_context **out, int device) {
     delete c;
     return PHX_ERR_HIP;
   }
+  c->lanes[0].st = c->stream;
+  c->lanes[0].st2 = c->stream2;
+  c->lanes[0].ev_fork = c->ev_fork;
+  c->lanes[0].ev_join = c->ev_join;
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     c->cu_count = cus;
@@ -210,11 +214,29 @@ void phx_destroy(phx_context *c) {
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   (void)hipEventDestroy(c->ev_fork);
   (void)hipEventDestroy(c->ev_join);
-  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  std::vector<hipStream_t> lane_streams;
+  for (int l = 0; l < 2; ++l) {
+    phx_context::Lane &ln = c->lanes[l];
+    if (l > 0) {  // lane 0's streams and fork events are the context's own
+      for (hipStream_t x : {ln.st, ln.st2})
+        if (x) {
+          (void)hipStreamSynchronize(x);
+          lane_streams.push_back(x);
+        }
+      for (hipEvent_t e : {ln.ev_fork, ln.ev_join})
+        if (e) (void)hipEventDestroy(e);
+    }
+    for (hipEvent_t e : {ln.ev_read, ln.ev_reduced})
+      if (e) (void)hipEventDestroy(e);
+    if (ln.graph) (void)hipGraphExecDestroy(ln.graph);
+    if (ln.h_items) (void)hipHostFree(ln.h_items);
+    if (ln.h_first) (void)hipHostFree(ln.h_first);
+  }
   if (c->pinned) (void)hipHostFree(c->pinned);
   delete c;
   if (s) (void)hipStreamDestroy(s);
   if (s2) (void)hipStreamDestroy(s2);
+  for (hipStream_t x : lane_streams) (void)hipStreamDestroy(x);
 }
 
 const char *phx_last_error(const phx_context *c) { return c ? c->error.c_str() : "no context"; }

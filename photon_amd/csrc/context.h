@@ -78,10 +78,27 @@ struct phx_context {
   // adjoint gradient accumulators (double)
   DBuf adj_gm, adj_gt;
 
-  // wavefront storage
-  DBuf q_o[2], q_d[2], q_tp[2], q_st[2], q_hit, sh_o[2], sh_d[2], sh_c[2], sample_L, aov_s, counters, stats_buf, spill, fetch;
-  DBuf vol_ev, vol_rng;  // k_volume's results, sized for the queue when a run's scenes hold volumes
-  uint32_t q_capacity = 0;
+  // Wavefront storage of one run of the bounce loop: its path and shadow
+  // queues, per-sample radiance, counters, item tables and captured loop.
+  // Lane 0 streams on `stream`/`stream2`; lane 1 (created on first use) lets
+  // a batch start its next run while the previous one drains.
+  struct Lane {
+    DBuf q_o[2], q_d[2], q_tp[2], q_st[2], q_hit, sh_o[2], sh_d[2], sh_c[2], sample_L, counters, stats_buf;
+    DBuf vol_ev, vol_rng;  // k_volume's results, sized for the queue when a run's scenes hold volumes
+    DBuf bt_items, bt_first;
+    uint32_t q_capacity = 0;
+    hipStream_t st = nullptr, st2 = nullptr;  // lane 0: aliases of `stream` and `stream2`
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_read = nullptr, ev_reduced = nullptr;  // counter readback landed; the run's reduces done
+    phx::RunItem *h_items = nullptr;  // pinned staging of the item tables (hipHostMalloc)
+    uint32_t *h_first = nullptr;
+    size_t h_count = 0;
+    // captured bounce loop and the launch arguments it was captured with
+    hipGraphExec_t graph = nullptr;
+    std::vector<unsigned char> graph_key;
+  };
+  Lane lanes[2];
+  DBuf aov_s, spill, fetch;
   uint32_t trace_grid = 0;  // persistent grid of the seam-A trace kernels and the resolve pass
   size_t spill_threads = 0;
   std::vector<hipEvent_t> events;  // kernel timing (phx_tuning.time_kernels)
@@ -97,15 +114,12 @@ struct phx_context {
   bool have_result = false;
 
   // batched views (phx_render_batch): run tables, concatenated frame buffers
-  DBuf bt_items, bt_first, bt_scenes, bt_accum, bt_color, bt_alpha, bt_depth, bt_normal, bt_albedo, bt_aov;
+  DBuf bt_scenes, bt_accum, bt_color, bt_alpha, bt_depth, bt_normal, bt_albedo, bt_aov;
   struct BatchView {
     uint32_t width, height, pix_base;
   };
   std::vector<BatchView> bt_views;
   bool bt_have_result = false;
-  // captured bounce loop of the batch and the launch arguments it was captured with
-  hipGraphExec_t graph = nullptr;
-  std::vector<unsigned char> graph_key;
 
   // render bridge (seam B)
   DBuf b_color, b_alpha, b_depth, b_normal, b_albedo, b_bg;

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <utility>
 
 #include "context.h"
@@ -350,34 +351,152 @@ hipError_t launch_replay_iterations(const ViewArgs &va, const StreamQueues &q, c
   return hipSuccess;
 }
 
-// Captures kIterBatch iterations of the bounce loop as a graph, unless the
-// graph already held was captured with the same launch arguments.
-int ensure_graph(phx_context *c, const ViewArgs &va, const StreamQueues &q, const LoopShape &ls, const Fork *fork) {
+using Lane = phx_context::Lane;
+
+// Captures kIterBatch iterations of the bounce loop on lane `ln` as a graph,
+// unless the graph it holds was captured with the same launch arguments.
+int ensure_graph(phx_context *c, Lane &ln, const ViewArgs &va, const StreamQueues &q, const LoopShape &ls,
+                 const Fork *fork) {
   std::vector<unsigned char> key(sizeof va + sizeof q + sizeof ls + 1);
   std::memcpy(key.data(), &va, sizeof va);
   std::memcpy(key.data() + sizeof va, &q, sizeof q);
   std::memcpy(key.data() + sizeof va + sizeof q, &ls, sizeof ls);
   key.back() = fork ? 1 : 0;
-  if (c->graph && key == c->graph_key) return PHX_OK;
-  if (c->graph) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
+  if (ln.graph && key == ln.graph_key) return PHX_OK;
+  if (ln.graph) {
+    (void)hipGraphExecDestroy(ln.graph);
+    ln.graph = nullptr;
   }
-  HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const hipError_t le = launch_iterations(va, q, ls, kIterBatch, nullptr, c->stream, fork);
+  HIP_TRY(c, hipStreamBeginCapture(ln.st, hipStreamCaptureModeThreadLocal));
+  const hipError_t le = launch_iterations(va, q, ls, kIterBatch, nullptr, ln.st, fork);
   hipGraph_t g = nullptr;
-  const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+  const hipError_t ce = hipStreamEndCapture(ln.st, &g);
   if (le != hipSuccess || ce != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
     return hip_fail(c, le != hipSuccess ? le : ce, "capturing the bounce loop");
   }
-  const hipError_t ie = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  const hipError_t ie = hipGraphInstantiate(&ln.graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (ie != hipSuccess) {
-    c->graph = nullptr;
+    ln.graph = nullptr;
     return hip_fail(c, ie, "hipGraphInstantiate");
   }
-  c->graph_key = std::move(key);
+  ln.graph_key = std::move(key);
+  return PHX_OK;
+}
+
+// Creates lane `l`'s streams (lane 1 only: lane 0 uses the context's) and
+// events on first use.
+int ensure_lane(phx_context *c, int l) {
+  Lane &ln = c->lanes[l];
+  if (!ln.st) HIP_TRY(c, hipStreamCreate(&ln.st));
+  if (!ln.st2) HIP_TRY(c, hipStreamCreate(&ln.st2));
+  for (hipEvent_t *e : {&ln.ev_fork, &ln.ev_join, &ln.ev_read, &ln.ev_reduced})
+    if (!*e) HIP_TRY(c, hipEventCreateWithFlags(e, hipEventDisableTiming));
+  return PHX_OK;
+}
+
+// Sizes lane `ln` for `cap` path slots, `samples` per-sample radiance records
+// and item tables of `pad` entries (none for pad 1).
+int reserve_lane(phx_context *c, Lane &ln, uint32_t cap, uint32_t samples, bool volumes, size_t pad) {
+  if (cap > ln.q_capacity) {
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(c, ln.q_o[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, ln.q_d[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, ln.q_tp[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, ln.q_st[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, ln.sh_o[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, ln.sh_d[b].reserve(size_t(cap) * 16));
+      HIP_TRY(c, ln.sh_c[b].reserve(size_t(cap) * 16));
+    }
+    HIP_TRY(c, ln.q_hit.reserve(size_t(cap) * 16));
+    ln.q_capacity = cap;
+  }
+  HIP_TRY(c, ln.sample_L.reserve(size_t(samples) * 16));
+  HIP_TRY(c, ln.counters.reserve(kCounterWords * 4));
+  HIP_TRY(c, ln.stats_buf.reserve(kStatWords * 8));
+  if (volumes) {
+    HIP_TRY(c, ln.vol_ev.reserve(size_t(cap) * 16));
+    HIP_TRY(c, ln.vol_rng.reserve(size_t(cap) * 4));
+  }
+  if (pad > 1) {
+    HIP_TRY(c, ln.bt_items.reserve(pad * sizeof(RunItem)));
+    HIP_TRY(c, ln.bt_first.reserve(pad * sizeof(uint32_t)));
+    if (pad > ln.h_count) {  // no copy from the old staging is pending: the lane's stream is idle
+      if (ln.h_items) (void)hipHostFree(ln.h_items);
+      if (ln.h_first) (void)hipHostFree(ln.h_first);
+      ln.h_items = nullptr;
+      ln.h_first = nullptr;
+      ln.h_count = 0;
+      HIP_TRY(c, hipHostMalloc(reinterpret_cast<void **>(&ln.h_items), pad * sizeof(RunItem), hipHostMallocDefault));
+      HIP_TRY(c, hipHostMalloc(reinterpret_cast<void **>(&ln.h_first), pad * sizeof(uint32_t), hipHostMallocDefault));
+      ln.h_count = pad;
+    }
+  }
+  return PHX_OK;
+}
+
+// The queue pointers of lane `ln`; its two traversal passes spill at
+// spill[0..) and spill[trace_threads * (kMaxStack - kShortStack)..).
+StreamQueues lane_queues(const Lane &ln, uint32_t *aov, int *spill, size_t trace_threads, bool volumes,
+                         uint32_t *pix_active) {
+  StreamQueues q{};
+  for (int b = 0; b < 2; ++b) {
+    q.o[b] = ln.q_o[b].as<float4>();
+    q.d[b] = ln.q_d[b].as<float4>();
+    q.tp[b] = ln.q_tp[b].as<float4>();
+    q.st[b] = ln.q_st[b].as<uint4>();
+    q.sh_o[b] = ln.sh_o[b].as<float4>();
+    q.sh_d[b] = ln.sh_d[b].as<float4>();
+    q.sh_c[b] = ln.sh_c[b].as<float4>();
+  }
+  q.hit = ln.q_hit.as<float4>();
+  q.sample_L = ln.sample_L.as<float4>();
+  q.aov_s = aov;
+  q.counters = ln.counters.as<uint32_t>();
+  q.stats = ln.stats_buf.as<unsigned long long>();
+  q.spill = spill;
+  q.spill_shadow = spill + trace_threads * (kMaxStack - kShortStack);
+  q.vol_ev = volumes ? ln.vol_ev.as<float4>() : nullptr;
+  q.vol_rng = volumes ? ln.vol_rng.as<uint32_t>() : nullptr;
+  q.pix_active = pix_active;
+  return q;
+}
+
+// The launch arguments of `run` on lane `ln`: scene and item in kernel
+// arguments for a run of one item, else the lane's item tables, uploaded on
+// its stream after the kernels of its previous run, which read them.  first[]
+// is padded to `pad` (a power of two) with ~0u so the per-lane binary search
+// has a fixed length.  The staging is rewritten only once the lane's previous
+// run has ended on the host, which follows its upload on the stream.
+int view_args(phx_context *c, Lane &ln, const Run &run, const std::vector<const phx_context *> &scenes, size_t pad,
+              ViewArgs &va) {
+  std::memset(&va, 0, sizeof va);
+  if (run.items.size() == 1) {
+    va.kind = 0;
+    va.one.scene = scenes[run.items[0].r.scene]->dev;
+    va.one.item = run.items[0].r;
+    return PHX_OK;
+  }
+  for (size_t k = 0; k < pad; ++k) {
+    ln.h_items[k] = k < run.items.size() ? run.items[k].r : run.items.back().r;
+    ln.h_first[k] = k < run.items.size() ? run.items[k].r.first : 0xffffffffu;
+  }
+  HIP_TRY(c, hipMemcpyAsync(ln.bt_items.p, ln.h_items, pad * sizeof(RunItem), hipMemcpyHostToDevice, ln.st));
+  HIP_TRY(c, hipMemcpyAsync(ln.bt_first.p, ln.h_first, pad * sizeof(uint32_t), hipMemcpyHostToDevice, ln.st));
+  if (scenes.size() == 1) {
+    va.kind = 1;
+    va.shared.scene = scenes[0]->dev;
+    va.shared.items = ln.bt_items.as<RunItem>();
+    va.shared.first = ln.bt_first.as<uint32_t>();
+    va.shared.count = uint32_t(pad);
+  } else {
+    va.kind = 2;
+    va.many.scenes = c->bt_scenes.as<DevScene>();
+    va.many.items = ln.bt_items.as<RunItem>();
+    va.many.first = ln.bt_first.as<uint32_t>();
+    va.many.count = uint32_t(pad);
+  }
   return PHX_OK;
 }
 
@@ -399,25 +518,6 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   }
   const uint32_t want = c->tuning.paths_in_flight ? c->tuning.paths_in_flight : kDefaultPaths;
   const uint32_t cap = std::max<uint32_t>(1, std::min<uint32_t>(want, max_total));
-  if (cap > c->q_capacity) {
-    for (int b = 0; b < 2; ++b) {
-      HIP_TRY(c, c->q_o[b].reserve(size_t(cap) * 16));
-      HIP_TRY(c, c->q_d[b].reserve(size_t(cap) * 16));
-      HIP_TRY(c, c->q_tp[b].reserve(size_t(cap) * 16));
-      HIP_TRY(c, c->q_st[b].reserve(size_t(cap) * 16));
-    }
-    HIP_TRY(c, c->q_hit.reserve(size_t(cap) * 16));
-    for (int b = 0; b < 2; ++b) {
-      HIP_TRY(c, c->sh_o[b].reserve(size_t(cap) * 16));
-      HIP_TRY(c, c->sh_d[b].reserve(size_t(cap) * 16));
-      HIP_TRY(c, c->sh_c[b].reserve(size_t(cap) * 16));
-    }
-    c->q_capacity = cap;
-  }
-  HIP_TRY(c, c->sample_L.reserve(size_t(max_total) * 16));
-  HIP_TRY(c, c->counters.reserve(kCounterWords * 4));
-  HIP_TRY(c, c->stats_buf.reserve(kStatWords * 8));
-  if (!c->pinned) HIP_TRY(c, hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 64, hipHostMallocDefault));
   // The launch shape of each run: the refill threshold from the depth of its
   // scenes' trees, the shading tier and the volume pass from what its scenes
   // and views hold (a batch's small views do not slow its solo views).
@@ -441,57 +541,7 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   };
   bool any_volumes = false;
   for (const phx_context *sc : scenes) any_volumes |= sc->dev.vol_count > 0;
-  if (any_volumes) {
-    HIP_TRY(c, c->vol_ev.reserve(size_t(cap) * 16));
-    HIP_TRY(c, c->vol_rng.reserve(size_t(cap) * 4));
-  }
-  // Two spill areas, one per traversal pass: k_shadow runs beside k_extend
-  // on the second stream, and a thread of each may spill at the same index.
-  const size_t trace_threads = size_t(ls0.trace_grid) * ls0.trace_block;
-  int r = ensure_trace_scratch(c, 2 * trace_threads);
-  if (r) return r;
 
-  // Item tables of the runs that have several items: first[] is padded to a
-  // power of two with ~0u so the per-lane binary search has a fixed length.
-  size_t pad = 1;
-  while (pad < max_items) pad <<= 1;
-  const bool one_scene = scenes.size() == 1;
-  if (max_items > 1) {
-    HIP_TRY(c, c->bt_items.reserve(pad * sizeof(RunItem)));
-    HIP_TRY(c, c->bt_first.reserve(pad * sizeof(uint32_t)));
-    if (!one_scene) {
-      std::vector<DevScene> table(scenes.size());
-      for (size_t i = 0; i < scenes.size(); ++i) table[i] = scenes[i]->dev;
-      HIP_TRY(c, c->bt_scenes.reserve(table.size() * sizeof(DevScene)));
-      HIP_TRY(c, hipStreamSynchronize(c->stream));
-      HIP_TRY(c, hipMemcpy(c->bt_scenes.p, table.data(), table.size() * sizeof(DevScene), hipMemcpyHostToDevice));
-    }
-  }
-
-  StreamQueues q{};
-  for (int b = 0; b < 2; ++b) {
-    q.o[b] = c->q_o[b].as<float4>();
-    q.d[b] = c->q_d[b].as<float4>();
-    q.tp[b] = c->q_tp[b].as<float4>();
-    q.st[b] = c->q_st[b].as<uint4>();
-  }
-  q.hit = c->q_hit.as<float4>();
-  for (int b = 0; b < 2; ++b) {
-    q.sh_o[b] = c->sh_o[b].as<float4>();
-    q.sh_d[b] = c->sh_d[b].as<float4>();
-    q.sh_c[b] = c->sh_c[b].as<float4>();
-  }
-  q.sample_L = c->sample_L.as<float4>();
-  q.aov_s = aov;
-  q.counters = c->counters.as<uint32_t>();
-  q.stats = c->stats_buf.as<unsigned long long>();
-  q.spill = c->spill.as<int>();
-  q.spill_shadow = q.spill + trace_threads * (kMaxStack - kShortStack);
-  q.vol_ev = any_volumes ? c->vol_ev.as<float4>() : nullptr;
-  q.vol_rng = any_volumes ? c->vol_rng.as<uint32_t>() : nullptr;
-  q.pix_active = adaptive ? c->f_active.as<uint32_t>() : nullptr;
-
-  hipStream_t st = c->stream;
   // PHX_TRACE_COUNTS=N: read the queue length every N iterations and print it.
   const char *tc = std::getenv("PHX_TRACE_COUNTS");
   const bool trace_counts = tc && std::atoi(tc) > 0;
@@ -500,103 +550,225 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
   // PHX_SERIAL_BOUNCE=1 (diagnostic): every launch of the bounce loop on one
   // stream, the any-hit pass not beside the closest-hit pass.
   const char *sb = std::getenv("PHX_SERIAL_BOUNCE");
-  const Fork fork_spec{c->stream2, c->ev_fork, c->ev_join};
-  const Fork *fork = sb && std::atoi(sb) > 0 ? nullptr : &fork_spec;
+  const bool serial = sb && std::atoi(sb) > 0;
+  // Two lanes for a batch of several runs (PHX_OVERLAP=0, diagnostic: one).
+  const char *ov = std::getenv("PHX_OVERLAP");
+  const int nlanes = !adj && !adaptive && !trace_counts && !tm.on && runs.size() > 1 && !(ov && std::atoi(ov) == 0)
+                         ? 2 : 1;
 
-  HIP_TRY(c, launch_clear_u32(reinterpret_cast<uint32_t *>(q.stats), kStatWords * 2, st));
+  size_t pad = 1;
+  while (pad < max_items) pad <<= 1;
+  // Two spill areas per lane, one per traversal pass: k_shadow runs beside
+  // k_extend on the lane's second stream, and a thread of each may spill at
+  // the same index.
+  const size_t trace_threads = size_t(ls0.trace_grid) * ls0.trace_block;
+  int r = ensure_trace_scratch(c, 2 * nlanes * trace_threads);
+  if (r) return r;
+  if (!c->pinned) HIP_TRY(c, hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 64, hipHostMallocDefault));
+  StreamQueues qs[2]{};
+  Fork forks[2]{};
+  for (int l = 0; l < nlanes; ++l) {
+    if ((r = ensure_lane(c, l))) return r;
+    Lane &ln = c->lanes[l];
+    if ((r = reserve_lane(c, ln, cap, max_total, any_volumes, max_items > 1 ? pad : 1))) return r;
+    qs[l] = lane_queues(ln, aov, c->spill.as<int>() + 2 * l * trace_threads * (kMaxStack - kShortStack),
+                        trace_threads, any_volumes, adaptive ? c->f_active.as<uint32_t>() : nullptr);
+    forks[l] = Fork{ln.st2, ln.ev_fork, ln.ev_join};
+  }
+  if (max_items > 1 && scenes.size() > 1) {
+    std::vector<DevScene> table(scenes.size());
+    for (size_t i = 0; i < scenes.size(); ++i) table[i] = scenes[i]->dev;
+    HIP_TRY(c, c->bt_scenes.reserve(table.size() * sizeof(DevScene)));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(c->bt_scenes.p, table.data(), table.size() * sizeof(DevScene), hipMemcpyHostToDevice));
+  }
+
+  hipStream_t st = c->stream;
+  const StreamQueues &q = qs[0];
   HIP_TRY(c, launch_aov_init(total_pixels, aov, st));
   if (adaptive) HIP_TRY(c, launch_adaptive_init(total_pixels, c->f_active.as<uint32_t>(), st));
-  std::vector<RunItem> items(pad);
-  std::vector<uint32_t> first(pad);
-  for (const Run &run : runs) {
-    if (run.items.empty()) continue;
-    const LoopShape ls = run_shape(run);
-    ViewArgs va;
-    std::memset(&va, 0, sizeof va);
-    if (run.items.size() == 1) {
-      va.kind = 0;
-      va.one.scene = scenes[run.items[0].r.scene]->dev;
-      va.one.item = run.items[0].r;
-    } else {
-      for (size_t k = 0; k < pad; ++k) {
-        items[k] = k < run.items.size() ? run.items[k].r : run.items.back().r;
-        first[k] = k < run.items.size() ? run.items[k].r.first : 0xffffffffu;
-      }
-      // the previous run's kernels must be done reading the tables
-      HIP_TRY(c, hipStreamSynchronize(st));
-      HIP_TRY(c, hipMemcpy(c->bt_items.p, items.data(), pad * sizeof(RunItem), hipMemcpyHostToDevice));
-      HIP_TRY(c, hipMemcpy(c->bt_first.p, first.data(), pad * sizeof(uint32_t), hipMemcpyHostToDevice));
-      if (one_scene) {
-        va.kind = 1;
-        va.shared.scene = scenes[0]->dev;
-        va.shared.items = c->bt_items.as<RunItem>();
-        va.shared.first = c->bt_first.as<uint32_t>();
-        va.shared.count = uint32_t(pad);
-      } else {
-        va.kind = 2;
-        va.many.scenes = c->bt_scenes.as<DevScene>();
-        va.many.items = c->bt_items.as<RunItem>();
-        va.many.first = c->bt_first.as<uint32_t>();
-        va.many.count = uint32_t(pad);
-      }
+  for (int l = 0; l < nlanes; ++l) {
+    HIP_TRY(c, launch_clear_u32(reinterpret_cast<uint32_t *>(qs[l].stats), kStatWords * 2, st));
+    if (l > 0) {  // lane 1 starts after the set-up above and the caller's work on `st`
+      HIP_TRY(c, hipEventRecord(c->lanes[0].ev_reduced, st));
+      HIP_TRY(c, hipStreamWaitEvent(c->lanes[l].st, c->lanes[0].ev_reduced, 0));
     }
-    const bool graph = use_graph && !adj && va.kind != 0 && !trace_counts && !tm.on;
-    if (graph && (r = ensure_graph(c, va, q, ls, fork))) return r;
-    for (int pass = 0; pass < (adj ? 2 : 1); ++pass) {
-      const bool replay = pass == 1;
-      HIP_TRY(c, launch_run_start(q, run.total, std::min(cap, run.total), st));
-      HIP_TRY(c, launch_regen(va, q, 0, ls, st, !replay));
-      uint32_t it = 0;
-      LoopShape lsb = ls;   // the launch shape of the next batch of iterations
-      uint32_t nb = batch;  // iterations of the next batch (launched one by one)
-      while (true) {
-        if (replay) {
-          HIP_TRY(c, launch_replay_iterations(va, q, *adj, ls, batch, st));
-          it += batch;
-        } else if (graph) {
-          HIP_TRY(c, hipGraphLaunch(c->graph, st));
-          it += kIterBatch;
-        } else {
-          HIP_TRY(c, launch_iterations(va, q, lsb, nb, tm.on ? &tm : nullptr, st, fork));
-          it += nb;
-        }
-        // the iteration counter, the samples started, then that iteration's queue
-        HIP_TRY(c, hipMemcpyAsync(&c->pinned[0], q.counters + kIter, 4, hipMemcpyDeviceToHost, st));
-        HIP_TRY(c, hipMemcpyAsync(&c->pinned[1], q.counters + kNextSample, 4, hipMemcpyDeviceToHost, st));
-        HIP_TRY(c, hipMemcpyAsync(&c->pinned[2], q.counters + ray_count_word(it), 4, hipMemcpyDeviceToHost, st));
-        HIP_TRY(c, hipStreamSynchronize(st));
-        if (c->pinned[0] != it)
-          return fail(c, PHX_ERR_OVERFLOW, "render: wavefront iteration counter out of step");
-        if (trace_counts)
-          std::fprintf(stderr, "[phx] %s iteration %u: queue %u, started %u of %u samples\n",
-                       replay ? "replay" : "render", it, c->pinned[2], c->pinned[1], run.total);
-        if (c->pinned[2] == 0 && c->pinned[1] >= run.total) break;
-        // Once every sample has started, no path is added: the queues of the
-        // iterations to come hold at most the one just read, and the shading
-        // grid (one thread per queue entry) shrinks to it.  A grid over all
-        // path slots costs about 28 us to launch even over an empty queue.
-        // The drain that follows takes a few iterations, each bound by its
-        // slowest rays; batches of half the size launch fewer iterations over
-        // an empty queue at its end (each costs ~60 us) for one more read.
-        if (c->pinned[1] >= run.total) {
-          lsb.shade_grid = std::max<uint32_t>(1, std::min(lsb.shade_grid, (c->pinned[2] + 255u) / 256u));
-          nb = std::max<uint32_t>(1, batch / 2);
-        }
-        if (it >= kIterationGuard)
-          return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");
-      }
-      if (!replay) HIP_TRY(c, trace_pending_shadow(va, q, ls, tm.on ? &tm : nullptr, st));
-    }
-    if (adj) continue;
+  }
+  auto reduce_run = [&](const Run &run, const StreamQueues &ql, hipStream_t s) -> int {
     for (const HostItem &h : run.items) {
       if (adaptive)
-        HIP_TRY(c, launch_reduce_adaptive(h.r.sp.npix, h.nspp, q.sample_L + h.r.first, fb.accum + h.r.pix_base,
+        HIP_TRY(c, launch_reduce_adaptive(h.r.sp.npix, h.nspp, ql.sample_L + h.r.first, fb.accum + h.r.pix_base,
                                           c->f_accum_sq.as<float4>() + h.r.pix_base, fb.nsamp + h.r.pix_base,
-                                          c->f_active.as<uint32_t>() + h.r.pix_base, h.r.s0, st));
+                                          c->f_active.as<uint32_t>() + h.r.pix_base, h.r.s0, s));
       else
-        HIP_TRY(c, launch_reduce(h.r.sp.npix, h.nspp, q.sample_L + h.r.first, fb.accum + h.r.pix_base, h.r.s0 == 0,
-                                 st));
+        HIP_TRY(c, launch_reduce(h.r.sp.npix, h.nspp, ql.sample_L + h.r.first, fb.accum + h.r.pix_base, h.r.s0 == 0,
+                                 s));
     }
+    return PHX_OK;
+  };
+
+  if (nlanes == 1) {
+    Lane &ln = c->lanes[0];
+    const Fork *fork = serial ? nullptr : &forks[0];
+    for (const Run &run : runs) {
+      if (run.items.empty()) continue;
+      const LoopShape ls = run_shape(run);
+      ViewArgs va;
+      if ((r = view_args(c, ln, run, scenes, pad, va))) return r;
+      const bool graph = use_graph && !adj && va.kind != 0 && !trace_counts && !tm.on;
+      if (graph && (r = ensure_graph(c, ln, va, q, ls, fork))) return r;
+      for (int pass = 0; pass < (adj ? 2 : 1); ++pass) {
+        const bool replay = pass == 1;
+        HIP_TRY(c, launch_run_start(q, run.total, std::min(cap, run.total), st));
+        HIP_TRY(c, launch_regen(va, q, 0, ls, st, !replay));
+        uint32_t it = 0;
+        LoopShape lsb = ls;   // the launch shape of the next batch of iterations
+        uint32_t nb = batch;  // iterations of the next batch (launched one by one)
+        while (true) {
+          if (replay) {
+            HIP_TRY(c, launch_replay_iterations(va, q, *adj, ls, batch, st));
+            it += batch;
+          } else if (graph) {
+            HIP_TRY(c, hipGraphLaunch(ln.graph, st));
+            it += kIterBatch;
+          } else {
+            HIP_TRY(c, launch_iterations(va, q, lsb, nb, tm.on ? &tm : nullptr, st, fork));
+            it += nb;
+          }
+          // the iteration counter, the samples started, then that iteration's queue
+          HIP_TRY(c, hipMemcpyAsync(&c->pinned[0], q.counters + kIter, 4, hipMemcpyDeviceToHost, st));
+          HIP_TRY(c, hipMemcpyAsync(&c->pinned[1], q.counters + kNextSample, 4, hipMemcpyDeviceToHost, st));
+          HIP_TRY(c, hipMemcpyAsync(&c->pinned[2], q.counters + ray_count_word(it), 4, hipMemcpyDeviceToHost, st));
+          HIP_TRY(c, hipStreamSynchronize(st));
+          if (c->pinned[0] != it)
+            return fail(c, PHX_ERR_OVERFLOW, "render: wavefront iteration counter out of step");
+          if (trace_counts)
+            std::fprintf(stderr, "[phx] %s iteration %u: queue %u, started %u of %u samples\n",
+                         replay ? "replay" : "render", it, c->pinned[2], c->pinned[1], run.total);
+          if (c->pinned[2] == 0 && c->pinned[1] >= run.total) break;
+          // Once every sample has started, no path is added: the queues of the
+          // iterations to come hold at most the one just read, and the shading
+          // grid (one thread per queue entry) shrinks to it.  A grid over all
+          // path slots costs about 28 us to launch even over an empty queue.
+          // The drain that follows takes a few iterations, each bound by its
+          // slowest rays; batches of half the size launch fewer iterations over
+          // an empty queue at its end (each costs ~60 us) for one more read.
+          if (c->pinned[1] >= run.total) {
+            lsb.shade_grid = std::max<uint32_t>(1, std::min(lsb.shade_grid, (c->pinned[2] + 255u) / 256u));
+            nb = std::max<uint32_t>(1, batch / 2);
+          }
+          if (it >= kIterationGuard)
+            return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");
+        }
+        if (!replay) HIP_TRY(c, trace_pending_shadow(va, q, ls, tm.on ? &tm : nullptr, st));
+      }
+      if (!adj && (r = reduce_run(run, q, st))) return r;
+    }
+  } else {
+    // Overlapped runs.  Once every sample of a run has started, its drain
+    // (a few iterations bound by their slowest rays) leaves most of the GPU
+    // idle, and the next run starts on the other lane beside it.  Run k
+    // streams on lane k % 2: it starts once run k - 2 has been reduced and run
+    // k - 1 drains.  Reduces are issued in run order, each after the previous
+    // run's on the other lane, so a view split over several runs adds its
+    // samples in the order one lane would; every view's result is the same
+    // as with one lane.  The host polls each lane's counter readback.
+    struct LaneRun {
+      long run = -1;  // the run the lane streams, -1 idle
+      ViewArgs va;
+      LoopShape ls, lsb;
+      bool graph = false, draining = false, finished = false;
+      uint32_t it = 0, nb = 0;
+    };
+    LaneRun lr[2];
+    auto launch_batch = [&](int l) -> int {
+      LaneRun &s = lr[l];
+      Lane &ln = c->lanes[l];
+      if (s.graph) {
+        HIP_TRY(c, hipGraphLaunch(ln.graph, ln.st));
+        s.it += kIterBatch;
+      } else {
+        HIP_TRY(c, launch_iterations(s.va, qs[l], s.lsb, s.nb, nullptr, ln.st, serial ? nullptr : &forks[l]));
+        s.it += s.nb;
+      }
+      uint32_t *pw = c->pinned + 4 * l;
+      HIP_TRY(c, hipMemcpyAsync(&pw[0], qs[l].counters + kIter, 4, hipMemcpyDeviceToHost, ln.st));
+      HIP_TRY(c, hipMemcpyAsync(&pw[1], qs[l].counters + kNextSample, 4, hipMemcpyDeviceToHost, ln.st));
+      HIP_TRY(c, hipMemcpyAsync(&pw[2], qs[l].counters + ray_count_word(s.it), 4, hipMemcpyDeviceToHost, ln.st));
+      HIP_TRY(c, hipEventRecord(ln.ev_read, ln.st));
+      return PHX_OK;
+    };
+    size_t next = 0, reduced = 0;  // the next run to start; runs reduced
+    while (reduced < runs.size()) {
+      bool progress = false;
+      const int ls_next = int(next % 2);
+      if (next < runs.size() && lr[ls_next].run < 0) {
+        const LaneRun &o = lr[1 - ls_next];
+        if (o.run < 0 || o.draining || o.finished) {
+          LaneRun &s = lr[ls_next];
+          Lane &ln = c->lanes[ls_next];
+          const Run &run = runs[next];
+          s = LaneRun{};
+          s.run = long(next++);
+          progress = true;
+          if (!run.items.empty()) {
+            s.ls = s.lsb = run_shape(run);
+            s.nb = batch;
+            if ((r = view_args(c, ln, run, scenes, pad, s.va))) return r;
+            s.graph = use_graph && s.va.kind != 0;
+            if (s.graph && (r = ensure_graph(c, ln, s.va, qs[ls_next], s.ls, serial ? nullptr : &forks[ls_next])))
+              return r;
+            HIP_TRY(c, launch_run_start(qs[ls_next], run.total, std::min(cap, run.total), ln.st));
+            HIP_TRY(c, launch_regen(s.va, qs[ls_next], 0, s.ls, ln.st, true));
+            if ((r = launch_batch(ls_next))) return r;
+          } else {
+            s.finished = true;
+          }
+        }
+      }
+      for (int l = 0; l < 2; ++l) {
+        LaneRun &s = lr[l];
+        if (s.run < 0 || s.finished) continue;
+        Lane &ln = c->lanes[l];
+        const hipError_t qe = hipEventQuery(ln.ev_read);
+        if (qe == hipErrorNotReady) continue;
+        HIP_TRY(c, qe);
+        progress = true;
+        const uint32_t *pw = c->pinned + 4 * l;
+        const Run &run = runs[size_t(s.run)];
+        if (pw[0] != s.it) return fail(c, PHX_ERR_OVERFLOW, "render: wavefront iteration counter out of step");
+        if (pw[2] == 0 && pw[1] >= run.total) {
+          HIP_TRY(c, trace_pending_shadow(s.va, qs[l], s.ls, nullptr, ln.st));
+          s.finished = true;
+          continue;
+        }
+        if (pw[1] >= run.total) {  // the drain, as with one lane
+          s.draining = true;
+          s.graph = false;
+          s.lsb.shade_grid = std::max<uint32_t>(1, std::min(s.lsb.shade_grid, (pw[2] + 255u) / 256u));
+          s.nb = std::max<uint32_t>(1, batch / 2);
+        }
+        if (s.it >= kIterationGuard)
+          return fail(c, PHX_ERR_OVERFLOW, "render: wavefront did not drain within its iteration guard");
+        if ((r = launch_batch(l))) return r;
+      }
+      while (reduced < runs.size()) {
+        const int l = int(reduced % 2);
+        LaneRun &s = lr[l];
+        if (s.run != long(reduced) || !s.finished) break;
+        Lane &ln = c->lanes[l];
+        if (reduced > 0) HIP_TRY(c, hipStreamWaitEvent(ln.st, c->lanes[1 - l].ev_reduced, 0));
+        if ((r = reduce_run(runs[reduced], qs[l], ln.st))) return r;
+        HIP_TRY(c, hipEventRecord(ln.ev_reduced, ln.st));
+        s = LaneRun{};
+        ++reduced;
+        progress = true;
+      }
+      if (!progress) std::this_thread::yield();
+    }
+    // the resolve and the statistics follow the last run's reduces
+    const int last = int((runs.size() - 1) % 2);
+    if (last != 0) HIP_TRY(c, hipStreamWaitEvent(st, c->lanes[last].ev_reduced, 0));
   }
   for (const ViewJob &j : views) {
     if (adj) break;
@@ -608,12 +780,16 @@ int stream_views(phx_context *c, const std::vector<ViewJob> &views, const std::v
     HIP_TRY(c, launch_resolve(j.scene->dev, j.sp, qv, fb.at(j.pix_base), j.spp, c->fetch.as<uint32_t>(),
                               c->trace_grid, st, hits));
   }
-  unsigned long long hs[kStatWords];
-  HIP_TRY(c, hipMemcpyAsync(hs, q.stats, sizeof hs, hipMemcpyDeviceToHost, st));
+  unsigned long long hs[2][kStatWords];
+  for (int l = 0; l < nlanes; ++l)
+    HIP_TRY(c, hipMemcpyAsync(hs[l], qs[l].stats, sizeof hs[l], hipMemcpyDeviceToHost, st));
   HIP_TRY(c, hipStreamSynchronize(st));
-  c->stats.closest_rays = hs[kStatClosest];
-  c->stats.shadow_rays = hs[kStatShadow];
-  c->stats.shade_events = hs[kStatShade];
+  c->stats.closest_rays = c->stats.shadow_rays = c->stats.shade_events = 0;
+  for (int l = 0; l < nlanes; ++l) {
+    c->stats.closest_rays += hs[l][kStatClosest];
+    c->stats.shadow_rays += hs[l][kStatShadow];
+    c->stats.shade_events += hs[l][kStatShade];
+  }
   uint64_t samples = 0;
   for (const ViewJob &j : views) samples += uint64_t(j.sp.npix) * j.spp;
   c->stats.samples = samples;

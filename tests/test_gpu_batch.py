@@ -79,15 +79,22 @@ def test_batch_equals_plain_renders(batch_setup, use_graph):
     assert st.samples == sum(p.width * p.height * p.samples_per_pixel for _, _, p in views)
 
 
-def test_batch_split_into_runs(batch_setup, monkeypatch):
-    # a small run budget splits views across runs and spp ranges
+@pytest.mark.parametrize("overlap", ["1", "0"])
+def test_batch_split_into_runs(batch_setup, monkeypatch, overlap):
+    # a small run budget splits views across runs and spp ranges; the runs
+    # stream on two lanes, each starting beside the previous one's drain
+    # (PHX_OVERLAP=0: one lane), and reduce in run order
     from photon_amd.backend import BatchRenderer
     views, plain, make = batch_setup
     monkeypatch.setenv("PHX_SAMPLE_BUDGET", "3000")
+    monkeypatch.setenv("PHX_OVERLAP", overlap)
     br = BatchRenderer(make())
-    br.render(views)
-    for v in range(len(views)):
-        _check_equal(br.read(v), plain[v], f"split view {v}")
+    for rep in range(2):  # the second call reuses both lanes' buffers and graphs
+        br.render(views)
+        for v in range(len(views)):
+            _check_equal(br.read(v), plain[v], f"split overlap={overlap} rep={rep} view {v}")
+    st = br.ctx.stats()
+    assert st.samples == sum(p.width * p.height * p.samples_per_pixel for _, _, p in views)
     # the plain path splits the same way and still gives the same frame
     pt, cam, p = views[1]
     pt.camera, pt.params = cam, p
