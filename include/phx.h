@@ -8,7 +8,11 @@
  * the library starts no host threads.  All kernels of a context run on the
  * context's HIP stream, or on the stream passed to a call that takes one; the
  * bounce loop of a render also runs its any-hit pass on a second stream of the
- * context, joined to the first before each pass that adds its results.
+ * context, joined to the first before each pass that adds its results.  A
+ * render of several runs (a batch, or a view larger than one run) streams
+ * every other run on a second pair of streams of the context, created on
+ * first use; its work is joined to the context's stream before the call
+ * returns or resolves.
  *
  * Data types mirror the reference's scene model.  These have the reference
  * type's exact size, field order and field offsets (x86-64 and gfx950 agree),
