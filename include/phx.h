@@ -339,7 +339,7 @@ typedef struct phx_tuning {
   uint32_t refill_lanes;         /* idle lanes of a traversal wave that make it take new rays, 1..64
                                     (0: 12 when the run's deepest BVH has 6+ levels, else 40) */
   uint32_t queue_chunk;          /* rays a wave takes per queue fetch, 64..4096, multiple of 64 (128) */
-  uint32_t paths_in_flight;      /* path slots of the wavefront (2^25) */
+  uint32_t paths_in_flight;      /* path slots of the wavefront (2^26) */
   uint32_t sort_by_material;     /* material groups of sorted shading, 0..15 (0: queue order) */
   uint32_t use_graph;            /* 1: phx_render_batch replays its bounce loop as a captured hipGraph */
   uint32_t time_kernels;         /* 1: time each kernel launch with HIP events (phx_stats) */

@@ -64,7 +64,7 @@ constexpr uint32_t kIterBatch = 8;
 // iteration, so a run needs at most total + max_it iterations; this guard only
 // turns a logic error into an error code instead of a hang.
 constexpr uint32_t kIterationGuard = 1u << 26;
-constexpr uint32_t kDefaultPaths = 1u << 25;
+constexpr uint32_t kDefaultPaths = 1u << 26;
 constexpr uint32_t kDefaultRefill = 16;
 // Without a tuned refill threshold, a run's traversal refills at 12 idle lanes
 // when its deepest BVH has at least 6 levels and at 40 otherwise: idle lanes
